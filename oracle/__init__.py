@@ -1,0 +1,171 @@
+"""CPU oracle for the MatFlat hot path -- TEST INFRASTRUCTURE ONLY.
+
+This package is the checker for the HIP kernels.  Only ``tests/``,
+``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` may
+import it.  The product package (``matflat_amd``) never imports it and has no
+CPU fallback.
+
+``libmatflat_oracle.so`` is a C restatement (``matflat_oracle.c``) of:
+
+* ``Internals.Dot``           MatFlat/Internals.cs:265-292
+* ``Factorization.Lu``        MatFlat/Factorization.Lu.cs:296-350
+* ``Factorization.Cholesky``  MatFlat/Factorization.Cholesky.cs:114-168
+* ``Blas.SolveTriangular``    MatFlat/Blas.SolveTriangular.cs:162-242, :397-421
+* ``Blas.MulMatMat``          MatFlat/Blas.MulMatMat.cs:226-368 (double),
+                              :407-648 (Complex)
+* LU solve composition        MatFlat/Factorization.LuInverse.cs:169-192
+
+compiled with ``-O2 -ffp-contract=off -fno-fast-math``.  Pinning (what makes
+these functions trustworthy, beyond being written from the reference) lives in
+``tests/test_oracle.py``: agreement with OpenBLAS (scipy) at the reference
+tests' tolerances, factor residual ratios, permutation/triangularity
+invariants, closed forms on special matrices and brute-force permutation
+search on tiny inputs.
+
+Buffers are 1-D float64 numpy arrays holding a column-major matrix with
+leading dimension ``ld`` (element (i, j) at ``[i + j*ld]``), exactly like the
+reference tests' arrays (MatFlatTest/Matrix.cs:9-28).  Complex buffers are
+numpy complex128 (interleaved re, im).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libmatflat_oracle.so")
+
+NOTRANS, TRANS, CONJTRANS, CONJNOTRANS = 111, 112, 113, 114
+UPPER, LOWER = 121, 122
+
+_lib = None
+
+
+def build() -> None:
+    """Compile libmatflat_oracle.so with the oracle's Makefile."""
+    import subprocess
+
+    subprocess.run(["make", "-s", "-B", "-C", _HERE], check=True)
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(_LIB_PATH)
+        d, i, p = ctypes.c_double, ctypes.c_int, ctypes.c_void_p
+        i64 = ctypes.c_int64
+        _lib.mfo_ddot.restype = d
+        _lib.mfo_ddot.argtypes = [i, p, i, p, i]
+        _lib.mfo_dgetrf.restype = i
+        _lib.mfo_dgetrf.argtypes = [i, i, p, i, p]
+        _lib.mfo_dpotrf.restype = i
+        _lib.mfo_dpotrf.argtypes = [i, p, i]
+        _lib.mfo_dtrsv.restype = i
+        _lib.mfo_dtrsv.argtypes = [i, i, i, p, i, p, i]
+        _lib.mfo_dtrsv_unit_lower.restype = None
+        _lib.mfo_dtrsv_unit_lower.argtypes = [i, p, i, p, i]
+        _lib.mfo_dgetrs.restype = None
+        _lib.mfo_dgetrs.argtypes = [i, p, i, p, p, p]
+        _lib.mfo_dposv.restype = i
+        _lib.mfo_dposv.argtypes = [i, p, i, p]
+        _lib.mfo_dgemm.restype = i
+        _lib.mfo_dgemm.argtypes = [i, i, i, i, i, d, p, i, p, i, d, p, i]
+        _lib.mfo_zgemm.restype = i
+        _lib.mfo_zgemm.argtypes = [i, i, i, i, i, p, p, i, p, i, p, p, i]
+        _lib.mfo_dposv_batch.restype = None
+        _lib.mfo_dposv_batch.argtypes = [i, p, p, p, p, p, p, p, i]
+        _lib.mfo_dgetrf_batch.restype = None
+        _lib.mfo_dgetrf_batch.argtypes = [i, i, i, i, i64, p, p, p, i]
+        _lib.mfo_dposv_backward.restype = None
+        _lib.mfo_dposv_backward.argtypes = [i, p, i, p, p, p, i, p]
+    return _lib
+
+
+def _ptr(a: np.ndarray) -> int:
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data
+
+
+def ddot(n, x, incx, y, incy) -> float:
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    return lib().mfo_ddot(n, _ptr(x), incx, _ptr(y), incy)
+
+
+def dgetrf(m: int, n: int, a: np.ndarray, lda: int):
+    """In-place LU of the (lda*n,) buffer ``a``.  Returns (piv int32[m], sign)."""
+    assert a.dtype == np.float64 and a.size >= lda * (n - 1) + m
+    piv = np.zeros(m, dtype=np.int32)
+    sign = lib().mfo_dgetrf(m, n, _ptr(a), lda, _ptr(piv))
+    return piv, sign
+
+
+def dpotrf(n: int, a: np.ndarray, lda: int) -> int:
+    """In-place Cholesky; returns info (0 or j+1 of the failing row)."""
+    assert a.dtype == np.float64
+    return lib().mfo_dpotrf(n, _ptr(a), lda)
+
+
+def dtrsv(uplo: int, trans: int, n: int, a: np.ndarray, lda: int, x: np.ndarray, incx: int = 1) -> int:
+    return lib().mfo_dtrsv(uplo, trans, n, _ptr(a), lda, _ptr(x), incx)
+
+
+def dtrsv_unit_lower(n, a, lda, x, incx=1) -> None:
+    lib().mfo_dtrsv_unit_lower(n, _ptr(a), lda, _ptr(x), incx)
+
+
+def dgetrs(n, lu, lda, piv, b) -> np.ndarray:
+    x = np.zeros(n, dtype=np.float64)
+    piv = np.ascontiguousarray(piv, dtype=np.int32)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    lib().mfo_dgetrs(n, _ptr(lu), lda, _ptr(piv), _ptr(b), _ptr(x))
+    return x
+
+
+def dposv(n, a, lda, x) -> int:
+    return lib().mfo_dposv(n, _ptr(a), lda, _ptr(x))
+
+
+def dgemm(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc) -> int:
+    return lib().mfo_dgemm(transa, transb, m, n, k, float(alpha), _ptr(a), lda, _ptr(b), ldb,
+                           float(beta), _ptr(c), ldc)
+
+
+def zgemm(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc) -> int:
+    al = np.array([complex(alpha).real, complex(alpha).imag], dtype=np.float64)
+    be = np.array([complex(beta).real, complex(beta).imag], dtype=np.float64)
+    assert a.dtype == np.complex128 and b.dtype == np.complex128 and c.dtype == np.complex128
+    return lib().mfo_zgemm(transa, transb, m, n, k, _ptr(al), _ptr(a), lda, _ptr(b), ldb,
+                           _ptr(be), _ptr(c), ldc)
+
+
+def dposv_batch(n, lda, off, a, xoff, x, threads=1) -> np.ndarray:
+    n = np.ascontiguousarray(n, dtype=np.int32)
+    lda = np.ascontiguousarray(lda, dtype=np.int32)
+    off = np.ascontiguousarray(off, dtype=np.int64)
+    xoff = np.ascontiguousarray(xoff, dtype=np.int64)
+    info = np.zeros(n.size, dtype=np.int32)
+    lib().mfo_dposv_batch(n.size, _ptr(n), _ptr(lda), _ptr(off), _ptr(a), _ptr(xoff), _ptr(x),
+                          _ptr(info), threads)
+    return info
+
+
+def dgetrf_batch(count, m, n, lda, stride, a, threads=1):
+    piv = np.zeros(count * m, dtype=np.int32)
+    sign = np.zeros(count, dtype=np.int32)
+    lib().mfo_dgetrf_batch(count, m, n, lda, stride, _ptr(a), _ptr(piv), _ptr(sign), threads)
+    return piv, sign
+
+
+def dposv_backward(n, l, lda, x, w, ldab=None):
+    ldab = n if ldab is None else ldab
+    abar = np.zeros(ldab * n, dtype=np.float64)
+    bbar = np.zeros(n, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    lib().mfo_dposv_backward(n, _ptr(l), lda, _ptr(x), _ptr(w), _ptr(abar), ldab, _ptr(bbar))
+    return abar, bbar

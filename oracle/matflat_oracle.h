@@ -1,0 +1,87 @@
+/*
+ * matflat_oracle.h -- CPU restatement of MatFlat's hot-path algorithms.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP
+ * kernels in matflat_amd/csrc.  Only tests/, __graft_entry__.smoke() and
+ * the cpu_baseline leg of bench.py may load it; the product path never
+ * calls it and has no CPU fallback.
+ *
+ * Every function follows the operation order of the cited reference
+ * (MatFlat 0.8.2, C#) so that IEEE binary64 results are bit-identical when
+ * compiled with -O2 -ffp-contract=off -fno-fast-math.  Matrices are
+ * column-major, element (i,j) at a[i + j*lda].  Complex numbers are
+ * interleaved (re, im) doubles.
+ */
+#ifndef MATFLAT_ORACLE_H
+#define MATFLAT_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* CBLAS enum values (Transpose.cs:8-29, Uplo.cs:8-19). */
+enum { MFO_NOTRANS = 111, MFO_TRANS = 112, MFO_CONJTRANS = 113, MFO_CONJNOTRANS = 114 };
+enum { MFO_UPPER = 121, MFO_LOWER = 122 };
+
+/* Internals.Dot(int n, double* x, int incx, double* y, int incy), Internals.cs:265-292. */
+double mfo_ddot(int n, const double* x, int incx, const double* y, int incy);
+
+/* Factorization.Lu (LuCore double), Factorization.Lu.cs:296-350.  Returns the pivot sign. */
+int mfo_dgetrf(int m, int n, double* a, int lda, int* piv);
+
+/* Factorization.Cholesky double, Factorization.Cholesky.cs:114-168.
+ * Returns 0, or j+1 for the first row whose diagonal is not > 0 (the
+ * reference throws there; the matrix then holds the same partial writes). */
+int mfo_dpotrf(int n, double* a, int lda);
+
+/* Blas.SolveTriangular double, Blas.SolveTriangular.cs:162-242. */
+int mfo_dtrsv(int uplo, int trans, int n, const double* a, int lda, double* x, int incx);
+
+/* Unit-diagonal lower solve that LuInverse emulates by copying L with a unit
+ * diagonal (Factorization.LuInverse.cs:174-182): x_i = (x_i - dot)/1.0. */
+void mfo_dtrsv_unit_lower(int n, const double* a, int lda, double* x, int incx);
+
+/* LU solve composition of Factorization.LuInverse.cs:169-192 for one
+ * right-hand side: y[j] = b[piv[j]]; y <- L1^-1 y; x <- U^-1 y. */
+void mfo_dgetrs(int n, const double* lu, int lda, const int* piv, const double* b, double* x);
+
+/* Cholesky followed by Lower/NoTrans and Lower/Trans solves. Returns info. */
+int mfo_dposv(int n, double* a, int lda, double* x);
+
+/* Blas.MulMatMat double, Blas.MulMatMat.cs:226-368, extended by
+ * C = alpha*dot + beta*C0 (C0 not read when beta == 0).  With alpha=1,
+ * beta=0 this is exactly the reference. */
+int mfo_dgemm(int transa, int transb, int m, int n, int k, double alpha,
+              const double* a, int lda, const double* b, int ldb,
+              double beta, double* c, int ldc);
+
+/* Blas.MulMatMat Complex, Blas.MulMatMat.cs:407-575 with Dot_NN/CN/NC/CC
+ * (:598-648), same alpha/beta extension (complex alpha, beta). */
+int mfo_zgemm(int transa, int transb, int m, int n, int k, const double* alpha,
+              const double* a, int lda, const double* b, int ldb,
+              const double* beta, double* c, int ldc);
+
+/* ---- batched drivers (CPU baseline; OpenMP over matrices) ---------------- */
+
+/* Cholesky + two solves over a batch stored back-to-back: matrix i at
+ * a + off[i], order n[i], leading dimension lda[i]; rhs at x + xoff[i].
+ * Parallel over matrices with OpenMP dynamic scheduling when threads > 1. */
+void mfo_dposv_batch(int count, const int* n, const int* lda, const int64_t* off,
+                     double* a, const int64_t* xoff, double* x, int* info, int threads);
+
+/* LU over a fixed-size batch (stride elements apart). */
+void mfo_dgetrf_batch(int count, int m, int n, int lda, int64_t stride, double* a,
+                      int* piv, int* sign, int threads);
+
+/* Adjoint of x = A^-1 b with A = L L^T: given L (lower, as produced by
+ * mfo_dpotrf), x and w = dloss/dx, returns bbar = A^-1 w and
+ * Abar = -(bbar x^T + x bbar^T)/2 (full symmetric n x n, leading dim ldab). */
+void mfo_dposv_backward(int n, const double* l, int lda, const double* x, const double* w,
+                        double* abar, int ldab, double* bbar);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
