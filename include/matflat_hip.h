@@ -1,0 +1,197 @@
+/*
+ * matflat_hip.h -- C ABI of libmatflat_hip.so, the MI355X (gfx950) batched
+ * implementation of MatFlat's dense hot path.
+ *
+ * Conventions shared by every entry point:
+ *  - Matrices are column-major; element (i,j) of a matrix with leading
+ *    dimension ld sits at a[i + j*ld].  Rows [rows, ld) are never read or
+ *    written.
+ *  - "vbatched" entry points take device arrays of per-matrix sizes, leading
+ *    dimensions and pointers (count entries each).  Every pointer argument
+ *    (arrays and the matrices they point to) is DEVICE memory.  max_n is a
+ *    host-side upper bound on the sizes; a matrix that exceeds it, or that
+ *    has an invalid size or leading dimension, gets a negative info entry
+ *    (-(argument position), LAPACK convention) and is left untouched.
+ *  - Enum values are the CBLAS integers MatFlat uses (Transpose.cs:8-29,
+ *    Uplo.cs:8-19): NoTrans 111, Trans 112, ConjTrans 113, ConjNoTrans 114,
+ *    Upper 121, Lower 122.
+ *  - Every call is asynchronous on the handle's stream and returns a status:
+ *    MFH_OK, or a negative value naming the invalid host argument
+ *    (-position), or MFH_ERR_HIP / MFH_ERR_ALLOC / MFH_ERR_UNSUPPORTED.
+ *    mfh_last_error() describes the most recent failure on this thread.
+ *  - Numerical failures are per matrix in the device info array: 0 = ok,
+ *    j+1 = the factorization failed at row j (MatFlat throws
+ *    MatrixFactorizationException there, MatrixFactorizationException.cs:9-30).
+ *  - A handle is used by one host thread at a time.
+ *
+ * The FP64 LU, Cholesky, triangular-solve and GEMM kernels reproduce
+ * MatFlat's summation order exactly (Internals.Dot, Internals.cs:265-292),
+ * so their results are bit-identical to MatFlat's CPU results.
+ */
+#ifndef MATFLAT_HIP_H
+#define MATFLAT_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MFH_OK 0
+#define MFH_ERR_HIP (-1001)
+#define MFH_ERR_ALLOC (-1002)
+#define MFH_ERR_UNSUPPORTED (-1003)
+#define MFH_ERR_STATE (-1004)
+
+#define MFH_NOTRANS 111
+#define MFH_TRANS 112
+#define MFH_CONJTRANS 113
+#define MFH_CONJNOTRANS 114
+#define MFH_UPPER 121
+#define MFH_LOWER 122
+#define MFH_NONUNIT 131
+#define MFH_UNIT 132
+
+/* Largest order the exact (MatFlat-order) factorization kernels take. */
+#define MFH_EXACT_MAX_N 128
+/* Largest order of the blocked MFMA factorizations (mixed precision). */
+#define MFH_BLOCKED_MAX_N 512
+
+typedef struct mfh_handle mfh_handle;
+typedef struct mfh_plan mfh_plan;
+
+/* ---- handle ------------------------------------------------------------ */
+int mfh_create(mfh_handle** handle);
+int mfh_destroy(mfh_handle* handle);
+/* stream is a hipStream_t (0 = the null stream). */
+int mfh_set_stream(mfh_handle* handle, void* stream);
+int mfh_get_stream(mfh_handle* handle, void** stream);
+int mfh_synchronize(mfh_handle* handle);
+const char* mfh_last_error(void);
+/* Library version as major*10000 + minor*100 + patch. */
+int mfh_version(void);
+
+/* ---- LU with partial pivoting -------------------------------------------
+ * Replaces Factorization.Lu(int m, int n, double* a, int lda, int* piv)
+ * (Factorization.Lu.cs:122-161, LuCore :296-350) for a batch.  Computes
+ * P*A = L*U in place; piv[i] is the original row now at row i (0-based
+ * permutation of length m); sign[b] = +-1 is the returned pivot sign.
+ * Singular matrices are not an error (MatFlat never throws here).
+ * m, n <= MFH_EXACT_MAX_N. */
+int mfh_dgetrf_vbatched(mfh_handle* h, int count, const int* m, const int* n,
+                        double* const* a, const int* lda, int* const* piv,
+                        int* sign, int* info, int max_m, int max_n);
+/* Same for count matrices of one shape spaced stride_a doubles apart, with
+ * pivots spaced stride_piv ints apart. */
+int mfh_dgetrf_strided_batched(mfh_handle* h, int count, int m, int n, double* a, int lda,
+                               int64_t stride_a, int* piv, int64_t stride_piv,
+                               int* sign, int* info);
+
+/* ---- Cholesky ------------------------------------------------------------
+ * Replaces Factorization.Cholesky(int n, double* a, int lda)
+ * (Factorization.Cholesky.cs:114-168).  Reads the upper triangle only,
+ * writes L to the lower triangle and zeroes the strict upper triangle.  On
+ * failure at row j info = j+1 and the matrix holds exactly the partial
+ * writes MatFlat leaves when it throws.  n <= MFH_EXACT_MAX_N. */
+int mfh_dpotrf_vbatched(mfh_handle* h, int count, const int* n, double* const* a,
+                        const int* lda, int* info, int max_n);
+
+/* ---- triangular solve ----------------------------------------------------
+ * Replaces Blas.SolveTriangular(Uplo, Transpose, int n, double* a, int lda,
+ * double* x, int incx) (Blas.SolveTriangular.cs:162-242).  diag MFH_UNIT is
+ * the unit-lower/upper variant LuInverse emulates by copying the factor
+ * (Factorization.LuInverse.cs:174-182); MFH_NONUNIT is MatFlat's own.  One
+ * (uplo, trans, diag) for the whole batch; n <= MFH_EXACT_MAX_N. */
+int mfh_dtrsv_vbatched(mfh_handle* h, int uplo, int trans, int diag, int count,
+                       const int* n, const double* const* a, const int* lda,
+                       double* const* x, const int* incx, int* info, int max_n);
+
+/* ---- SPD solve (Cholesky + two triangular solves, fused) -----------------
+ * Per matrix: Factorization.Cholesky, then SolveTriangular(Lower, NoTrans)
+ * and SolveTriangular(Lower, Trans) on x (Factorization.Cholesky.cs:114-168,
+ * Blas.SolveTriangular.cs:214-231).  x holds b on entry (unit stride).  On
+ * a failed factorization x is left unchanged.  n <= MFH_EXACT_MAX_N. */
+int mfh_dposv_vbatched(mfh_handle* h, int count, const int* n, double* const* a,
+                       const int* lda, double* const* x, int* info, int max_n);
+
+/* ---- LU solve --------------------------------------------------------------
+ * The solve composition of Factorization.LuInverse.cs:169-192 for one
+ * right-hand side per matrix: y[j] = b[piv[j]]; y <- L1^-1 y (unit lower);
+ * x <- U^-1 y (Upper, NoTrans).  lu and piv as produced by
+ * mfh_dgetrf_vbatched; b and x may alias.  n <= MFH_EXACT_MAX_N. */
+int mfh_dgetrs_vbatched(mfh_handle* h, int count, const int* n, const double* const* lu,
+                        const int* lda, const int* const* piv, const double* const* b,
+                        double* const* x, int* info, int max_n);
+
+/* ---- GEMM ------------------------------------------------------------------
+ * Replaces Blas.MulMatMat(Transpose, Transpose, m, n, k, a, lda, b, ldb, c,
+ * ldc) (Blas.MulMatMat.cs:226-368 double, :407-648 Complex) for a batch with
+ * per-product transposes, extended to C = alpha*op(A)*op(B) + beta*C.  C is
+ * not read when beta == 0; with alpha = 1, beta = 0 the result is exactly
+ * MatFlat's.  A product whose c pointer equals its a or b pointer gets
+ * info = -1 (MatFlat's "In-place operation is not supported").
+ * m, n, k <= 4096 (larger k is fine; sizes are not staged whole).
+ * Complex data is interleaved (re, im) doubles; alpha/beta point to host
+ * arrays of count (re, im) pairs... see mfh_zgemm_vbatched. */
+int mfh_dgemm_vbatched(mfh_handle* h, int count, const int* transa, const int* transb,
+                       const int* m, const int* n, const int* k, const double* alpha,
+                       const double* const* a, const int* lda, const double* const* b,
+                       const int* ldb, const double* beta, double* const* c, const int* ldc,
+                       int* info, int max_m, int max_n, int max_k);
+/* alpha and beta: device arrays of count doubles (dgemm) or count (re, im)
+ * pairs (zgemm), one per product. */
+int mfh_zgemm_vbatched(mfh_handle* h, int count, const int* transa, const int* transb,
+                       const int* m, const int* n, const int* k, const double* alpha,
+                       const double* const* a, const int* lda, const double* const* b,
+                       const int* ldb, const double* beta, double* const* c, const int* ldc,
+                       int* info, int max_m, int max_n, int max_k);
+
+/* ---- mixed-precision LU solve with FP64 iterative refinement --------------
+ * No single MatFlat counterpart: it composes Factorization.Lu and the LU
+ * solve of LuInverse (above) in lower precision and refines in FP64.
+ * Factors each A (FP64, n <= MFH_BLOCKED_MAX_N) in FP32 with partial
+ * pivoting (BF16x3 split MFMA trailing update when n >= 256, FP32
+ * accumulation), then iterates x <- x + A_lo^-1 (b - A x) with the residual,
+ * the update and the test in FP64 until
+ * ||b - A x||_inf / (||A||_inf ||x||_inf + ||b||_inf) <= tol_per_n * n or
+ * max_iter refinements.  iters[b] = refinements taken, berr[b] = final
+ * backward error; info[b] = 0, or n+1 if it did not converge.  A and b are
+ * not modified; x receives the solution. */
+int mfh_dgesv_mixed_vbatched(mfh_handle* h, int count, const int* n, const double* const* a,
+                             const int* lda, const double* const* b, double* const* x,
+                             int max_iter, double tol_per_n, int* iters, double* berr,
+                             int* info, int max_n);
+/* Plain FP64 blocked LU solve for the same inputs (the comparison path of
+ * the mixed-precision solver), with berr as above. */
+int mfh_dgesv_vbatched(mfh_handle* h, int count, const int* n, const double* const* a,
+                       const int* lda, const double* const* b, double* const* x,
+                       double* berr, int* info, int max_n);
+
+/* ---- adjoint of the SPD solve ----------------------------------------------
+ * No MatFlat counterpart (MatFlat has no derivatives).  For x = A^-1 b with
+ * A = L L^T (l from mfh_dposv_vbatched / mfh_dpotrf_vbatched) and
+ * xbar = dloss/dx: bbar = A^-1 xbar and
+ * abar = -(bbar x^T + x bbar^T)/2, written as a full symmetric n x n matrix
+ * with leading dimension ldabar.  n <= MFH_EXACT_MAX_N. */
+int mfh_dposv_backward_vbatched(mfh_handle* h, int count, const int* n, const double* const* l,
+                                const int* lda, const double* const* x, const double* const* xbar,
+                                double* const* abar, const int* ldabar, double* const* bbar,
+                                int* info, int max_n);
+
+/* ---- plans: one batch captured into a hipGraph ------------------------------
+ * op names the entry point; the plan records one call with the given
+ * arguments (device arrays stay owned by the caller and must outlive the
+ * plan) and replays it with mfh_plan_execute on the handle's stream. */
+#define MFH_OP_DPOSV 1
+#define MFH_OP_DGETRF 2
+#define MFH_OP_DPOTRF 3
+int mfh_plan_create_dposv(mfh_handle* h, int count, const int* n, double* const* a,
+                          const int* lda, double* const* x, int* info, int max_n,
+                          mfh_plan** plan);
+int mfh_plan_execute(mfh_plan* plan);
+int mfh_plan_destroy(mfh_plan* plan);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

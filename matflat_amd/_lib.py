@@ -1,0 +1,76 @@
+"""ctypes binding of libmatflat_hip.so (declarations in include/matflat_hip.h).
+
+The library is built in-tree by ``python -m matflat_amd.build``.  Loading it
+fails loudly when it is missing: there is no CPU fallback anywhere in this
+package.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmatflat_hip.so")
+
+MFH_OK = 0
+MFH_ERR_HIP = -1001
+MFH_ERR_ALLOC = -1002
+MFH_ERR_UNSUPPORTED = -1003
+MFH_ERR_STATE = -1004
+
+NOTRANS, TRANS, CONJTRANS, CONJNOTRANS = 111, 112, 113, 114
+UPPER, LOWER = 121, 122
+NONUNIT, UNIT = 131, 132
+EXACT_MAX_N = 128
+BLOCKED_MAX_N = 512
+
+_i, _d, _p, _i64 = ctypes.c_int, ctypes.c_double, ctypes.c_void_p, ctypes.c_int64
+
+# name -> argtypes (all return int)
+SIGNATURES = {
+    "mfh_create": [_p],
+    "mfh_destroy": [_p],
+    "mfh_set_stream": [_p, _p],
+    "mfh_get_stream": [_p, _p],
+    "mfh_synchronize": [_p],
+    "mfh_version": [],
+    "mfh_dgetrf_vbatched": [_p, _i, _p, _p, _p, _p, _p, _p, _p, _i, _i],
+    "mfh_dgetrf_strided_batched": [_p, _i, _i, _i, _p, _i, _i64, _p, _i64, _p, _p],
+    "mfh_dpotrf_vbatched": [_p, _i, _p, _p, _p, _p, _i],
+    "mfh_dtrsv_vbatched": [_p, _i, _i, _i, _i, _p, _p, _p, _p, _p, _p, _i],
+    "mfh_dposv_vbatched": [_p, _i, _p, _p, _p, _p, _p, _i],
+    "mfh_dgetrs_vbatched": [_p, _i, _p, _p, _p, _p, _p, _p, _p, _i],
+    "mfh_dgemm_vbatched": [_p, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i, _i, _i],
+    "mfh_zgemm_vbatched": [_p, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i, _i, _i],
+    "mfh_dgesv_mixed_vbatched": [_p, _i, _p, _p, _p, _p, _p, _i, _d, _p, _p, _p, _i],
+    "mfh_dgesv_vbatched": [_p, _i, _p, _p, _p, _p, _p, _p, _p, _i],
+    "mfh_dposv_backward_vbatched": [_p, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i],
+    "mfh_plan_create_dposv": [_p, _i, _p, _p, _p, _p, _p, _i, _p],
+    "mfh_plan_execute": [_p],
+    "mfh_plan_destroy": [_p],
+}
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    """Load libmatflat_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python -m matflat_amd.build` "
+                "(the HIP library is required; there is no CPU fallback)")
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, args in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = ctypes.c_int
+            fn.argtypes = args
+        lib.mfh_last_error.restype = ctypes.c_char_p
+        lib.mfh_last_error.argtypes = []
+        _lib = lib
+    return _lib
+
+
+def last_error() -> str:
+    return load().mfh_last_error().decode(errors="replace")

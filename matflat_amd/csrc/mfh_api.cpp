@@ -1,0 +1,401 @@
+// mfh_api.cpp -- the C ABI of libmatflat_hip.so: argument validation,
+// handles, workspace and hipGraph plans.  Host argument checks mirror the
+// order and meaning of MatFlat's own (ArgumentException /
+// ArgumentNullException before any write, e.g. Factorization.Lu.cs:124-147);
+// a failed check returns -(argument position) and launches nothing.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <string>
+
+#include "mfh_internal.h"
+
+namespace {
+thread_local std::string g_error;
+}
+
+void mfh_set_error(const std::string& msg) { g_error = msg; }
+
+int mfh_fail(int status, const std::string& msg)
+{
+    g_error = msg;
+    return status;
+}
+
+int mfh_ws_reserve(mfh_handle* h, size_t bytes)
+{
+    if (h->ws_bytes >= bytes) return MFH_OK;
+    if (h->capturing) return mfh_fail(MFH_ERR_STATE, "workspace must be reserved before graph capture");
+    size_t want = bytes < (1u << 20) ? (1u << 20) : bytes + bytes / 4;
+    if (h->ws) {
+        MFH_HIP_CHECK(hipStreamSynchronize(h->stream));
+        MFH_HIP_CHECK(hipFree(h->ws));
+        h->ws = nullptr;
+        h->ws_bytes = 0;
+    }
+    hipError_t e = hipMalloc(&h->ws, want);
+    if (e != hipSuccess) return mfh_fail(MFH_ERR_ALLOC, std::string("hipMalloc workspace: ") + hipGetErrorString(e));
+    h->ws_bytes = want;
+    return MFH_OK;
+}
+
+#define MFH_REQUIRE(cond, pos, msg)                  \
+    do {                                             \
+        if (!(cond)) return mfh_fail(-(pos), (msg)); \
+    } while (0)
+
+extern "C" {
+
+const char* mfh_last_error(void) { return g_error.c_str(); }
+
+int mfh_version(void) { return 100; }
+
+int mfh_create(mfh_handle** handle)
+{
+    MFH_REQUIRE(handle != nullptr, 1, "handle must not be null");
+    mfh_handle* h = new mfh_handle();
+    hipError_t e = hipGetDevice(&h->device);
+    if (e == hipSuccess)
+        e = hipDeviceGetAttribute(&h->num_cus, hipDeviceAttributeMultiprocessorCount, h->device);
+    if (e != hipSuccess) {
+        delete h;
+        return mfh_fail(MFH_ERR_HIP, std::string("no usable HIP device: ") + hipGetErrorString(e));
+    }
+    *handle = h;
+    return MFH_OK;
+}
+
+int mfh_destroy(mfh_handle* h)
+{
+    MFH_REQUIRE(h != nullptr, 1, "handle must not be null");
+    if (h->ws) {
+        hipStreamSynchronize(h->stream);
+        hipFree(h->ws);
+    }
+    delete h;
+    return MFH_OK;
+}
+
+int mfh_set_stream(mfh_handle* h, void* stream)
+{
+    MFH_REQUIRE(h != nullptr, 1, "handle must not be null");
+    h->stream = (hipStream_t)stream;
+    return MFH_OK;
+}
+
+int mfh_get_stream(mfh_handle* h, void** stream)
+{
+    MFH_REQUIRE(h != nullptr, 1, "handle must not be null");
+    MFH_REQUIRE(stream != nullptr, 2, "stream must not be null");
+    *stream = (void*)h->stream;
+    return MFH_OK;
+}
+
+int mfh_synchronize(mfh_handle* h)
+{
+    MFH_REQUIRE(h != nullptr, 1, "handle must not be null");
+    MFH_HIP_CHECK(hipStreamSynchronize(h->stream));
+    return MFH_OK;
+}
+
+int mfh_dgetrf_vbatched(mfh_handle* h, int count, const int* m, const int* n, double* const* a, const int* lda,
+                        int* const* piv, int* sign, int* info, int max_m, int max_n)
+{
+    MFH_REQUIRE(h != nullptr, 1, "handle must not be null");
+    MFH_REQUIRE(count >= 0, 2, "count must be >= 0");
+    if (count == 0) return MFH_OK;
+    MFH_REQUIRE(m != nullptr, 3, "m must not be null");
+    MFH_REQUIRE(n != nullptr, 4, "n must not be null");
+    MFH_REQUIRE(a != nullptr, 5, "a must not be null");
+    MFH_REQUIRE(lda != nullptr, 6, "lda must not be null");
+    MFH_REQUIRE(piv != nullptr, 7, "piv must not be null");
+    MFH_REQUIRE(sign != nullptr, 8, "sign must not be null");
+    MFH_REQUIRE(info != nullptr, 9, "info must not be null");
+    MFH_REQUIRE(max_m >= 1 && max_m <= MFH_EXACT_MAX_N, 10, "max_m must be in [1, MFH_EXACT_MAX_N]");
+    MFH_REQUIRE(max_n >= 1 && max_n <= MFH_EXACT_MAX_N, 11, "max_n must be in [1, MFH_EXACT_MAX_N]");
+    MfhGetrfBatch B{m, n, a, lda, piv, 0, 0, 0, nullptr, 0, nullptr, 0, sign, info};
+    return mfh_launch_getrf(h, count, B, max_m, max_n);
+}
+
+int mfh_dgetrf_strided_batched(mfh_handle* h, int count, int m, int n, double* a, int lda, int64_t stride_a, int* piv,
+                               int64_t stride_piv, int* sign, int* info)
+{
+    MFH_REQUIRE(h != nullptr, 1, "handle must not be null");
+    MFH_REQUIRE(count >= 0, 2, "count must be >= 0");
+    if (count == 0) return MFH_OK;
+    MFH_REQUIRE(m >= 1 && m <= MFH_EXACT_MAX_N, 3, "The number of rows must be in [1, MFH_EXACT_MAX_N].");
+    MFH_REQUIRE(n >= 1 && n <= MFH_EXACT_MAX_N, 4, "The number of columns must be in [1, MFH_EXACT_MAX_N].");
+    MFH_REQUIRE(a != nullptr, 5, "a must not be null");
+    MFH_REQUIRE(lda >= m, 6, "The leading dimension must be greater than or equal to the number of rows.");
+    MFH_REQUIRE(stride_a >= (int64_t)lda * n, 7, "stride_a must be >= lda*n");
+    MFH_REQUIRE(piv != nullptr, 8, "piv must not be null");
+    MFH_REQUIRE(stride_piv >= m, 9, "stride_piv must be >= m");
+    MFH_REQUIRE(sign != nullptr, 10, "sign must not be null");
+    MFH_REQUIRE(info != nullptr, 11, "info must not be null");
+    MfhGetrfBatch B{nullptr, nullptr, nullptr, nullptr, nullptr, m, n, lda, a, stride_a, piv, stride_piv, sign, info};
+    return mfh_launch_getrf(h, count, B, m, n);
+}
+
+int mfh_dpotrf_vbatched(mfh_handle* h, int count, const int* n, double* const* a, const int* lda, int* info,
+                        int max_n)
+{
+    MFH_REQUIRE(h != nullptr, 1, "handle must not be null");
+    MFH_REQUIRE(count >= 0, 2, "count must be >= 0");
+    if (count == 0) return MFH_OK;
+    MFH_REQUIRE(n != nullptr, 3, "n must not be null");
+    MFH_REQUIRE(a != nullptr, 4, "a must not be null");
+    MFH_REQUIRE(lda != nullptr, 5, "lda must not be null");
+    MFH_REQUIRE(info != nullptr, 6, "info must not be null");
+    MFH_REQUIRE(max_n >= 1 && max_n <= MFH_EXACT_MAX_N, 7, "max_n must be in [1, MFH_EXACT_MAX_N]");
+    MfhCholBatch B{n, a, lda, nullptr, info};
+    return mfh_launch_potrf(h, count, B, max_n, false);
+}
+
+int mfh_dposv_vbatched(mfh_handle* h, int count, const int* n, double* const* a, const int* lda, double* const* x,
+                       int* info, int max_n)
+{
+    MFH_REQUIRE(h != nullptr, 1, "handle must not be null");
+    MFH_REQUIRE(count >= 0, 2, "count must be >= 0");
+    if (count == 0) return MFH_OK;
+    MFH_REQUIRE(n != nullptr, 3, "n must not be null");
+    MFH_REQUIRE(a != nullptr, 4, "a must not be null");
+    MFH_REQUIRE(lda != nullptr, 5, "lda must not be null");
+    MFH_REQUIRE(x != nullptr, 6, "x must not be null");
+    MFH_REQUIRE(info != nullptr, 7, "info must not be null");
+    MFH_REQUIRE(max_n >= 1 && max_n <= MFH_EXACT_MAX_N, 8, "max_n must be in [1, MFH_EXACT_MAX_N]");
+    MfhCholBatch B{n, a, lda, x, info};
+    return mfh_launch_potrf(h, count, B, max_n, true);
+}
+
+int mfh_dtrsv_vbatched(mfh_handle* h, int uplo, int trans, int diag, int count, const int* n, const double* const* a,
+                       const int* lda, double* const* x, const int* incx, int* info, int max_n)
+{
+    MFH_REQUIRE(h != nullptr, 1, "handle must not be null");
+    MFH_REQUIRE(uplo == MFH_UPPER || uplo == MFH_LOWER, 2, "Invalid enum value.");
+    MFH_REQUIRE(trans == MFH_NOTRANS || trans == MFH_TRANS, 3, "Invalid enum value.");
+    MFH_REQUIRE(diag == MFH_NONUNIT || diag == MFH_UNIT, 4, "Invalid enum value.");
+    MFH_REQUIRE(count >= 0, 5, "count must be >= 0");
+    if (count == 0) return MFH_OK;
+    MFH_REQUIRE(n != nullptr, 6, "n must not be null");
+    MFH_REQUIRE(a != nullptr, 7, "a must not be null");
+    MFH_REQUIRE(lda != nullptr, 8, "lda must not be null");
+    MFH_REQUIRE(x != nullptr, 9, "x must not be null");
+    MFH_REQUIRE(incx != nullptr, 10, "incx must not be null");
+    MFH_REQUIRE(info != nullptr, 11, "info must not be null");
+    MFH_REQUIRE(max_n >= 1 && max_n <= MFH_EXACT_MAX_N, 12, "max_n must be in [1, MFH_EXACT_MAX_N]");
+    MfhTrsvBatch B{n, a, lda, x, incx, info, uplo, trans, diag == MFH_UNIT ? 1 : 0};
+    return mfh_launch_trsv(h, count, B, max_n);
+}
+
+int mfh_dgetrs_vbatched(mfh_handle* h, int count, const int* n, const double* const* lu, const int* lda,
+                        const int* const* piv, const double* const* b, double* const* x, int* info, int max_n)
+{
+    MFH_REQUIRE(h != nullptr, 1, "handle must not be null");
+    MFH_REQUIRE(count >= 0, 2, "count must be >= 0");
+    if (count == 0) return MFH_OK;
+    MFH_REQUIRE(n != nullptr, 3, "n must not be null");
+    MFH_REQUIRE(lu != nullptr, 4, "lu must not be null");
+    MFH_REQUIRE(lda != nullptr, 5, "lda must not be null");
+    MFH_REQUIRE(piv != nullptr, 6, "piv must not be null");
+    MFH_REQUIRE(b != nullptr, 7, "b must not be null");
+    MFH_REQUIRE(x != nullptr, 8, "x must not be null");
+    MFH_REQUIRE(info != nullptr, 9, "info must not be null");
+    MFH_REQUIRE(max_n >= 1 && max_n <= MFH_EXACT_MAX_N, 10, "max_n must be in [1, MFH_EXACT_MAX_N]");
+    MfhGetrsBatch B{n, lu, lda, piv, b, x, info};
+    return mfh_launch_getrs(h, count, B, max_n);
+}
+
+static int gemm_common(mfh_handle* h, int count, const int* transa, const int* transb, const int* m, const int* n,
+                       const int* k, const double* alpha, const double* const* a, const int* lda,
+                       const double* const* b, const int* ldb, const double* beta, double* const* c,
+                       const int* ldc, int* info, int max_m, int max_n, int max_k, bool cplx)
+{
+    MFH_REQUIRE(h != nullptr, 1, "handle must not be null");
+    MFH_REQUIRE(count >= 0, 2, "count must be >= 0");
+    if (count == 0) return MFH_OK;
+    MFH_REQUIRE(transa != nullptr, 3, "transa must not be null");
+    MFH_REQUIRE(transb != nullptr, 4, "transb must not be null");
+    MFH_REQUIRE(m != nullptr, 5, "m must not be null");
+    MFH_REQUIRE(n != nullptr, 6, "n must not be null");
+    MFH_REQUIRE(k != nullptr, 7, "k must not be null");
+    MFH_REQUIRE(alpha != nullptr, 8, "alpha must not be null");
+    MFH_REQUIRE(a != nullptr, 9, "a must not be null");
+    MFH_REQUIRE(lda != nullptr, 10, "lda must not be null");
+    MFH_REQUIRE(b != nullptr, 11, "b must not be null");
+    MFH_REQUIRE(ldb != nullptr, 12, "ldb must not be null");
+    MFH_REQUIRE(beta != nullptr, 13, "beta must not be null");
+    MFH_REQUIRE(c != nullptr, 14, "c must not be null");
+    MFH_REQUIRE(ldc != nullptr, 15, "ldc must not be null");
+    MFH_REQUIRE(info != nullptr, 16, "info must not be null");
+    MFH_REQUIRE(max_m >= 1 && max_m <= 4096, 17, "max_m must be in [1, 4096]");
+    MFH_REQUIRE(max_n >= 1 && max_n <= 4096, 18, "max_n must be in [1, 4096]");
+    MFH_REQUIRE(max_k >= 1, 19, "max_k must be >= 1");
+    MfhGemmBatch B{transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, info};
+    return mfh_launch_gemm(h, count, B, max_m, max_n, max_k, cplx);
+}
+
+int mfh_dgemm_vbatched(mfh_handle* h, int count, const int* transa, const int* transb, const int* m, const int* n,
+                       const int* k, const double* alpha, const double* const* a, const int* lda,
+                       const double* const* b, const int* ldb, const double* beta, double* const* c, const int* ldc,
+                       int* info, int max_m, int max_n, int max_k)
+{
+    return gemm_common(h, count, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, info, max_m, max_n,
+                       max_k, false);
+}
+
+int mfh_zgemm_vbatched(mfh_handle* h, int count, const int* transa, const int* transb, const int* m, const int* n,
+                       const int* k, const double* alpha, const double* const* a, const int* lda,
+                       const double* const* b, const int* ldb, const double* beta, double* const* c, const int* ldc,
+                       int* info, int max_m, int max_n, int max_k)
+{
+    return gemm_common(h, count, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, info, max_m, max_n,
+                       max_k, true);
+}
+
+int mfh_dgesv_mixed_vbatched(mfh_handle* h, int count, const int* n, const double* const* a, const int* lda,
+                             const double* const* b, double* const* x, int max_iter, double tol_per_n, int* iters,
+                             double* berr, int* info, int max_n)
+{
+    MFH_REQUIRE(h != nullptr, 1, "handle must not be null");
+    MFH_REQUIRE(count >= 0, 2, "count must be >= 0");
+    if (count == 0) return MFH_OK;
+    MFH_REQUIRE(n != nullptr, 3, "n must not be null");
+    MFH_REQUIRE(a != nullptr, 4, "a must not be null");
+    MFH_REQUIRE(lda != nullptr, 5, "lda must not be null");
+    MFH_REQUIRE(b != nullptr, 6, "b must not be null");
+    MFH_REQUIRE(x != nullptr, 7, "x must not be null");
+    MFH_REQUIRE(max_iter >= 0 && max_iter <= 100, 8, "max_iter must be in [0, 100]");
+    MFH_REQUIRE(tol_per_n > 0.0, 9, "tol_per_n must be > 0");
+    MFH_REQUIRE(iters != nullptr, 10, "iters must not be null");
+    MFH_REQUIRE(berr != nullptr, 11, "berr must not be null");
+    MFH_REQUIRE(info != nullptr, 12, "info must not be null");
+    MFH_REQUIRE(max_n >= 1 && max_n <= MFH_BLOCKED_MAX_N, 13, "max_n must be in [1, MFH_BLOCKED_MAX_N]");
+    return mfh_launch_gesv_mixed(h, count, n, a, lda, b, x, max_iter, tol_per_n, iters, berr, info, max_n, true);
+}
+
+int mfh_dgesv_vbatched(mfh_handle* h, int count, const int* n, const double* const* a, const int* lda,
+                       const double* const* b, double* const* x, double* berr, int* info, int max_n)
+{
+    MFH_REQUIRE(h != nullptr, 1, "handle must not be null");
+    MFH_REQUIRE(count >= 0, 2, "count must be >= 0");
+    if (count == 0) return MFH_OK;
+    MFH_REQUIRE(n != nullptr, 3, "n must not be null");
+    MFH_REQUIRE(a != nullptr, 4, "a must not be null");
+    MFH_REQUIRE(lda != nullptr, 5, "lda must not be null");
+    MFH_REQUIRE(b != nullptr, 6, "b must not be null");
+    MFH_REQUIRE(x != nullptr, 7, "x must not be null");
+    MFH_REQUIRE(berr != nullptr, 8, "berr must not be null");
+    MFH_REQUIRE(info != nullptr, 9, "info must not be null");
+    MFH_REQUIRE(max_n >= 1 && max_n <= MFH_BLOCKED_MAX_N, 10, "max_n must be in [1, MFH_BLOCKED_MAX_N]");
+    return mfh_launch_gesv_mixed(h, count, n, a, lda, b, x, 0, 1.0, nullptr, berr, info, max_n, false);
+}
+
+int mfh_dposv_backward_vbatched(mfh_handle* h, int count, const int* n, const double* const* l, const int* lda,
+                                const double* const* x, const double* const* xbar, double* const* abar,
+                                const int* ldabar, double* const* bbar, int* info, int max_n)
+{
+    MFH_REQUIRE(h != nullptr, 1, "handle must not be null");
+    MFH_REQUIRE(count >= 0, 2, "count must be >= 0");
+    if (count == 0) return MFH_OK;
+    MFH_REQUIRE(n != nullptr, 3, "n must not be null");
+    MFH_REQUIRE(l != nullptr, 4, "l must not be null");
+    MFH_REQUIRE(lda != nullptr, 5, "lda must not be null");
+    MFH_REQUIRE(x != nullptr, 6, "x must not be null");
+    MFH_REQUIRE(xbar != nullptr, 7, "xbar must not be null");
+    MFH_REQUIRE(abar != nullptr, 8, "abar must not be null");
+    MFH_REQUIRE(ldabar != nullptr, 9, "ldabar must not be null");
+    MFH_REQUIRE(bbar != nullptr, 10, "bbar must not be null");
+    MFH_REQUIRE(info != nullptr, 11, "info must not be null");
+    MFH_REQUIRE(max_n >= 1 && max_n <= MFH_EXACT_MAX_N, 12, "max_n must be in [1, MFH_EXACT_MAX_N]");
+    MfhAdjointBatch B{n, l, lda, x, xbar, abar, ldabar, bbar, info};
+    return mfh_launch_posv_backward(h, count, B, max_n);
+}
+
+// ---------------------------------------------------------------- plans
+}  // extern "C"
+
+struct mfh_plan {
+    mfh_handle* h = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    void* ws = nullptr;  // the plan's own workspace: later calls on the handle may regrow the handle's
+};
+
+extern "C" {
+
+int mfh_plan_create_dposv(mfh_handle* h, int count, const int* n, double* const* a, const int* lda, double* const* x,
+                          int* info, int max_n, mfh_plan** plan)
+{
+    MFH_REQUIRE(h != nullptr, 1, "handle must not be null");
+    MFH_REQUIRE(plan != nullptr, 9, "plan must not be null");
+    MFH_REQUIRE(count >= 1, 2, "count must be >= 1");
+    // The plan owns a workspace sized exactly as the launcher asks for it;
+    // the capture runs on a private stream (the handle's stream may be the
+    // null stream, which cannot be captured) with the handle temporarily
+    // pointing at the plan's workspace.
+    const size_t ws_bytes = sizeof(int) * (2 * MFH_NBUCKETS + (size_t)MFH_NBUCKETS * count);
+    void* pws = nullptr;
+    MFH_HIP_CHECK(hipMalloc(&pws, ws_bytes));
+    hipStream_t cap;
+    hipError_t e = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        hipFree(pws);
+        return mfh_fail(MFH_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    hipStream_t saved = h->stream;
+    void* saved_ws = h->ws;
+    size_t saved_bytes = h->ws_bytes;
+    h->stream = cap;
+    h->ws = pws;
+    h->ws_bytes = ws_bytes;
+    h->capturing = true;
+    int st = MFH_OK;
+    hipGraph_t graph = nullptr;
+    e = hipStreamBeginCapture(cap, hipStreamCaptureModeRelaxed);
+    if (e == hipSuccess) {
+        st = mfh_dposv_vbatched(h, count, n, a, lda, x, info, max_n);
+        e = hipStreamEndCapture(cap, &graph);
+    }
+    h->stream = saved;
+    h->ws = saved_ws;
+    h->ws_bytes = saved_bytes;
+    h->capturing = false;
+    hipStreamDestroy(cap);
+    hipGraphExec_t exec = nullptr;
+    if (st == MFH_OK && e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (st != MFH_OK || e != hipSuccess) {
+        if (graph) hipGraphDestroy(graph);
+        hipFree(pws);
+        if (st != MFH_OK) return st;
+        return mfh_fail(MFH_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+    }
+    mfh_plan* p = new mfh_plan();
+    p->h = h;
+    p->graph = graph;
+    p->exec = exec;
+    p->ws = pws;
+    *plan = p;
+    return MFH_OK;
+}
+
+int mfh_plan_execute(mfh_plan* plan)
+{
+    MFH_REQUIRE(plan != nullptr, 1, "plan must not be null");
+    MFH_HIP_CHECK(hipGraphLaunch(plan->exec, plan->h->stream));
+    return MFH_OK;
+}
+
+int mfh_plan_destroy(mfh_plan* plan)
+{
+    MFH_REQUIRE(plan != nullptr, 1, "plan must not be null");
+    if (plan->exec) hipGraphExecDestroy(plan->exec);
+    if (plan->graph) hipGraphDestroy(plan->graph);
+    if (plan->ws) {
+        hipStreamSynchronize(plan->h->stream);
+        hipFree(plan->ws);
+    }
+    delete plan;
+    return MFH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,86 @@
+// mfh_device.h -- device helpers shared by the gfx950 kernels.
+//
+// The "exact" helpers reproduce MatFlat's floating-point operation order
+// one IEEE binary64 operation at a time; every file that includes them is
+// compiled with -ffp-contract=off and the helpers carry the pragma as well,
+// so no multiply-add is ever fused.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define MFH_WAVE 64
+
+// Internals.Dot (MatFlat/Internals.cs:265-292): odd length starts with
+// x0*y0, even length with 0.0; then sum = sum + (x_k*y_k + x_{k+1}*y_{k+1}).
+__device__ __forceinline__ double mfh_dot_exact(int len, const double* x, int incx, const double* y, int incy)
+{
+#pragma clang fp contract(off)
+    double s;
+    int c;
+    if (len & 1) {
+        s = x[0] * y[0];
+        c = 1;
+    } else {
+        s = 0.0;
+        c = 0;
+    }
+    for (; c < len; c += 2) {
+        double p0 = x[c * incx] * y[c * incy];
+        double p1 = x[(c + 1) * incx] * y[(c + 1) * incy];
+        s = s + (p0 + p1);
+    }
+    return s;
+}
+
+// Pivot key of Factorization.Lu.cs:323-330 for a parallel argmax.  The
+// reference scans p = j; if |w_i| > |w_p| then p = i.  That picks the first
+// maximum of |w|, never a NaN below the diagonal (comparisons with NaN are
+// false), and keeps p = j when w_j itself is NaN.  Mapping NaN to -1 (below
+// every |w| >= 0) turns the scan into an associative (key, lowest index)
+// max-reduction; the w_j-is-NaN case is handled by the caller.
+__device__ __forceinline__ double mfh_pivot_key(double w)
+{
+    return __builtin_isnan(w) ? -1.0 : __builtin_fabs(w);
+}
+
+__device__ __forceinline__ void mfh_pivot_combine(double& key, int& idx, double okey, int oidx)
+{
+    if (okey > key || (okey == key && oidx < idx)) {
+        key = okey;
+        idx = oidx;
+    }
+}
+
+// Wave-wide (key, index) max with lowest-index tie break.
+__device__ __forceinline__ void mfh_wave_argmax(double& key, int& idx)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        double ok = __shfl_xor(key, off, MFH_WAVE);
+        int oi = __shfl_xor(idx, off, MFH_WAVE);
+        mfh_pivot_combine(key, idx, ok, oi);
+    }
+}
+
+// Work-queue helpers for persistent kernels: counters live in the handle's
+// workspace and are reset by mfh_k_reset_counters before each batch.
+struct MfhQueue {
+    int* head;         // next item to hand out
+    const int* count;  // number of items in the bucket
+    const int* items;  // matrix ids of the bucket
+};
+
+// Pops the next item for the whole workgroup; returns -1 when the bucket is
+// drained.  slot is a __shared__ int.
+__device__ __forceinline__ int mfh_pop(const MfhQueue& q, int* slot)
+{
+    if (threadIdx.x == 0) {
+        int it = atomicAdd(q.head, 1);
+        *slot = (it < *q.count) ? q.items[it] : -1;
+    }
+    __syncthreads();
+    int v = *slot;
+    __syncthreads();
+    return v;
+}

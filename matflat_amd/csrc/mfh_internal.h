@@ -1,0 +1,124 @@
+// mfh_internal.h -- host-side declarations shared by the library's sources.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "../../include/matflat_hip.h"
+
+// Size buckets of the exact kernels: bucket b holds keys in (16b, 16(b+1)];
+// the key is the number of rows a workgroup handles (n + 1 for the SPD solve).
+#define MFH_NBUCKETS 9
+#define MFH_BUCKET_STEP 16
+
+struct mfh_handle {
+    hipStream_t stream = nullptr;
+    int device = 0;
+    int num_cus = 0;
+    void* ws = nullptr;  // device workspace
+    size_t ws_bytes = 0;
+    bool capturing = false;  // plans pre-size the workspace; no allocation while set
+};
+
+// Thread-local error text behind mfh_last_error().
+void mfh_set_error(const std::string& msg);
+int mfh_fail(int status, const std::string& msg);
+
+// Grows the handle workspace to at least bytes (device memory).
+int mfh_ws_reserve(mfh_handle* h, size_t bytes);
+
+#define MFH_HIP_CHECK(expr)                                                          \
+    do {                                                                             \
+        hipError_t e_ = (expr);                                                      \
+        if (e_ != hipSuccess)                                                        \
+            return mfh_fail(MFH_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// Batch descriptors passed by value to kernels.  Pointer arrays may be null,
+// in which case the strided fields describe a uniform batch.
+struct MfhGetrfBatch {
+    const int* m;
+    const int* n;
+    double* const* a;
+    const int* lda;
+    int* const* piv;
+    int m0, n0, lda0;
+    double* a0;
+    int64_t stride_a;
+    int* piv0;
+    int64_t stride_piv;
+    int* sign;
+    int* info;
+};
+
+struct MfhCholBatch {
+    const int* n;
+    double* const* a;
+    const int* lda;
+    double* const* x;  // posv only
+    int* info;
+};
+
+struct MfhTrsvBatch {
+    const int* n;
+    const double* const* a;
+    const int* lda;
+    double* const* x;
+    const int* incx;
+    int* info;
+    int uplo, trans, unit;
+};
+
+struct MfhGetrsBatch {
+    const int* n;
+    const double* const* lu;
+    const int* lda;
+    const int* const* piv;
+    const double* const* b;
+    double* const* x;
+    int* info;
+};
+
+struct MfhGemmBatch {
+    const int* transa;
+    const int* transb;
+    const int* m;
+    const int* n;
+    const int* k;
+    const double* alpha;
+    const double* const* a;
+    const int* lda;
+    const double* const* b;
+    const int* ldb;
+    const double* beta;
+    double* const* c;
+    const int* ldc;
+    int* info;
+};
+
+struct MfhAdjointBatch {
+    const int* n;
+    const double* const* l;
+    const int* lda;
+    const double* const* x;
+    const double* const* xbar;
+    double* const* abar;
+    const int* ldabar;
+    double* const* bbar;
+    int* info;
+};
+
+// Launchers (mfh_exact.hip, mfh_gemm.hip, mfh_adjoint.hip, mfh_mixed.hip).
+int mfh_launch_getrf(mfh_handle* h, int count, const MfhGetrfBatch& B, int max_m, int max_n);
+int mfh_launch_potrf(mfh_handle* h, int count, const MfhCholBatch& B, int max_n, bool posv);
+int mfh_launch_trsv(mfh_handle* h, int count, const MfhTrsvBatch& B, int max_n);
+int mfh_launch_getrs(mfh_handle* h, int count, const MfhGetrsBatch& B, int max_n);
+int mfh_launch_gemm(mfh_handle* h, int count, const MfhGemmBatch& B, int max_m, int max_n,
+                    int max_k, bool complex);
+int mfh_launch_posv_backward(mfh_handle* h, int count, const MfhAdjointBatch& B, int max_n);
+int mfh_launch_gesv_mixed(mfh_handle* h, int count, const int* n, const double* const* a,
+                          const int* lda, const double* const* b, double* const* x, int max_iter,
+                          double tol_per_n, int* iters, double* berr, int* info, int max_n,
+                          bool mixed);

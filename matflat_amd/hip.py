@@ -1,0 +1,216 @@
+"""Batched device API over libmatflat_hip.so (the MatFlat.Hip surface).
+
+Every function here takes torch tensors that already live on the GPU and
+calls the C ABI on torch's current stream; nothing is computed on the CPU.
+Per-matrix arrays (sizes, leading dimensions, pointer arrays, info) are
+device int32/int64 tensors; ``VBatch`` builds them from a packed buffer and
+offsets.
+
+Errors: a negative status from the library raises ``ArgumentError`` (the
+analogue of MatFlat's ArgumentException, e.g. Factorization.Lu.cs:124-147)
+with the library's message; other failures raise ``HipError``.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import torch
+
+from . import _lib
+from ._lib import BLOCKED_MAX_N, EXACT_MAX_N, LOWER, NONUNIT, NOTRANS, TRANS, UNIT, UPPER  # noqa: F401
+
+
+class MatFlatException(Exception):
+    """MatFlatException.cs:9-30."""
+
+
+class MatrixFactorizationException(MatFlatException):
+    """MatrixFactorizationException.cs:9-30: the factorization failed."""
+
+
+class ArgumentError(ValueError):
+    """An invalid argument (MatFlat raises ArgumentException/ArgumentNullException)."""
+
+    def __init__(self, position: int, message: str):
+        super().__init__(f"argument {position}: {message}")
+        self.position = position
+
+
+class HipError(RuntimeError):
+    pass
+
+
+def _check(status: int) -> None:
+    if status == _lib.MFH_OK:
+        return
+    msg = _lib.last_error()
+    if -1000 < status < 0:
+        raise ArgumentError(-status, msg)
+    raise HipError(f"libmatflat_hip status {status}: {msg}")
+
+
+def _p(t) -> int:
+    if t is None:
+        return 0
+    if isinstance(t, int):
+        return t
+    return t.data_ptr()
+
+
+class Handle:
+    """Owns an mfh_handle bound to torch's current stream at call time."""
+
+    def __init__(self, device: int | None = None):
+        self.lib = _lib.load()
+        if device is not None:
+            torch.cuda.set_device(device)
+        torch.cuda.init()
+        h = ctypes.c_void_p()
+        _check(self.lib.mfh_create(ctypes.byref(h)))
+        self.h = h
+
+    def bind_stream(self, stream: torch.cuda.Stream | None = None) -> None:
+        s = stream if stream is not None else torch.cuda.current_stream()
+        _check(self.lib.mfh_set_stream(self.h, ctypes.c_void_p(s.cuda_stream)))
+
+    def synchronize(self) -> None:
+        _check(self.lib.mfh_synchronize(self.h))
+
+    def close(self) -> None:
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.lib.mfh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default: Handle | None = None
+
+
+def default_handle() -> Handle:
+    global _default
+    if _default is None:
+        _default = Handle()
+    _default.bind_stream()
+    return _default
+
+
+@dataclass
+class VBatch:
+    """Device-side per-matrix arrays for a batch packed in one buffer.
+
+    ``buf`` is a 1-D device tensor; matrix i starts at element ``offsets[i]``.
+    ``ptrs`` is the device int64 array of element addresses the C ABI takes.
+    """
+
+    buf: torch.Tensor
+    offsets: torch.Tensor  # int64, device
+    ptrs: torch.Tensor     # int64, device
+
+    @staticmethod
+    def of(buf: torch.Tensor, offsets) -> "VBatch":
+        offs = torch.as_tensor(offsets, dtype=torch.int64, device=buf.device)
+        ptrs = buf.data_ptr() + offs * buf.element_size()
+        return VBatch(buf, offs, ptrs.contiguous())
+
+
+def _i32(x, device) -> torch.Tensor:
+    return torch.as_tensor(x, dtype=torch.int32, device=device).contiguous()
+
+
+# ---------------------------------------------------------------------------
+def dgetrf_vbatched(m, n, a_ptrs, lda, piv_ptrs, sign, info, max_m, max_n, handle: Handle | None = None):
+    h = handle or default_handle()
+    _check(h.lib.mfh_dgetrf_vbatched(h.h, int(m.numel()), _p(m), _p(n), _p(a_ptrs), _p(lda), _p(piv_ptrs),
+                                     _p(sign), _p(info), int(max_m), int(max_n)))
+
+
+def dgetrf_strided_batched(count, m, n, a, lda, stride_a, piv, stride_piv, sign, info, handle: Handle | None = None):
+    h = handle or default_handle()
+    _check(h.lib.mfh_dgetrf_strided_batched(h.h, int(count), int(m), int(n), _p(a), int(lda), int(stride_a),
+                                            _p(piv), int(stride_piv), _p(sign), _p(info)))
+
+
+def dpotrf_vbatched(n, a_ptrs, lda, info, max_n, handle: Handle | None = None):
+    h = handle or default_handle()
+    _check(h.lib.mfh_dpotrf_vbatched(h.h, int(n.numel()), _p(n), _p(a_ptrs), _p(lda), _p(info), int(max_n)))
+
+
+def dposv_vbatched(n, a_ptrs, lda, x_ptrs, info, max_n, handle: Handle | None = None):
+    h = handle or default_handle()
+    _check(h.lib.mfh_dposv_vbatched(h.h, int(n.numel()), _p(n), _p(a_ptrs), _p(lda), _p(x_ptrs), _p(info),
+                                    int(max_n)))
+
+
+def dtrsv_vbatched(uplo, trans, diag, n, a_ptrs, lda, x_ptrs, incx, info, max_n, handle: Handle | None = None):
+    h = handle or default_handle()
+    _check(h.lib.mfh_dtrsv_vbatched(h.h, int(uplo), int(trans), int(diag), int(n.numel()), _p(n), _p(a_ptrs),
+                                    _p(lda), _p(x_ptrs), _p(incx), _p(info), int(max_n)))
+
+
+def dgetrs_vbatched(n, lu_ptrs, lda, piv_ptrs, b_ptrs, x_ptrs, info, max_n, handle: Handle | None = None):
+    h = handle or default_handle()
+    _check(h.lib.mfh_dgetrs_vbatched(h.h, int(n.numel()), _p(n), _p(lu_ptrs), _p(lda), _p(piv_ptrs), _p(b_ptrs),
+                                     _p(x_ptrs), _p(info), int(max_n)))
+
+
+def gemm_vbatched(transa, transb, m, n, k, alpha, a_ptrs, lda, b_ptrs, ldb, beta, c_ptrs, ldc, info,
+                  max_m, max_n, max_k, complex_: bool = False, handle: Handle | None = None):
+    h = handle or default_handle()
+    fn = h.lib.mfh_zgemm_vbatched if complex_ else h.lib.mfh_dgemm_vbatched
+    _check(fn(h.h, int(m.numel()), _p(transa), _p(transb), _p(m), _p(n), _p(k), _p(alpha), _p(a_ptrs), _p(lda),
+              _p(b_ptrs), _p(ldb), _p(beta), _p(c_ptrs), _p(ldc), _p(info), int(max_m), int(max_n), int(max_k)))
+
+
+def dgesv_mixed_vbatched(n, a_ptrs, lda, b_ptrs, x_ptrs, max_iter, tol_per_n, iters, berr, info, max_n,
+                         handle: Handle | None = None):
+    h = handle or default_handle()
+    _check(h.lib.mfh_dgesv_mixed_vbatched(h.h, int(n.numel()), _p(n), _p(a_ptrs), _p(lda), _p(b_ptrs), _p(x_ptrs),
+                                          int(max_iter), float(tol_per_n), _p(iters), _p(berr), _p(info),
+                                          int(max_n)))
+
+
+def dgesv_vbatched(n, a_ptrs, lda, b_ptrs, x_ptrs, berr, info, max_n, handle: Handle | None = None):
+    h = handle or default_handle()
+    _check(h.lib.mfh_dgesv_vbatched(h.h, int(n.numel()), _p(n), _p(a_ptrs), _p(lda), _p(b_ptrs), _p(x_ptrs),
+                                    _p(berr), _p(info), int(max_n)))
+
+
+def dposv_backward_vbatched(n, l_ptrs, lda, x_ptrs, xbar_ptrs, abar_ptrs, ldabar, bbar_ptrs, info, max_n,
+                            handle: Handle | None = None):
+    h = handle or default_handle()
+    _check(h.lib.mfh_dposv_backward_vbatched(h.h, int(n.numel()), _p(n), _p(l_ptrs), _p(lda), _p(x_ptrs),
+                                             _p(xbar_ptrs), _p(abar_ptrs), _p(ldabar), _p(bbar_ptrs), _p(info),
+                                             int(max_n)))
+
+
+class DposvPlan:
+    """A batched SPD solve captured once into a hipGraph and replayed."""
+
+    def __init__(self, n, a_ptrs, lda, x_ptrs, info, max_n, handle: Handle | None = None):
+        self.handle = handle or default_handle()
+        self._keep = (n, a_ptrs, lda, x_ptrs, info)
+        p = ctypes.c_void_p()
+        _check(self.handle.lib.mfh_plan_create_dposv(self.handle.h, int(n.numel()), _p(n), _p(a_ptrs), _p(lda),
+                                                     _p(x_ptrs), _p(info), int(max_n), ctypes.byref(p)))
+        self.p = p
+
+    def execute(self) -> None:
+        self.handle.bind_stream()
+        _check(self.handle.lib.mfh_plan_execute(self.p))
+
+    def close(self) -> None:
+        if self.p is not None and self.p.value:
+            self.handle.lib.mfh_plan_destroy(self.p)
+            self.p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

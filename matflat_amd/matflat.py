@@ -1,0 +1,184 @@
+"""MatFlat's hot-path signatures, executed on the GPU.
+
+``Factorization.Lu``, ``Factorization.Cholesky``, ``Blas.SolveTriangular``
+and ``Blas.MulMatMat`` keep the reference's argument lists, argument checks
+(same order, same conditions) and exceptions, with device tensors in place
+of pointers: ``a`` is a 1-D float64 (complex128 for complex GEMM) CUDA
+tensor holding a column-major matrix with leading dimension ``lda``, exactly
+the buffer the reference's pointer addresses.  Each call runs the batched
+HIP kernel with a batch of one and waits for it (the reference is
+synchronous).
+
+Reference signatures:
+* Factorization.Lu(int m, int n, double* a, int lda, int* piv) -> int
+  (MatFlat/Factorization.Lu.cs:122-161)
+* Factorization.Cholesky(int n, double* a, int lda)
+  (MatFlat/Factorization.Cholesky.cs:114-168)
+* Blas.SolveTriangular(Uplo, Transpose, int n, double* a, int lda, double* x, int incx)
+  (MatFlat/Blas.SolveTriangular.cs:162-242)
+* Blas.MulMatMat(Transpose, Transpose, int m, int n, int k, double* a, int lda,
+  double* b, int ldb, double* c, int ldc)  (MatFlat/Blas.MulMatMat.cs:226-368, :407-575)
+"""
+from __future__ import annotations
+
+from enum import IntEnum
+
+import torch
+
+from . import hip
+from .hip import ArgumentError, MatFlatException, MatrixFactorizationException  # noqa: F401
+
+
+class Transpose(IntEnum):
+    """Transpose.cs:8-29 (CBLAS values)."""
+
+    NoTrans = 111
+    Trans = 112
+    ConjTrans = 113
+    ConjNoTrans = 114
+
+
+class Uplo(IntEnum):
+    """Uplo.cs:8-19 (CBLAS values)."""
+
+    Upper = 121
+    Lower = 122
+
+
+class ArgumentException(ArgumentError):
+    def __init__(self, message: str, param: str | None = None):
+        ValueError.__init__(self, f"{message} (Parameter '{param}')" if param else message)
+        self.param = param
+        self.position = 0
+
+
+class ArgumentNullException(ArgumentException):
+    def __init__(self, param: str):
+        super().__init__("Value cannot be null.", param)
+
+
+def _dev(t: torch.Tensor) -> torch.device:
+    return t.device
+
+
+def _ptrs(*ts: torch.Tensor) -> torch.Tensor:
+    return torch.tensor([t.data_ptr() for t in ts], dtype=torch.int64, device=ts[0].device)
+
+
+def _i32(v, device) -> torch.Tensor:
+    return torch.tensor(v if isinstance(v, (list, tuple)) else [v], dtype=torch.int32, device=device)
+
+
+def _need(buf: torch.Tensor | None, elems: int, name: str) -> None:
+    if buf is not None and buf.numel() < elems:
+        raise ArgumentException(f"The buffer is too small ({buf.numel()} < {elems}).", name)
+
+
+class Factorization:
+    @staticmethod
+    def Lu(m: int, n: int, a: torch.Tensor | None, lda: int, piv: torch.Tensor | None) -> int:
+        if m <= 0:
+            raise ArgumentException("The number of rows must be greater than or equal to one.", "m")
+        if n <= 0:
+            raise ArgumentException("The number of columns must be greater than or equal to one.", "n")
+        if a is None:
+            raise ArgumentNullException("a")
+        if lda < m:
+            raise ArgumentException("The leading dimension must be greater than or equal to the number of rows.",
+                                    "lda")
+        if piv is None:
+            raise ArgumentNullException("piv")
+        _need(a, lda * (n - 1) + m, "a")
+        _need(piv, m, "piv")
+        dev = _dev(a)
+        sign = torch.zeros(1, dtype=torch.int32, device=dev)
+        info = torch.zeros(1, dtype=torch.int32, device=dev)
+        hip.dgetrf_strided_batched(1, m, n, a, lda, lda * n, piv, m, sign, info)
+        torch.cuda.synchronize()
+        return int(sign.item())
+
+    @staticmethod
+    def Cholesky(n: int, a: torch.Tensor | None, lda: int) -> None:
+        if n <= 0:
+            raise ArgumentException("The order of the matrix must be greater than or equal to one.", "n")
+        if a is None:
+            raise ArgumentNullException("a")
+        if lda < n:
+            raise ArgumentException(
+                "The leading dimension must be greater than or equal to the order of the matrix.", "lda")
+        _need(a, lda * (n - 1) + n, "a")
+        dev = _dev(a)
+        info = torch.zeros(1, dtype=torch.int32, device=dev)
+        hip.dpotrf_vbatched(_i32(n, dev), _ptrs(a), _i32(lda, dev), info, n)
+        if int(info.item()) != 0:
+            raise MatrixFactorizationException(
+                "Cholesky decomposition failed. The matrix must be positive definite.")
+
+
+class Blas:
+    @staticmethod
+    def SolveTriangular(uplo: int, transa: int, n: int, a: torch.Tensor | None, lda: int, x: torch.Tensor | None,
+                        incx: int) -> None:
+        if n <= 0:
+            raise ArgumentException("The order of the matrix must be greater than or equal to one.", "n")
+        if a is None:
+            raise ArgumentNullException("a")
+        if lda < n:
+            raise ArgumentException("The leading dimension must be greater than or equal to the number of rows.",
+                                    "lda")
+        if x is None:
+            raise ArgumentNullException("x")
+        if incx <= 0:
+            raise ArgumentException("The value must be greater than or equal to one.", "incx")
+        if uplo not in (Uplo.Upper, Uplo.Lower):
+            raise ArgumentException("Invalid enum value.", "uplo")
+        if transa not in (Transpose.NoTrans, Transpose.Trans):
+            raise ArgumentException("Invalid enum value.", "transa")
+        _need(a, lda * (n - 1) + n, "a")
+        _need(x, incx * (n - 1) + 1, "x")
+        dev = _dev(a)
+        info = torch.zeros(1, dtype=torch.int32, device=dev)
+        hip.dtrsv_vbatched(int(uplo), int(transa), hip.NONUNIT, _i32(n, dev), _ptrs(a), _i32(lda, dev), _ptrs(x),
+                           _i32(incx, dev), info, n)
+        torch.cuda.synchronize()
+
+    @staticmethod
+    def MulMatMat(transa: int, transb: int, m: int, n: int, k: int, a: torch.Tensor | None, lda: int,
+                  b: torch.Tensor | None, ldb: int, c: torch.Tensor | None, ldc: int) -> None:
+        cplx = a is not None and a.is_complex()
+        valid = (111, 112, 113, 114) if cplx else (111, 112)
+        if m <= 0:
+            raise ArgumentException("The value must be greater than or equal to one.", "m")
+        if n <= 0:
+            raise ArgumentException("The value must be greater than or equal to one.", "n")
+        if k <= 0:
+            raise ArgumentException("The value must be greater than or equal to one.", "k")
+        if a is None:
+            raise ArgumentNullException("a")
+        if transa not in valid:
+            raise ArgumentException("Invalid enum value.", "transa")
+        if lda < (k if transa in (112, 113) else m):
+            raise ArgumentException("The leading dimension must be greater than or equal to the number of rows.",
+                                    "lda")
+        if b is None:
+            raise ArgumentNullException("b")
+        if transb not in valid:
+            raise ArgumentException("Invalid enum value.", "transb")
+        if ldb < (n if transb in (112, 113) else k):
+            raise ArgumentException("The leading dimension must be greater than or equal to the number of rows.",
+                                    "ldb")
+        if c is None:
+            raise ArgumentNullException("c")
+        if ldc < m:
+            raise ArgumentException("The leading dimension must be greater than or equal to the number of rows.",
+                                    "ldc")
+        if a.data_ptr() == c.data_ptr() or b.data_ptr() == c.data_ptr():
+            raise ArgumentException("In-place operation is not supported.")
+        dev = _dev(a)
+        info = torch.zeros(1, dtype=torch.int32, device=dev)
+        one = torch.tensor([1.0, 0.0] if cplx else [1.0], dtype=torch.float64, device=dev)
+        zero = torch.zeros(2 if cplx else 1, dtype=torch.float64, device=dev)
+        hip.gemm_vbatched(_i32(int(transa), dev), _i32(int(transb), dev), _i32(m, dev), _i32(n, dev), _i32(k, dev),
+                          one, _ptrs(a), _i32(lda, dev), _ptrs(b), _i32(ldb, dev), zero, _ptrs(c), _i32(ldc, dev),
+                          info, m, n, k, complex_=cplx)
+        torch.cuda.synchronize()

@@ -1,0 +1,385 @@
+"""GPU parity of the exact kernels against the CPU oracle (MatFlat's operation
+order restated in oracle/matflat_oracle.c).  The bar is bit-identical output:
+factors, pivots, signs, solutions and untouched padding."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from gpu_util import Packed, assert_bits_equal, f64, i32
+from helpers import rand_buf, rand_cbuf, spd_upper_buf
+from test_oracle import GEMM_CASES, LU_CASES
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("needs a GPU")
+    from matflat_amd import hip  # noqa: F401  (fails loudly if the library is missing)
+    yield
+
+
+def hipmod():
+    from matflat_amd import hip
+    return hip
+
+
+# ------------------------------------------------------------------ LU
+def run_getrf(cases, bufs):
+    hip = hipmod()
+    ms = [c[0] for c in cases]
+    ns = [c[1] for c in cases]
+    ldas = [c[2] for c in cases]
+    A = Packed(bufs)
+    P = Packed([np.full(m, -7, dtype=np.int32) for m in ms], dtype=torch.int32)
+    sign = i32(np.zeros(len(cases)))
+    info = i32(np.full(len(cases), 99))
+    hip.dgetrf_vbatched(i32(ms), i32(ns), A.ptrs, i32(ldas), P.ptrs, sign, info, max(ms), max(ns))
+    torch.cuda.synchronize()
+    return A.all(), P.all(), sign.cpu().numpy(), info.cpu().numpy()
+
+
+def check_getrf(cases, bufs, what):
+    got_a, got_p, got_s, got_i = run_getrf(cases, [b.copy() for b in bufs])
+    assert np.all(got_i == 0), got_i
+    for idx, ((m, n, lda), b) in enumerate(zip(cases, bufs)):
+        want = b.copy()
+        piv, sign = oracle.dgetrf(m, n, want, lda)
+        assert_bits_equal(got_a[idx], want, f"{what} A case {(m, n, lda)}")
+        assert np.array_equal(got_p[idx], piv), (m, n, lda)
+        assert got_s[idx] == sign, (m, n, lda)
+
+
+def test_lu_reference_shapes_one_batch():
+    bufs = [rand_buf(42, m, n, lda) for (m, n, lda) in LU_CASES]
+    check_getrf(LU_CASES, bufs, "LuTests shapes")
+
+
+def test_lu_special_matrices():
+    cases = [(3, 3, 3)] * 6
+    bufs = [np.zeros(9), np.arange(1.0, 10.0),
+            np.array([np.nan, 5.0, 3.0, 1.0, 2.0, 3.0, 4.0, 5.0, 7.0]),
+            np.array([1.0, np.nan, 3.0, 1.0, 2.0, 3.0, 4.0, 5.0, 7.0]),
+            np.array([1.0, np.inf, -np.inf, 1.0, 2.0, 3.0, 4.0, 5.0, 7.0]),
+            np.array([1.0, -2.0, 2.0, 1.0, 2.0, 3.0, 4.0, 5.0, 7.0])]
+    check_getrf(cases, bufs, "special")
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_lu_variable_sizes(seed):
+    rng = np.random.default_rng(seed)
+    cases = []
+    for _ in range(300):
+        m = int(rng.integers(1, 129))
+        n = int(rng.integers(1, 129))
+        lda = m + int(rng.integers(0, 5))
+        cases.append((m, n, lda))
+    bufs = [rand_buf(1000 * seed + i, m, n, lda, dist="normal") for i, (m, n, lda) in enumerate(cases)]
+    check_getrf(cases, bufs, "var")
+
+
+def test_lu_strided_n64_batch():
+    hip = hipmod()
+    count, n = 1024, 64
+    rng = np.random.default_rng(1)
+    a = rng.standard_normal(count * n * n)
+    dev = torch.from_numpy(a.copy()).cuda()
+    piv = torch.zeros(count * n, dtype=torch.int32, device="cuda")
+    sign = torch.zeros(count, dtype=torch.int32, device="cuda")
+    info = torch.full((count,), 99, dtype=torch.int32, device="cuda")
+    hip.dgetrf_strided_batched(count, n, n, dev, n, n * n, piv, n, sign, info)
+    torch.cuda.synchronize()
+    want_piv, want_sign = oracle.dgetrf_batch(count, n, n, n, n * n, a, threads=8)
+    assert_bits_equal(dev.cpu().numpy(), a, "n=64 batch")
+    assert np.array_equal(piv.cpu().numpy(), want_piv)
+    assert np.array_equal(sign.cpu().numpy(), want_sign)
+    assert torch.all(info == 0)
+
+
+# ------------------------------------------------------------------ Cholesky / posv
+def chol_cases(seed, count, nmax=128, pad=8):
+    rng = np.random.default_rng(seed)
+    ns = rng.integers(1, nmax + 1, size=count)
+    ldas = [int(n + rng.integers(0, pad)) for n in ns]
+    return [int(n) for n in ns], ldas
+
+
+def test_cholesky_variable_sizes_nan_lower():
+    hip = hipmod()
+    ns, ldas = chol_cases(3, 200)
+    bufs = [spd_upper_buf(i, n, l) for i, (n, l) in enumerate(zip(ns, ldas))]
+    A = Packed(bufs)
+    info = i32(np.full(len(ns), 99))
+    hip.dpotrf_vbatched(i32(ns), A.ptrs, i32(ldas), info, max(ns))
+    torch.cuda.synchronize()
+    got = A.all()
+    assert np.all(info.cpu().numpy() == 0)
+    for i, (n, l) in enumerate(zip(ns, ldas)):
+        want = bufs[i].copy()
+        assert oracle.dpotrf(n, want, l) == 0
+        assert_bits_equal(got[i], want, f"potrf n={n} lda={l}")
+
+
+def test_cholesky_failures_match_partial_writes():
+    hip = hipmod()
+    bufs, ns = [], []
+    for seed in range(20):
+        n = 2 + seed
+        rng = np.random.default_rng(seed)
+        up = np.triu(2 * rng.random((n, n)) - 1) + np.diag(rng.random(n) * (seed % 3))
+        buf = np.full(n * n, np.nan)
+        for j in range(n):
+            buf[j * n:j * n + j + 1] = up[: j + 1, j]
+        bufs.append(buf)
+        ns.append(n)
+    bufs.append(np.array([np.nan, np.nan, 1.0, 1.0]))  # NaN diagonal
+    ns.append(2)
+    A = Packed(bufs)
+    info = i32(np.full(len(ns), 99))
+    hip.dpotrf_vbatched(i32(ns), A.ptrs, i32(ns), info, max(ns))
+    torch.cuda.synchronize()
+    got = A.all()
+    gi = info.cpu().numpy()
+    for i, n in enumerate(ns):
+        want = bufs[i].copy()
+        winfo = oracle.dpotrf(n, want, n)
+        assert gi[i] == winfo, (i, n)
+        assert_bits_equal(got[i], want, f"potrf failure case {i}")
+
+
+def test_posv_variable_sizes():
+    hip = hipmod()
+    ns, ldas = chol_cases(5, 300, nmax=128)
+    bufs = [spd_upper_buf(100 + i, n, l) for i, (n, l) in enumerate(zip(ns, ldas))]
+    xs = [np.random.default_rng(7 + i).standard_normal(n) for i, n in enumerate(ns)]
+    A = Packed(bufs)
+    X = Packed(xs)
+    info = i32(np.full(len(ns), 99))
+    hip.dposv_vbatched(i32(ns), A.ptrs, i32(ldas), X.ptrs, info, max(ns))
+    torch.cuda.synchronize()
+    ga, gx = A.all(), X.all()
+    assert np.all(info.cpu().numpy() == 0)
+    for i, (n, l) in enumerate(zip(ns, ldas)):
+        wa, wx = bufs[i].copy(), xs[i].copy()
+        assert oracle.dposv(n, wa, l, wx) == 0
+        assert_bits_equal(ga[i], wa, f"posv A n={n}")
+        assert_bits_equal(gx[i], wx, f"posv x n={n}")
+
+
+def test_posv_plan_replays_match_eager():
+    hip = hipmod()
+    ns, ldas = chol_cases(11, 500, nmax=100)
+    bufs = [spd_upper_buf(300 + i, n, l) for i, (n, l) in enumerate(zip(ns, ldas))]
+    xs = [np.random.default_rng(70 + i).standard_normal(n) for i, n in enumerate(ns)]
+    A0, X0 = Packed(bufs), Packed(xs)
+    A, X = Packed(bufs), Packed(xs)
+    info = i32(np.full(len(ns), 99))
+    plan = hip.DposvPlan(i32(ns), A.ptrs, i32(ldas), X.ptrs, info, max(ns))
+    wants = []
+    for i, (n, l) in enumerate(zip(ns, ldas)):
+        wa, wx = bufs[i].copy(), xs[i].copy()
+        oracle.dposv(n, wa, l, wx)
+        wants.append((wa, wx))
+    for rep in range(4):
+        A.dev.copy_(A0.dev)
+        X.dev.copy_(X0.dev)
+        info.fill_(99)
+        torch.cuda.synchronize()
+        plan.execute()
+        torch.cuda.synchronize()
+        _ = (A.dev.sum() + X.dev.sum()).item()  # eager work between replays
+        ga, gx = A.all(), X.all()
+        assert np.all(info.cpu().numpy() == 0), rep
+        for i in range(len(ns)):
+            assert_bits_equal(ga[i], wants[i][0], f"replay {rep} A {i}")
+            assert_bits_equal(gx[i], wants[i][1], f"replay {rep} x {i}")
+    plan.close()
+
+
+# ------------------------------------------------------------------ TRSV
+@pytest.mark.parametrize("uplo", [121, 122])
+@pytest.mark.parametrize("trans", [111, 112])
+@pytest.mark.parametrize("diag", [131, 132])
+def test_trsv_variants(uplo, trans, diag):
+    hip = hipmod()
+    rng = np.random.default_rng(uplo * 7 + trans + diag)
+    ns = [1, 2, 3, 7, 16, 33, 64, 65, 100, 128] + [int(v) for v in rng.integers(1, 129, size=40)]
+    ldas = [n + int(rng.integers(0, 4)) for n in ns]
+    incs = [int(rng.integers(1, 4)) for _ in ns]
+    bufs = []
+    for i, (n, l) in enumerate(zip(ns, ldas)):
+        a = rand_buf(500 + i, n, n, l)
+        for j in range(n):
+            a[j * l + j] = 2.0 + abs(a[j * l + j]) * np.sqrt(n)
+        bufs.append(a)
+    xs = [rand_buf(900 + i, 1, n * inc, 1) for i, (n, inc) in enumerate(zip(ns, incs))]
+    A, X = Packed(bufs), Packed(xs)
+    info = i32(np.full(len(ns), 99))
+    hip.dtrsv_vbatched(uplo, trans, diag, i32(ns), A.ptrs, i32(ldas), X.ptrs, i32(incs), info, max(ns))
+    torch.cuda.synchronize()
+    gx = X.all()
+    assert np.all(info.cpu().numpy() == 0)
+    for i, (n, l, inc) in enumerate(zip(ns, ldas, incs)):
+        want = xs[i].copy()
+        if diag == 131:
+            oracle.dtrsv(uplo, trans, n, bufs[i], l, want, inc)
+        else:
+            a = bufs[i].copy()
+            for j in range(n):
+                a[j * l + j] = 1.0  # LuInverse's unit-diagonal copy
+            oracle.dtrsv(uplo, trans, n, a, l, want, inc)
+        assert_bits_equal(gx[i], want, f"trsv {uplo}/{trans}/{diag} n={n} inc={inc}")
+
+
+# ------------------------------------------------------------------ LU solve
+def test_getrs_after_getrf():
+    hip = hipmod()
+    rng = np.random.default_rng(21)
+    ns = [int(v) for v in rng.integers(1, 129, size=120)]
+    cases = [(n, n, n + 1) for n in ns]
+    bufs = [rand_buf(40 + i, n, n, n + 1, dist="normal") for i, n in enumerate(ns)]
+    A = Packed(bufs)
+    P = Packed([np.zeros(n, dtype=np.int32) for n in ns], dtype=torch.int32)
+    sign = i32(np.zeros(len(ns)))
+    info = i32(np.full(len(ns), 9))
+    hip.dgetrf_vbatched(i32(ns), i32(ns), A.ptrs, i32([c[2] for c in cases]), P.ptrs, sign, info, 128, 128)
+    bs = [rng.standard_normal(n) for n in ns]
+    Bv = Packed(bs)
+    Xv = Packed([np.zeros(n) for n in ns])
+    info2 = i32(np.full(len(ns), 9))
+    hip.dgetrs_vbatched(i32(ns), A.ptrs, i32([c[2] for c in cases]), P.ptrs, Bv.ptrs, Xv.ptrs, info2, 128)
+    torch.cuda.synchronize()
+    gx = Xv.all()
+    assert np.all(info2.cpu().numpy() == 0)
+    for i, n in enumerate(ns):
+        lu = bufs[i].copy()
+        piv, _ = oracle.dgetrf(n, n, lu, n + 1)
+        want = oracle.dgetrs(n, lu, n + 1, piv, bs[i])
+        assert_bits_equal(gx[i], want, f"getrs n={n}")
+
+
+# ------------------------------------------------------------------ GEMM
+def gemm_batch(complex_: bool, seed: int, count: int, shapes=None):
+    rng = np.random.default_rng(seed)
+    ops = [111, 112, 113, 114] if complex_ else [111, 112]
+    recs = []
+    for i in range(count):
+        if shapes is not None and i < len(shapes):
+            m, n, k = shapes[i][:3]
+        else:
+            m, n, k = (int(v) for v in rng.integers(1, 129, size=3))
+        ta, tb = int(rng.choice(ops)), int(rng.choice(ops))
+        ra, ca = (m, k) if ta in (111, 114) else (k, m)
+        rb, cb = (k, n) if tb in (111, 114) else (n, k)
+        lda, ldb, ldc = ra + int(rng.integers(0, 4)), rb + int(rng.integers(0, 4)), m + int(rng.integers(0, 4))
+        gen = rand_cbuf if complex_ else rand_buf
+        a, b, c = gen(3 * i, ra, ca, lda, "normal"), gen(3 * i + 1, rb, cb, ldb, "normal"), gen(3 * i + 2, m, n, ldc)
+        if i % 3 == 0:
+            alpha, beta = (1.0, 0.0)
+        else:
+            alpha, beta = rng.uniform(-1, 1), rng.uniform(-1, 1)
+            if complex_:
+                alpha, beta = complex(alpha, rng.uniform(-1, 1)), complex(beta, rng.uniform(-1, 1))
+        recs.append((ta, tb, m, n, k, lda, ldb, ldc, a, b, c, alpha, beta))
+    return recs
+
+
+@pytest.mark.parametrize("complex_", [False, True])
+def test_gemm_vbatched_bit_exact(complex_):
+    hip = hipmod()
+    recs = gemm_batch(complex_, 17, 400, shapes=[(m, n, k) for (m, n, k, *_r) in GEMM_CASES])
+    dt = torch.complex128 if complex_ else torch.float64
+    A = Packed([r[8] for r in recs], dtype=dt)
+    Bm = Packed([r[9] for r in recs], dtype=dt)
+    C = Packed([r[10] for r in recs], dtype=dt)
+    if complex_:
+        al = f64(np.array([[complex(r[11]).real, complex(r[11]).imag] for r in recs]).reshape(-1))
+        be = f64(np.array([[complex(r[12]).real, complex(r[12]).imag] for r in recs]).reshape(-1))
+    else:
+        al = f64([r[11] for r in recs])
+        be = f64([r[12] for r in recs])
+    info = i32(np.full(len(recs), 99))
+    col = lambda j: i32([r[j] for r in recs])  # noqa: E731
+    hip.gemm_vbatched(col(0), col(1), col(2), col(3), col(4), al, A.ptrs, col(5), Bm.ptrs, col(6), be, C.ptrs,
+                      col(7), info, 128, 128, 128, complex_=complex_)
+    torch.cuda.synchronize()
+    gc = C.all()
+    assert np.all(info.cpu().numpy() == 0)
+    for i, (ta, tb, m, n, k, lda, ldb, ldc, a, b, c, alpha, beta) in enumerate(recs):
+        want = c.copy()
+        fn = oracle.zgemm if complex_ else oracle.dgemm
+        fn(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, want, ldc)
+        assert_bits_equal(gc[i], want, f"gemm {i} {(ta, tb, m, n, k)}")
+
+
+def test_gemm_aliasing_rejected_per_product():
+    hip = hipmod()
+    buf = torch.zeros(64, dtype=torch.float64, device="cuda")
+    p = torch.tensor([buf.data_ptr()], dtype=torch.int64, device="cuda")
+    info = i32([99])
+    one = f64([1.0])
+    hip.gemm_vbatched(i32([111]), i32([111]), i32([4]), i32([4]), i32([4]), one, p, i32([4]), p, i32([4]),
+                      f64([0.0]), p, i32([4]), info, 4, 4, 4)
+    torch.cuda.synchronize()
+    assert int(info.item()) == -1
+
+
+# ------------------------------------------------------------------ adjoint
+def test_posv_backward_matches_oracle():
+    hip = hipmod()
+    rng = np.random.default_rng(31)
+    ns = [1, 2, 5, 32, 64, 100, 128] + [int(v) for v in rng.integers(1, 129, size=60)]
+    ls, xs, ws = [], [], []
+    for i, n in enumerate(ns):
+        buf = spd_upper_buf(600 + i, n, n)
+        x = rng.standard_normal(n)
+        assert oracle.dposv(n, buf, n, x) == 0
+        ls.append(buf)
+        xs.append(x)
+        ws.append(rng.standard_normal(n))
+    L, X, W = Packed(ls), Packed(xs), Packed(ws)
+    AB = Packed([np.zeros(n * n) for n in ns])
+    BB = Packed([np.zeros(n) for n in ns])
+    info = i32(np.full(len(ns), 99))
+    hip.dposv_backward_vbatched(i32(ns), L.ptrs, i32(ns), X.ptrs, W.ptrs, AB.ptrs, i32(ns), BB.ptrs, info, 128)
+    torch.cuda.synchronize()
+    assert np.all(info.cpu().numpy() == 0)
+    gab, gbb = AB.all(), BB.all()
+    for i, n in enumerate(ns):
+        abar, bbar = oracle.dposv_backward(n, ls[i], n, xs[i], ws[i])
+        assert_bits_equal(gbb[i], bbar, f"bbar n={n}")
+        assert_bits_equal(gab[i], abar, f"abar n={n}")
+
+
+# ------------------------------------------------------------------ MatFlat signatures
+def test_matflat_signatures_on_device():
+    from matflat_amd.matflat import (ArgumentException, Blas, Factorization, MatrixFactorizationException,
+                                     Transpose, Uplo)
+    m, n, lda = 23, 11, 31
+    a = rand_buf(42, m, n, lda)
+    dev = torch.from_numpy(a.copy()).cuda()
+    piv = torch.zeros(m, dtype=torch.int32, device="cuda")
+    sign = Factorization.Lu(m, n, dev, lda, piv)
+    want = a.copy()
+    wpiv, wsign = oracle.dgetrf(m, n, want, lda)
+    assert_bits_equal(dev.cpu().numpy(), want, "Lu")
+    assert np.array_equal(piv.cpu().numpy(), wpiv) and sign == wsign
+    with pytest.raises(ArgumentException):
+        Factorization.Lu(0, 3, dev, 3, piv)
+    with pytest.raises(ArgumentException):
+        Factorization.Lu(4, 3, dev, 3, piv)
+    bad = torch.tensor([1.0, np.nan, 2.0, -5.0], dtype=torch.float64, device="cuda")
+    with pytest.raises(MatrixFactorizationException):
+        Factorization.Cholesky(2, bad, 2)
+    x = torch.ones(5, dtype=torch.float64, device="cuda")
+    tri = torch.from_numpy(np.eye(5).reshape(-1) * 2.0).cuda()
+    Blas.SolveTriangular(Uplo.Lower, Transpose.NoTrans, 5, tri, 5, x, 1)
+    assert torch.all(x == 0.5)
+    c = torch.zeros(4, dtype=torch.float64, device="cuda")
+    aa = torch.tensor([1.0, 2.0, 3.0, 4.0], dtype=torch.float64, device="cuda")
+    Blas.MulMatMat(Transpose.NoTrans, Transpose.NoTrans, 2, 2, 2, aa, 2, aa, 2, c, 2)
+    assert c.cpu().tolist() == [7.0, 10.0, 15.0, 22.0]
+    with pytest.raises(ArgumentException):
+        Blas.MulMatMat(Transpose.NoTrans, Transpose.NoTrans, 2, 2, 2, aa, 2, aa, 2, aa, 2)
