@@ -1,0 +1,195 @@
+"""Benchmark: variable-size batched FP64 SPD solve (Cholesky + two triangular
+solves) on one MI355X.
+
+Workload (one step): 100,000 SPD matrices, order n ~ U{10..100}, leading
+dimension 8*ceil(n/8), A = B B^T + n I with B ~ N(0,1) (upper triangle
+stored, strict lower triangle NaN: it is never read), one N(0,1) right-hand
+side each.  A step is one replay of the hipGraph that holds the whole batch
+(device bucketing + the persistent solve kernels); inputs are resident in
+HBM before timing starts.  Each step factors and solves in place; the
+kernels are data-independent (no pivoting, no early exits on SPD input), so
+every step performs the same work on the same bytes.
+
+Prints one JSON line (see README for the fields).  --gpus N with N > 1 exits:
+this build runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "solves/s, batched var-size LU/Chol n≤512: FP32+IR vs FP64 at equal bkwd err"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def make_spd_batch(count: int, nmin: int, nmax: int, seed: int, device: str = "cuda"):
+    """Generates the batch on the GPU.  Returns (buf, offsets, ns, ldas, b)."""
+    import torch
+
+    rng = np.random.default_rng(seed)
+    ns = rng.integers(nmin, nmax + 1, size=count).astype(np.int32)
+    ldas = ((ns + 7) // 8 * 8).astype(np.int32)
+    # memory grouped by size, batch order shuffled
+    order = np.argsort(ns, kind="stable")
+    sizes = ldas.astype(np.int64) * ns
+    offsets = np.zeros(count, dtype=np.int64)
+    offsets[order] = np.concatenate([[0], np.cumsum(sizes[order])[:-1]])
+    total = int(sizes.sum())
+    buf = torch.full((total,), float("nan"), dtype=torch.float64, device=device)
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    for n in range(nmin, nmax + 1):
+        idx = np.flatnonzero(ns == n)
+        if idx.size == 0:
+            continue
+        lda = int((n + 7) // 8 * 8)
+        c = idx.size
+        bm = torch.randn((c, n, n), dtype=torch.float64, device=device, generator=g)
+        a = torch.bmm(bm, bm.transpose(1, 2).contiguous())
+        a += n * torch.eye(n, dtype=torch.float64, device=device)
+        # column-major slab [c][col][row(lda)], upper triangle only
+        slab = torch.full((c, n, lda), float("nan"), dtype=torch.float64, device=device)
+        upper = torch.triu(torch.ones(n, n, dtype=torch.bool, device=device))  # [row, col], row <= col
+        at = a.transpose(1, 2)  # at[b, col, row] = a[b, row, col]
+        slab[:, :, :n] = torch.where(upper.T.unsqueeze(0), at, torch.full_like(at, float("nan")))
+        start = int(offsets[idx].min())
+        buf[start:start + c * n * lda] = slab.reshape(-1)
+        del bm, a, slab, at
+    xoff = np.concatenate([[0], np.cumsum(ns.astype(np.int64))[:-1]])
+    b = torch.randn(int(ns.sum()), dtype=torch.float64, device=device, generator=g)
+    return buf, offsets, ns, ldas, xoff, b
+
+
+def alg_bytes_posv(ns: np.ndarray) -> float:
+    n = ns.astype(np.float64)
+    return float(np.sum(8.0 * (n * (n + 1) / 2 + n * n + 2 * n) + 4.0))
+
+
+def cpu_baseline(buf_host, offsets, ns, ldas, xoff, b_host, max_seconds=20.0):
+    import oracle
+
+    threads = max(1, min(16, os.cpu_count() or 1))
+    count = ns.size
+    reps, elapsed = 0, []
+    t_start = time.perf_counter()
+    while True:
+        a = buf_host.copy()
+        x = b_host.copy()
+        t0 = time.perf_counter()
+        info = oracle.dposv_batch(ns, ldas, offsets, a, xoff, x, threads=threads)
+        elapsed.append(time.perf_counter() - t0)
+        reps += 1
+        assert np.all(info == 0)
+        if reps >= 5 or time.perf_counter() - t_start > max_seconds:
+            break
+    med = float(np.median(elapsed))
+    return {"value": count / med, "unit": "solves/s", "cores": threads, "kind": "port",
+            "sample": f"the full {count}-matrix batch, C restatement of MatFlat (oracle/matflat_oracle.c, "
+                      f"-O2 -ffp-contract=off) with OpenMP dynamic scheduling, median of {reps} runs"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--count", type=int, default=100_000)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (matrices are independent: "
+                                   "a multi-GPU run would shard the batch, see DESIGN.md)"}))
+        return 2
+
+    import torch
+
+    from matflat_amd import hip
+
+    seed, nmin, nmax = 2, 10, 100
+    log(f"generating {args.count} SPD matrices n~U{{{nmin}..{nmax}}}")
+    buf, offsets, ns, ldas, xoff, b = make_spd_batch(args.count, nmin, nmax, seed)
+    torch.cuda.synchronize()
+    buf_host = buf.cpu().numpy() if not args.no_cpu_baseline else None
+    b_host = b.cpu().numpy() if not args.no_cpu_baseline else None
+
+    dev = torch.device("cuda")
+    n_t = torch.from_numpy(ns).to(dev)
+    lda_t = torch.from_numpy(ldas).to(dev)
+    a_ptrs = (buf.data_ptr() + 8 * torch.from_numpy(offsets).to(dev)).contiguous()
+    x_ptrs = (b.data_ptr() + 8 * torch.from_numpy(xoff).to(dev)).contiguous()
+    info = torch.zeros(args.count, dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream()
+    handle = hip.Handle()
+    handle.bind_stream(stream)
+    plan = hip.DposvPlan(n_t, a_ptrs, lda_t, x_ptrs, info, nmax, handle=handle)
+
+    for _ in range(args.warmup):
+        plan.execute()
+    torch.cuda.synchronize()
+    if int((info != 0).sum().item()) != 0:
+        log("factorization failures during warmup")
+        return 1
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    start.record(stream)
+    for _ in range(args.steps):
+        plan.execute()
+    stop.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = start.elapsed_time(stop) / args.steps
+    log(f"{args.steps} steps: {ms:.4f} ms/step (events), wall {wall * 1e3 / args.steps:.4f} ms/step")
+
+    value = args.count / (ms / 1e3)
+    bytes_step = alg_bytes_posv(ns)
+    achieved = bytes_step / (ms / 1e3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_potrf_exact<T,true> bucket launches (whole posv step)",
+                "alg_bytes_per_step": bytes_step}
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline")
+        cpu = cpu_baseline(buf_host, offsets, ns, ldas, xoff, b_host)
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded N(0,1); A = B B^T + n I)",
+        "config": {"workload": "batched FP64 Cholesky + forward/back triangular solve (MatFlat order), "
+                               f"{args.count} SPD matrices, n~U{{{nmin}..{nmax}}}, lda=8*ceil(n/8), one rhs each, "
+                               "one hipGraph replay per step",
+                   "count": args.count, "n_min": nmin, "n_max": nmax, "seed": seed},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    plan.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -333,7 +333,7 @@ int mfh_plan_create_dposv(mfh_handle* h, int count, const int* n, double* const*
     // the capture runs on a private stream (the handle's stream may be the
     // null stream, which cannot be captured) with the handle temporarily
     // pointing at the plan's workspace.
-    const size_t ws_bytes = sizeof(int) * (2 * MFH_NBUCKETS + (size_t)MFH_NBUCKETS * count);
+    const size_t ws_bytes = mfh_exact_ws_bytes(count);
     void* pws = nullptr;
     MFH_HIP_CHECK(hipMalloc(&pws, ws_bytes));
     hipStream_t cap;

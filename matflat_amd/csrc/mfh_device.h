@@ -63,24 +63,18 @@ __device__ __forceinline__ void mfh_wave_argmax(double& key, int& idx)
     }
 }
 
-// Work-queue helpers for persistent kernels: counters live in the handle's
-// workspace and are reset by mfh_k_reset_counters before each batch.
+// Persistent-kernel work lists.  A bucket's items are the matrix ids of one
+// contiguous range of the device-side counting sort (largest size first).
+// Workgroup w takes items w, w + grid, w + 2*grid, ...: no shared counter,
+// so no single-address atomic traffic (which serialises in L2), and
+// neighbouring workgroups get neighbouring (similar-size) matrices.
 struct MfhQueue {
-    int* head;         // next item to hand out
-    const int* count;  // number of items in the bucket
-    const int* items;  // matrix ids of the bucket
+    const int* start;  // device: first item of the bucket in items[]
+    const int* count;  // device: number of items in the bucket
+    const int* items;  // matrix ids sorted by size key, descending
 };
 
-// Pops the next item for the whole workgroup; returns -1 when the bucket is
-// drained.  slot is a __shared__ int.
-__device__ __forceinline__ int mfh_pop(const MfhQueue& q, int* slot)
+__device__ __forceinline__ int mfh_item(const MfhQueue& q, int idx)
 {
-    if (threadIdx.x == 0) {
-        int it = atomicAdd(q.head, 1);
-        *slot = (it < *q.count) ? q.items[it] : -1;
-    }
-    __syncthreads();
-    int v = *slot;
-    __syncthreads();
-    return v;
+    return (idx < *q.count) ? q.items[*q.start + idx] : -1;
 }

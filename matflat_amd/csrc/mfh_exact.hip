@@ -1,22 +1,25 @@
 // mfh_exact.hip -- batched LU, Cholesky, SPD solve, triangular solve and LU
 // solve for gfx950, reproducing MatFlat's FP64 operation order exactly.
 //
-// Design (MI355X):
-//  * One workgroup per matrix, the whole matrix staged in LDS (n <= 128, at
-//    most 133 KB of the CU's 160 KB); one thread per row, so a workgroup is
-//    64 * ceil(rows / 64) threads (1-3 wavefronts).
-//  * Persistent kernels: matrices are bucketed by size on the device (16-wide
-//    buckets), and each bucket's kernel is launched with as many workgroups
-//    as fit on the 256 CUs; workgroups pop matrix ids from the bucket's
-//    atomic queue until it is drained.  The LDS allocation and thread count
-//    therefore follow the bucket, not the largest matrix of the batch.
-//  * The factorizations are scheduled column by column (Crout order): at
-//    step k every row computes its dot product for column k in parallel.
-//    Each output element is still exactly one Internals.Dot of MatFlat's
-//    length and pairing, fed with the same operands, so the results are
-//    bit-identical to MatFlat's row/column-sequential loops.
-//  * Pivot search is a wavefront (|w|, index) max-reduction with MatFlat's
-//    first-maximum and NaN rules (mfh_pivot_key).
+// Scheduling (MI355X):
+//  * A device-side counting sort orders the batch by size key (largest
+//    first) without any host round trip, and splits it into 16-wide size
+//    buckets.  Each bucket runs one persistent launch sized to the CUs; its
+//    workgroups walk the bucket's items with a fixed stride, so there is no
+//    shared work counter (a single L2 address serialises atomics) and
+//    neighbouring workgroups hold similar sizes.
+//  * The Cholesky / SPD-solve kernel packs several matrices into one
+//    wavefront (8, 16, 32 or 64 lanes per matrix by size) with the strict
+//    lower triangle packed in LDS, and every lane owns several rows: at
+//    step k all rows below k compute their dot product for column k at
+//    once, so one wavefront carries 2-8 independent dot-product chains per
+//    lane (instruction-level parallelism instead of occupancy, which the
+//    LDS footprint limits).
+//  * LU, triangular solve and LU solve keep one workgroup per matrix with
+//    the matrix in LDS and one thread per row.
+//  * Every output element is exactly one Internals.Dot of MatFlat's length
+//    and pairing, fed with the same operands (MatFlat/Internals.cs:265-292),
+//    so the results are bit-identical to MatFlat's sequential loops.
 #include "mfh_device.h"
 #include "mfh_internal.h"
 
@@ -26,73 +29,190 @@
 
 namespace {
 
-constexpr int kBucketBound(int b) { return MFH_BUCKET_STEP * (b + 1); }
+constexpr int kKeys = MFH_NBUCKETS * MFH_BUCKET_STEP + 1;  // size keys 0..144
 
-__host__ __device__ inline int mfh_bucket_of(int key) { return (key + MFH_BUCKET_STEP - 1) / MFH_BUCKET_STEP - 1; }
+__host__ __device__ inline int bucket_of(int key) { return (key + MFH_BUCKET_STEP - 1) / MFH_BUCKET_STEP - 1; }
 
-size_t exact_lds_bytes(int nb) { return (size_t)nb * nb * 8 + (size_t)nb * 8 + (size_t)nb * 4 + 64; }
-
-// Workspace: counts[NB] heads[NB] items[NB * count]
-struct BucketWs {
-    int* counts;
-    int* heads;
+// ---------------------------------------------------------------- sorting
+// Workspace (ints): keys[count] items[count] hist[kKeys] cursor[kKeys]
+// bstart[NB] bcount[NB].
+struct SortWs {
+    int* keys;
     int* items;
+    int* hist;
+    int* cursor;
+    int* bstart;
+    int* bcount;
 };
 
-BucketWs bucket_ws(mfh_handle* h, int count)
+size_t sort_ws_bytes(int count) { return sizeof(int) * (2 * (size_t)count + 2 * kKeys + 2 * MFH_NBUCKETS); }
+
+SortWs sort_ws(void* base, int count)
 {
-    int* base = (int*)h->ws;
-    return BucketWs{base, base + MFH_NBUCKETS, base + 2 * MFH_NBUCKETS};
+    int* p = (int*)base;
+    SortWs w;
+    w.keys = p;
+    w.items = p + count;
+    w.hist = p + 2 * (size_t)count;
+    w.cursor = w.hist + kKeys;
+    w.bstart = w.cursor + kKeys;
+    w.bcount = w.bstart + MFH_NBUCKETS;
+    return w;
 }
 
-size_t bucket_ws_bytes(int count) { return sizeof(int) * (2 * MFH_NBUCKETS + (size_t)MFH_NBUCKETS * count); }
-
-__global__ void k_reset(int* p, int n)
+__global__ void k_sort_reset(SortWs ws)
 {
-    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
+    for (int i = threadIdx.x; i < kKeys; i += blockDim.x) {
+        ws.hist[i] = 0;
+        ws.cursor[i] = 0;
+    }
 }
 
-__device__ inline void bucket_push(BucketWs ws, int count, int key, int i)
+// Validates matrix i through op.key(i) (which writes a negative info and
+// returns -1 for an invalid matrix) and counts the key in a block-local
+// histogram; one global atomic per key per block.
+template <typename Op>
+__global__ __launch_bounds__(256) void k_sort_hist(Op op, int count, SortWs ws)
 {
-    int b = mfh_bucket_of(key);
-    int pos = atomicAdd(&ws.counts[b], 1);
-    ws.items[(size_t)b * count + pos] = i;
+    __shared__ int h[kKeys];
+    for (int t = threadIdx.x; t < kKeys; t += blockDim.x) h[t] = 0;
+    __syncthreads();
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int key = -1;
+    if (i < count) {
+        key = op.key(i);
+        ws.keys[i] = key;
+    }
+    if (key >= 0) atomicAdd(&h[key], 1);
+    __syncthreads();
+    for (int t = threadIdx.x; t < kKeys; t += blockDim.x)
+        if (h[t]) atomicAdd(&ws.hist[t], h[t]);
+}
+
+// Descending-key exclusive offsets, per-key cursors and bucket ranges.
+__global__ void k_sort_offsets(SortWs ws)
+{
+    if (threadIdx.x != 0) return;
+    int run = 0;
+    for (int k = kKeys - 1; k >= 0; --k) {
+        ws.cursor[k] = run;
+        run += ws.hist[k];
+    }
+    for (int b = 0; b < MFH_NBUCKETS; ++b) {
+        const int hi = MFH_BUCKET_STEP * (b + 1), lo = MFH_BUCKET_STEP * b;
+        int c = 0;
+        for (int k = lo + 1; k <= hi; ++k) c += ws.hist[k];
+        ws.bstart[b] = ws.cursor[hi];
+        ws.bcount[b] = c;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_sort_scatter(int count, SortWs ws)
+{
+    __shared__ int h[kKeys];
+    __shared__ int base[kKeys];
+    for (int t = threadIdx.x; t < kKeys; t += blockDim.x) h[t] = 0;
+    __syncthreads();
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int key = (i < count) ? ws.keys[i] : -1;
+    int lpos = -1;
+    if (key >= 0) lpos = atomicAdd(&h[key], 1);
+    __syncthreads();
+    for (int t = threadIdx.x; t < kKeys; t += blockDim.x)
+        if (h[t]) base[t] = atomicAdd(&ws.cursor[t], h[t]);
+    __syncthreads();
+    if (key >= 0) ws.items[base[key] + lpos] = i;
+}
+
+template <typename Op>
+int device_sort(mfh_handle* h, const Op& op, int count, SortWs* out)
+{
+    int st = mfh_ws_reserve(h, sort_ws_bytes(count));
+    if (st) return st;
+    SortWs ws = sort_ws(h->ws, count);
+    const unsigned blocks = (unsigned)((count + 255) / 256);
+    hipLaunchKernelGGL(k_sort_reset, dim3(1), dim3(256), 0, h->stream, ws);
+    hipLaunchKernelGGL(k_sort_hist<Op>, dim3(blocks), dim3(256), 0, h->stream, op, count, ws);
+    hipLaunchKernelGGL(k_sort_offsets, dim3(1), dim3(64), 0, h->stream, ws);
+    hipLaunchKernelGGL(k_sort_scatter, dim3(blocks), dim3(256), 0, h->stream, count, ws);
+    MFH_HIP_CHECK(hipGetLastError());
+    *out = ws;
+    return MFH_OK;
+}
+
+MfhQueue bucket_queue(const SortWs& ws, int b) { return MfhQueue{ws.bstart + b, ws.bcount + b, ws.items}; }
+
+// Launches a persistent kernel for one bucket: as many workgroups as fit
+// on the device (by occupancy), capped by what the bucket could need.
+template <typename K, typename... Args>
+int launch_bucket(mfh_handle* h, K kernel, int threads, size_t lds, long max_groups, Args... args)
+{
+    MFH_HIP_CHECK(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    int per_cu = 0;
+    MFH_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, lds));
+    if (per_cu < 1) per_cu = 1;
+    long grid = (long)per_cu * h->num_cus;
+    if (grid > max_groups) grid = max_groups;
+    if (grid < 1) grid = 1;
+    hipLaunchKernelGGL(kernel, dim3((unsigned)grid), dim3(threads), lds, h->stream, args...);
+    MFH_HIP_CHECK(hipGetLastError());
+    return MFH_OK;
+}
+
+int threads_for(int rows) { return MFH_WAVE * ((rows + MFH_WAVE - 1) / MFH_WAVE); }
+
+int nbuckets_for(int max_key)
+{
+    int nb = bucket_of(max_key) + 1;
+    return nb > MFH_NBUCKETS ? MFH_NBUCKETS : nb;
+}
+
+int bucket_bound(int b, int max_key)
+{
+    int hi = MFH_BUCKET_STEP * (b + 1);
+    return hi < max_key ? hi : max_key;
 }
 
 // ------------------------------------------------------------------ LU
-__global__ void k_bucketize_getrf(MfhGetrfBatch B, int count, BucketWs ws, int max_m, int max_n)
-{
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= count) return;
-    int m = B.m ? B.m[i] : B.m0;
-    int n = B.n ? B.n[i] : B.n0;
-    int lda = B.lda ? B.lda[i] : B.lda0;
-    double* a = B.a ? B.a[i] : B.a0;
-    int* piv = B.piv ? B.piv[i] : B.piv0;
-    int bad = 0;
-    if (m < 1 || m > max_m || m > MFH_EXACT_MAX_N) bad = -3;
-    else if (n < 1 || n > max_n || n > MFH_EXACT_MAX_N) bad = -4;
-    else if (a == nullptr) bad = -5;
-    else if (lda < m) bad = -6;
-    else if (piv == nullptr) bad = -7;
-    if (bad) {
-        B.info[i] = bad;
-        return;
+struct GetrfKey {
+    MfhGetrfBatch B;
+    int max_m, max_n;
+    __device__ int key(int i) const
+    {
+        const int m = B.m ? B.m[i] : B.m0;
+        const int n = B.n ? B.n[i] : B.n0;
+        const int lda = B.lda ? B.lda[i] : B.lda0;
+        const double* a = B.a ? B.a[i] : B.a0;
+        const int* piv = B.piv ? B.piv[i] : B.piv0;
+        int bad = 0;
+        if (m < 1 || m > max_m || m > MFH_EXACT_MAX_N) bad = -3;
+        else if (n < 1 || n > max_n || n > MFH_EXACT_MAX_N) bad = -4;
+        else if (a == nullptr) bad = -5;
+        else if (lda < m) bad = -6;
+        else if (piv == nullptr) bad = -7;
+        if (bad) {
+            B.info[i] = bad;
+            return -1;
+        }
+        return m > n ? m : n;
     }
-    bucket_push(ws, count, m > n ? m : n, i);
-}
+};
 
+// Factorization.Lu.cs:296-350 in Crout order: at step k the column k
+// candidates (rows i >= k, dots of length k) are computed in parallel, the
+// pivot is a (|w|, index) max-reduction, rows are swapped across all n
+// columns, the multipliers are divided, and row k of U (columns j > k, dots
+// of length k) is computed in parallel.
 template <int T>
 __global__ __launch_bounds__(T) void k_getrf_exact(MfhGetrfBatch B, MfhQueue q)
 {
     extern __shared__ double lds[];
-    __shared__ int s_slot;
     __shared__ double s_rkey[T / MFH_WAVE];
     __shared__ int s_ridx[T / MFH_WAVE];
     __shared__ int s_p;
     const int tid = threadIdx.x;
-    for (;;) {
-        const int b = mfh_pop(q, &s_slot);
+    for (int it = blockIdx.x;; it += gridDim.x) {
+        const int b = mfh_item(q, it);
         if (b < 0) break;
         const int m = B.m ? B.m[b] : B.m0;
         const int n = B.n ? B.n[b] : B.n0;
@@ -113,8 +233,6 @@ __global__ __launch_bounds__(T) void k_getrf_exact(MfhGetrfBatch B, MfhQueue q)
         int sign = 1;
         const int kmax = m < n ? m : n;
         for (int k = 0; k < kmax; ++k) {
-            // Column k below (and at) the diagonal: Factorization.Lu.cs:315-320
-            // for rows i >= j, a dot of length j.
             const int i = k + tid;
             const bool act = i < m;
             double w = 0.0;
@@ -122,7 +240,6 @@ __global__ __launch_bounds__(T) void k_getrf_exact(MfhGetrfBatch B, MfhQueue q)
                 w = L[i + k * m] - mfh_dot_exact(k, L + i, m, L + k * m, 1);
                 L[i + k * m] = w;
             }
-            // Pivot (Lu.cs:323-330).
             double key = act ? mfh_pivot_key(w) : -2.0;
             int idx = act ? i : INT_MAX;
             mfh_wave_argmax(key, idx);
@@ -140,7 +257,6 @@ __global__ __launch_bounds__(T) void k_getrf_exact(MfhGetrfBatch B, MfhQueue q)
             __syncthreads();
             const int p = s_p;
             if (p != k) {
-                // Internals.SwapRows over all n columns (Lu.cs:332-337).
                 for (int j = tid; j < n; j += T) {
                     double t = L[k + j * m];
                     L[k + j * m] = L[p + j * m];
@@ -154,11 +270,8 @@ __global__ __launch_bounds__(T) void k_getrf_exact(MfhGetrfBatch B, MfhQueue q)
                 sign = -sign;
                 __syncthreads();
             }
-            // Multipliers by true division (Lu.cs:339-344).
             const double diag = L[k + k * m];
             if (act && i > k && diag != 0.0) L[i + k * m] = L[i + k * m] / diag;
-            // Row k of U for columns j > k: the dot of length k that the
-            // reference computes when it reaches column j (Lu.cs:315-320, i < j).
             const int j = k + 1 + tid;
             if (j < n) L[k + j * m] = L[k + j * m] - mfh_dot_exact(k, L + k, m, L + j * m, 1);
             __syncthreads();
@@ -179,140 +292,296 @@ __global__ __launch_bounds__(T) void k_getrf_exact(MfhGetrfBatch B, MfhQueue q)
 }
 
 // ------------------------------------------------------------ Cholesky / posv
-__global__ void k_bucketize_chol(MfhCholBatch B, int count, BucketWs ws, int max_n, int posv)
-{
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= count) return;
-    int n = B.n[i];
-    int bad = 0;
-    if (n < 1 || n > max_n || n > MFH_EXACT_MAX_N) bad = -3;
-    else if (B.a[i] == nullptr) bad = -4;
-    else if (B.lda[i] < n) bad = -5;
-    else if (posv && B.x[i] == nullptr) bad = -6;
-    if (bad) {
-        B.info[i] = bad;
-        return;
+struct CholKey {
+    MfhCholBatch B;
+    int max_n, posv;
+    __device__ int key(int i) const
+    {
+        const int n = B.n[i];
+        int bad = 0;
+        if (n < 1 || n > max_n || n > MFH_EXACT_MAX_N) bad = -3;
+        else if (B.a[i] == nullptr) bad = -4;
+        else if (B.lda[i] < n) bad = -5;
+        else if (posv && B.x[i] == nullptr) bad = -6;
+        if (bad) {
+            B.info[i] = bad;
+            return -1;
+        }
+        return posv ? n + 1 : n;
     }
-    bucket_push(ws, count, posv ? n + 1 : n, i);
-}
+};
 
-// Factorization.Cholesky.cs:114-168, scheduled by columns: at step k every
+// Packed strict lower triangle of an NB x NB matrix, row-major: element
+// (j, c), c < j, at rowp(j) + c.  Row j is contiguous, so the two elements
+// of a dot-product pair are adjacent (one ds_read2_b64), and a column of the
+// input's upper triangle lands in one contiguous row.
+__device__ __forceinline__ int rowp(int j) { return (j * (j - 1)) >> 1; }
+
+// Factorization.Cholesky.cs:114-168 scheduled by columns.  At step k every
 // row j > k computes L[j,k] = (A[k,j] - Dot(k, L[j,0:k], L[k,0:k])) / L[k,k]
 // and adds t*t to its own running sum (the reference's s, accumulated in
-// the same k order).  The thread of row k+1 then finishes its diagonal in
-// the same step.  With POSV the right-hand side is carried as an extra row
-// n (b in column n of the LDS copy): its "L[n,k]" is exactly the forward
-// substitution of SolveTriangular(Lower, NoTrans) (Blas.SolveTriangular.cs:
-// 216-222), after which the Lower/Trans sweep (:224-231) runs with one
-// thread per unknown.
-template <int T, bool POSV>
-__global__ __launch_bounds__(T) void k_potrf_exact(MfhCholBatch B, MfhQueue q)
+// the same k order); the owner of row k+1 then finishes that diagonal.  The
+// strict upper triangle is loaded transposed into the packed strict lower
+// triangle, so A[k,j] sits exactly where L[j,k] is written.
+//
+// Look-ahead: the dot product of step k is Internals.Dot's pairwise sum; all
+// of it except the last pair (k-2, k-1) only involves columns finished two
+// steps earlier, so each step computes that bulk for the NEXT step while
+// the short critical chain of the current step (last pair, subtraction,
+// division, the next diagonal's square root) is in flight.  The result is
+// the same sum in the same order.
+//
+// With POSV the right-hand side is carried as an extra row n (b[c] in place
+// of A[c,n]): its "L[n,k]" is exactly the forward substitution of
+// SolveTriangular(Lower, NoTrans) (Blas.SolveTriangular.cs:216-222; the
+// products are the same pairs, multiplied in the other operand order, which
+// is exact).  The Lower/Trans sweep (:224-231) then runs bottom-up with one
+// lane per unknown.
+//
+// G lanes serve one matrix (64/G matrices per wavefront); lane l owns rows
+// l, l+G, ..., l+(S-1)G.  NB is the packed dimension (rows, including the
+// right-hand-side row), per_mat the LDS doubles per matrix.
+template <int G, int S, bool POSV>
+__global__ __launch_bounds__(64) void k_chol_packed(MfhCholBatch B, MfhQueue q, int NB, int per_mat)
 {
     extern __shared__ double lds[];
-    __shared__ int s_slot;
-    __shared__ int s_fail;
-    const int tid = threadIdx.x;
-    for (;;) {
-        const int b = mfh_pop(q, &s_slot);
-        if (b < 0) break;
-        const int n = B.n[b];
-        const int lda = B.lda[b];
-        double* A = B.a[b];
-        double* X = POSV ? B.x[b] : nullptr;
-        const int rows = POSV ? n + 1 : n;
-        const int ls = rows;
-        double* S = lds;
-        double* xs = lds + ls * rows;
+    constexpr int MW = MFH_WAVE / G;
+    const int lane = threadIdx.x;
+    const int grp = lane / G;
+    const int l = lane - grp * G;
+    double* Mx = lds + (size_t)grp * per_mat;
+    const int tri = NB * (NB - 1) / 2;
+    // Broadcast slots, double-buffered by step parity: a step reads slot
+    // [k & 1] while the owner of the next row writes slot [(k + 1) & 1], so
+    // no location is read and written by different lanes in the same step.
+    double* dgp = Mx + tri;      // diagonal L[k,k] for step k at dgp[k & 1]
+    double* xsp = Mx + tri + 2;  // x_j of back-substitution step j at xsp[j & 1]
+    int* failp = (int*)(Mx + tri + 4);
 
-        // Upper triangle only (the lower one is never read).
-        for (int j = 0; j < n; ++j)
-            for (int i = tid; i <= j; i += T) S[i + j * ls] = A[i + (size_t)j * lda];
-        if (POSV)
-            for (int i = tid; i < n; i += T) S[i + n * ls] = X[i];
-        if (tid == 0) {
-            s_fail = 0;
-            double d = S[0] - 0.0;
-            if (d > 0) S[0] = __builtin_sqrt(d);
-            else s_fail = 1;
+    for (int round = 0;; ++round) {
+        const int idx0 = (round * (int)gridDim.x + (int)blockIdx.x) * MW;
+        if (idx0 >= *q.count) break;
+        const int b = mfh_item(q, idx0 + grp);
+        const int n = b >= 0 ? B.n[b] : 0;
+        const int N = POSV ? n + 1 : n;
+        int nmax = n;
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            int o = __shfl_xor(nmax, off, MFH_WAVE);
+            nmax = o > nmax ? o : nmax;
         }
-        double sj = 0.0;
+        // Wave-uniform loop bounds keep every step loop a scalar loop.
+        nmax = __builtin_amdgcn_readfirstlane(nmax);
+        double* A = b >= 0 ? B.a[b] : nullptr;
+        const int lda = b >= 0 ? B.lda[b] : 0;
+        double* X = (POSV && b >= 0) ? B.x[b] : nullptr;
+
+        // Column j of the strict upper triangle -> row j (both contiguous).
+        for (int j = 1; j < nmax; ++j) {
+            if (j < n)
+                for (int c = l; c < j; c += G) Mx[rowp(j) + c] = A[c + (size_t)j * lda];
+        }
+        if (POSV)
+            for (int c = l; c < n; c += G) Mx[rowp(n) + c] = X[c];
+        double ajj[S], ss[S], dd[S], lm1[S], lm2[S], bulk[S];
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const int j = l + s * G;
+            ajj[s] = (j < n) ? A[j + (size_t)j * lda] : 0.0;
+            ss[s] = 0.0;
+            dd[s] = 0.0;
+            lm1[s] = 0.0;
+            lm2[s] = 0.0;
+            bulk[s] = 0.0;
+        }
+        if (l == 0) {
+            *failp = 0;
+            if (n > 0) {
+                double d = ajj[0] - 0.0;
+                if (d > 0) {
+                    dd[0] = __builtin_sqrt(d);
+                    dgp[0] = dd[0];
+                } else {
+                    *failp = 1;
+                }
+            }
+        }
         __syncthreads();
-        int fail = s_fail;
+        int fail = (n > 0) ? *(volatile int*)failp : 0;
         const int steps = POSV ? n : n - 1;
-        for (int k = 0; k < steps && !fail; ++k) {
-            const int j = tid;
-            if (j > k && j < rows) {
-                double t = (S[k + j * ls] - mfh_dot_exact(k, S + j, ls, S + k, ls)) / S[k + k * ls];
-                S[j + k * ls] = t;
-                if (j < n) {
-                    sj = sj + t * t;
-                    if (j == k + 1) {
-                        double d = S[j + j * ls] - sj;
-                        if (d > 0) S[j + j * ls] = __builtin_sqrt(d);
-                        else s_fail = j + 1;
+        const int smax = POSV ? nmax : nmax - 1;
+        for (int k = 0; k < smax; ++k) {
+            const bool gact = (k < steps) && (fail == 0);
+            double nbulk[S];
+#pragma unroll
+            for (int s = 0; s < S; ++s) nbulk[s] = bulk[s];
+            // (1) look-ahead: Dot(k+1) for rows j > k+1 up to (not including)
+            // its last pair (k-1, k): columns 0..k-2, all final.  It is
+            // computed first, into nbulk, and only becomes bulk after the
+            // column-k finish below has used the previous step's bulk.
+            const int L1 = k + 1;
+            if (L1 >= 3) {
+                const int kr = rowp(L1);
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    if (s * G + G - 1 <= k + 1) continue;  // slot retired (uniform)
+                    int jr = l + s * G;
+                    jr = jr < k + 2 ? k + 2 : jr;
+                    jr = jr < NB - 1 ? jr : NB - 1;
+                    const int pr = rowp(jr);
+                    double acc;
+                    int c;
+                    if (L1 & 1) {
+                        acc = Mx[pr] * Mx[kr];
+                        c = 1;
+                    } else {
+                        acc = 0.0;
+                        c = 0;
                     }
+#pragma unroll 4
+                    for (; c <= k - 3; c += 2) {
+                        const double p0 = Mx[pr + c] * Mx[kr + c];
+                        const double p1 = Mx[pr + c + 1] * Mx[kr + c + 1];
+                        acc = acc + (p0 + p1);
+                    }
+                    nbulk[s] = acc;
+                }
+            }
+            // (2) column k: finish Dot(k) with its last pair and divide.
+            const double diag = dgp[k & 1];
+            double rk1 = 0.0, rk2 = 0.0;
+            if (k >= 2) {
+                rk2 = Mx[rowp(k) + k - 2];
+                rk1 = Mx[rowp(k) + k - 1];
+            } else if (k == 1) {
+                rk1 = Mx[0];  // L[1,0]
+            }
+            if (gact) {
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    const int j = l + s * G;
+                    if (j > k && j < N) {
+                        double dot;
+                        if (k == 0) dot = 0.0;
+                        else if (k == 1) dot = lm1[s] * rk1;
+                        else {
+                            const double p0 = lm2[s] * rk2;
+                            const double p1 = lm1[s] * rk1;
+                            dot = bulk[s] + (p0 + p1);
+                        }
+                        const double t = (Mx[rowp(j) + k] - dot) / diag;
+                        Mx[rowp(j) + k] = t;
+                        lm2[s] = lm1[s];
+                        lm1[s] = t;
+                        if (j < n) {
+                            ss[s] = ss[s] + t * t;
+                            if (j == k + 1) {
+                                const double d = ajj[s] - ss[s];
+                                if (d > 0) {
+                                    dd[s] = __builtin_sqrt(d);
+                                    dgp[(k + 1) & 1] = dd[s];
+                                } else {
+                                    *failp = j + 1;
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int s = 0; s < S; ++s) bulk[s] = nbulk[s];
+            __syncthreads();
+            fail = (n > 0) ? *(volatile int*)failp : 0;
+        }
+
+        if (b >= 0) {
+            if (fail) {
+                // MatFlat throws at row fail-1 after writing rows 0..fail-2
+                // completely and row fail-1 left of the diagonal.
+                const int jf = fail - 1;
+                for (int c = 0; c < jf; ++c)
+                    for (int i = c + 1 + l; i <= jf; i += G) A[i + (size_t)c * lda] = Mx[rowp(i) + c];
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    const int j = l + s * G;
+                    if (j < jf) A[j + (size_t)j * lda] = dd[s];
+                }
+                if (l == 0) B.info[b] = fail;
+            } else {
+                for (int c = 0; c < n; ++c) {
+#pragma unroll
+                    for (int s = 0; s < S; ++s) {
+                        const int i = l + s * G;
+                        if (i < n) A[i + (size_t)c * lda] = (i < c) ? 0.0 : (i == c ? dd[s] : Mx[rowp(i) + c]);
+                    }
+                }
+                if (l == 0) B.info[b] = 0;
+            }
+        }
+
+        if (POSV) {
+            const bool ok = (b >= 0) && (fail == 0);
+            double xr[S];
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                const int r = l + s * G;
+                xr[s] = (ok && r < n) ? Mx[rowp(n) + r] : 0.0;
+                if (ok && r == n - 1) {
+                    xr[s] = xr[s] / dd[s];
+                    xsp[(n - 1) & 1] = xr[s];
                 }
             }
             __syncthreads();
-            fail = s_fail;
-        }
-
-        if (fail) {
-            // The reference throws at row fail-1 after writing rows
-            // 0..fail-2 completely and row fail-1 left of the diagonal.
-            const int jf = fail - 1;
-            for (int c = 0; c < jf; ++c)
-                for (int i = c + tid; i <= jf; i += T) {
-                    if (i == c && c == jf) continue;
-                    A[i + (size_t)c * lda] = S[i + c * ls];
-                }
-            if (tid == 0) B.info[b] = fail;
-        } else {
-            for (int c = 0; c < n; ++c)
-                for (int i = tid; i < n; i += T) A[i + (size_t)c * lda] = (i < c) ? 0.0 : S[i + c * ls];
-            if (POSV) {
-                double xr = (tid < n) ? S[n + tid * ls] : 0.0;
-                if (tid == n - 1) {
-                    xr = xr / S[(n - 1) + (n - 1) * ls];
-                    xs[n - 1] = xr;
-                }
-                __syncthreads();
-                for (int j = n - 1; j >= 1; --j) {
-                    if (tid < j) {
-                        xr = xr - S[j + tid * ls] * xs[j];
-                        if (tid == j - 1) {
-                            xr = xr / S[tid + tid * ls];
-                            xs[tid] = xr;
+            for (int j = nmax - 1; j >= 1; --j) {
+                if (ok && j < n) {
+                    const double xj = xsp[j & 1];
+                    const int pj = rowp(j);
+#pragma unroll
+                    for (int s = 0; s < S; ++s) {
+                        const int r = l + s * G;
+                        if (r < j) {
+                            xr[s] = xr[s] - Mx[pj + r] * xj;
+                            if (r == j - 1) {
+                                xr[s] = xr[s] / dd[s];
+                                xsp[(j - 1) & 1] = xr[s];
+                            }
                         }
                     }
-                    __syncthreads();
                 }
-                if (tid < n) X[tid] = xr;
+                __syncthreads();
             }
-            if (tid == 0) B.info[b] = 0;
+            if (ok) {
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    const int r = l + s * G;
+                    if (r < n) X[r] = xr[s];
+                }
+            }
         }
         __syncthreads();
     }
 }
 
 // ------------------------------------------------------------ triangular solve
-__global__ void k_bucketize_trsv(MfhTrsvBatch B, int count, BucketWs ws, int max_n)
-{
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= count) return;
-    int n = B.n[i];
-    int bad = 0;
-    if (n < 1 || n > max_n || n > MFH_EXACT_MAX_N) bad = -6;
-    else if (B.a[i] == nullptr) bad = -7;
-    else if (B.lda[i] < n) bad = -8;
-    else if (B.x[i] == nullptr) bad = -9;
-    else if (B.incx[i] < 1) bad = -10;
-    if (bad) {
-        B.info[i] = bad;
-        return;
+struct TrsvKey {
+    MfhTrsvBatch B;
+    int max_n;
+    __device__ int key(int i) const
+    {
+        const int n = B.n[i];
+        int bad = 0;
+        if (n < 1 || n > max_n || n > MFH_EXACT_MAX_N) bad = -6;
+        else if (B.a[i] == nullptr) bad = -7;
+        else if (B.lda[i] < n) bad = -8;
+        else if (B.x[i] == nullptr) bad = -9;
+        else if (B.incx[i] < 1) bad = -10;
+        if (bad) {
+            B.info[i] = bad;
+            return -1;
+        }
+        return n;
     }
-    bucket_push(ws, count, n, i);
-}
+};
 
 // Progressive form of a dot-product substitution: thread i owns unknown i
 // and folds the product with x_c into its running Internals.Dot as soon as
@@ -334,13 +603,12 @@ template <int T>
 __global__ __launch_bounds__(T) void k_trsv_exact(MfhTrsvBatch B, MfhQueue q)
 {
     extern __shared__ double lds[];
-    __shared__ int s_slot;
     const int tid = threadIdx.x;
     const bool upper = B.uplo == MFH_UPPER;
     const bool trans = B.trans == MFH_TRANS;
     const bool unit = B.unit != 0;
-    for (;;) {
-        const int b = mfh_pop(q, &s_slot);
+    for (int it = blockIdx.x;; it += gridDim.x) {
+        const int b = mfh_item(q, it);
         if (b < 0) break;
         const int n = B.n[b];
         const int lda = B.lda[b];
@@ -407,24 +675,26 @@ __global__ __launch_bounds__(T) void k_trsv_exact(MfhTrsvBatch B, MfhQueue q)
 }
 
 // ------------------------------------------------------------ LU solve
-__global__ void k_bucketize_getrs(MfhGetrsBatch B, int count, BucketWs ws, int max_n)
-{
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= count) return;
-    int n = B.n[i];
-    int bad = 0;
-    if (n < 1 || n > max_n || n > MFH_EXACT_MAX_N) bad = -3;
-    else if (B.lu[i] == nullptr) bad = -4;
-    else if (B.lda[i] < n) bad = -5;
-    else if (B.piv[i] == nullptr) bad = -6;
-    else if (B.b[i] == nullptr) bad = -7;
-    else if (B.x[i] == nullptr) bad = -8;
-    if (bad) {
-        B.info[i] = bad;
-        return;
+struct GetrsKey {
+    MfhGetrsBatch B;
+    int max_n;
+    __device__ int key(int i) const
+    {
+        const int n = B.n[i];
+        int bad = 0;
+        if (n < 1 || n > max_n || n > MFH_EXACT_MAX_N) bad = -3;
+        else if (B.lu[i] == nullptr) bad = -4;
+        else if (B.lda[i] < n) bad = -5;
+        else if (B.piv[i] == nullptr) bad = -6;
+        else if (B.b[i] == nullptr) bad = -7;
+        else if (B.x[i] == nullptr) bad = -8;
+        if (bad) {
+            B.info[i] = bad;
+            return -1;
+        }
+        return n;
     }
-    bucket_push(ws, count, n, i);
-}
+};
 
 // Factorization.LuInverse.cs:169-192 for one column: permute, unit-lower
 // dot-form sweep (the reference divides by the 1.0 it stores on the copy's
@@ -433,10 +703,9 @@ template <int T>
 __global__ __launch_bounds__(T) void k_getrs_exact(MfhGetrsBatch B, MfhQueue q)
 {
     extern __shared__ double lds[];
-    __shared__ int s_slot;
     const int tid = threadIdx.x;
-    for (;;) {
-        const int b = mfh_pop(q, &s_slot);
+    for (int it = blockIdx.x;; it += gridDim.x) {
+        const int b = mfh_item(q, it);
         if (b < 0) break;
         const int n = B.n[b];
         const int lda = B.lda[b];
@@ -493,58 +762,65 @@ __global__ __launch_bounds__(T) void k_getrs_exact(MfhGetrsBatch B, MfhQueue q)
     }
 }
 
-template <typename Batch, typename K>
-int run_bucket(mfh_handle* h, K kernel, int threads, int nb, int count, const Batch& B, BucketWs ws, int bucket)
+size_t square_lds_bytes(int nb) { return (size_t)nb * nb * 8 + (size_t)nb * 8 + (size_t)nb * 4 + 64; }
+
+// Per-bucket shape of the packed Cholesky kernel: lanes per matrix G and
+// rows per lane S (G*S >= the bucket's largest row count).
+struct CholShape {
+    int G, S;
+};
+
+CholShape chol_shape(int b)
 {
-    const size_t lds = exact_lds_bytes(nb > MFH_EXACT_MAX_N + 1 ? MFH_EXACT_MAX_N + 1 : nb);
-    MFH_HIP_CHECK(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    int per_cu = 0;
-    MFH_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, lds));
-    if (per_cu < 1) per_cu = 1;
-    long grid = (long)per_cu * h->num_cus;
-    if (grid > count) grid = count;
-    if (grid < 1) grid = 1;
-    MfhQueue q{ws.heads + bucket, ws.counts + bucket, ws.items + (size_t)bucket * count};
-    hipLaunchKernelGGL(kernel, dim3((unsigned)grid), dim3(threads), lds, h->stream, B, q);
-    MFH_HIP_CHECK(hipGetLastError());
-    return MFH_OK;
+    switch (b) {
+    case 0: return {8, 2};
+    case 1: return {16, 2};
+    case 2: return {16, 3};
+    case 3: return {32, 2};
+    case 8: return {64, 3};
+    default: return {64, 2};
+    }
 }
 
-int threads_for(int nb) { return MFH_WAVE * ((nb + MFH_WAVE - 1) / MFH_WAVE); }
-
-int prepare_buckets(mfh_handle* h, int count, BucketWs* ws)
+template <bool POSV>
+int run_chol_bucket(mfh_handle* h, int b, int nb_rows, int count, const MfhCholBatch& B, const SortWs& ws)
 {
-    int st = mfh_ws_reserve(h, bucket_ws_bytes(count));
-    if (st != MFH_OK) return st;
-    *ws = bucket_ws(h, count);
-    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, h->stream, ws->counts, 2 * MFH_NBUCKETS);
-    MFH_HIP_CHECK(hipGetLastError());
-    return MFH_OK;
-}
-
-int nbuckets_for(int max_key)
-{
-    int nb = mfh_bucket_of(max_key) + 1;
-    return nb > MFH_NBUCKETS ? MFH_NBUCKETS : nb;
+    const CholShape sh = chol_shape(b);
+    int per_mat = nb_rows * (nb_rows - 1) / 2 + 5;
+    per_mat = (per_mat + 1) & ~1;
+    const int mw = MFH_WAVE / sh.G;
+    const size_t lds = (size_t)per_mat * 8 * mw;
+    const long max_groups = ((long)count + mw - 1) / mw;
+    const MfhQueue q = bucket_queue(ws, b);
+#define MFH_CHOL_CASE(G_, S_)                                                                                 \
+    if (sh.G == G_ && sh.S == S_)                                                                             \
+        return launch_bucket(h, k_chol_packed<G_, S_, POSV>, MFH_WAVE, lds, max_groups, B, q, nb_rows, per_mat);
+    MFH_CHOL_CASE(8, 2)
+    MFH_CHOL_CASE(16, 2)
+    MFH_CHOL_CASE(16, 3)
+    MFH_CHOL_CASE(32, 2)
+    MFH_CHOL_CASE(64, 2)
+    MFH_CHOL_CASE(64, 3)
+#undef MFH_CHOL_CASE
+    return mfh_fail(MFH_ERR_UNSUPPORTED, "no Cholesky kernel for this bucket shape");
 }
 
 }  // namespace
 
 int mfh_launch_getrf(mfh_handle* h, int count, const MfhGetrfBatch& B, int max_m, int max_n)
 {
-    BucketWs ws;
-    int st = prepare_buckets(h, count, &ws);
+    SortWs ws;
+    int st = device_sort(h, GetrfKey{B, max_m, max_n}, count, &ws);
     if (st) return st;
-    hipLaunchKernelGGL(k_bucketize_getrf, dim3((count + 255) / 256), dim3(256), 0, h->stream, B, count, ws,
-                       max_m, max_n);
-    MFH_HIP_CHECK(hipGetLastError());
     int key = max_m > max_n ? max_m : max_n;
     if (key > MFH_EXACT_MAX_N) key = MFH_EXACT_MAX_N;
     for (int b = 0; b < nbuckets_for(key); ++b) {
-        int nb = kBucketBound(b);
-        int t = threads_for(nb);
-        if (t == 64) st = run_bucket(h, k_getrf_exact<64>, 64, nb, count, B, ws, b);
-        else st = run_bucket(h, k_getrf_exact<128>, 128, nb, count, B, ws, b);
+        const int nb = bucket_bound(b, key);
+        const int t = threads_for(nb);
+        const size_t lds = square_lds_bytes(nb);
+        const MfhQueue q = bucket_queue(ws, b);
+        if (t == 64) st = launch_bucket(h, k_getrf_exact<64>, 64, lds, count, B, q);
+        else st = launch_bucket(h, k_getrf_exact<128>, 128, lds, count, B, q);
         if (st) return st;
     }
     return MFH_OK;
@@ -552,25 +828,15 @@ int mfh_launch_getrf(mfh_handle* h, int count, const MfhGetrfBatch& B, int max_m
 
 int mfh_launch_potrf(mfh_handle* h, int count, const MfhCholBatch& B, int max_n, bool posv)
 {
-    BucketWs ws;
-    int st = prepare_buckets(h, count, &ws);
+    SortWs ws;
+    int st = device_sort(h, CholKey{B, max_n, posv ? 1 : 0}, count, &ws);
     if (st) return st;
-    hipLaunchKernelGGL(k_bucketize_chol, dim3((count + 255) / 256), dim3(256), 0, h->stream, B, count, ws,
-                       max_n, posv ? 1 : 0);
-    MFH_HIP_CHECK(hipGetLastError());
-    int n_cap = max_n > MFH_EXACT_MAX_N ? MFH_EXACT_MAX_N : max_n;
-    int key = posv ? n_cap + 1 : n_cap;
+    const int n_cap = max_n > MFH_EXACT_MAX_N ? MFH_EXACT_MAX_N : max_n;
+    const int key = posv ? n_cap + 1 : n_cap;
     for (int b = 0; b < nbuckets_for(key); ++b) {
-        int nb = kBucketBound(b);
-        int t = threads_for(nb);
-        if (posv) {
-            if (t == 64) st = run_bucket(h, k_potrf_exact<64, true>, 64, nb, count, B, ws, b);
-            else if (t == 128) st = run_bucket(h, k_potrf_exact<128, true>, 128, nb, count, B, ws, b);
-            else st = run_bucket(h, k_potrf_exact<192, true>, 192, nb, count, B, ws, b);
-        } else {
-            if (t == 64) st = run_bucket(h, k_potrf_exact<64, false>, 64, nb, count, B, ws, b);
-            else st = run_bucket(h, k_potrf_exact<128, false>, 128, nb, count, B, ws, b);
-        }
+        const int nb_rows = bucket_bound(b, key) < 2 ? 2 : bucket_bound(b, key);
+        st = posv ? run_chol_bucket<true>(h, b, nb_rows, count, B, ws)
+                  : run_chol_bucket<false>(h, b, nb_rows, count, B, ws);
         if (st) return st;
     }
     return MFH_OK;
@@ -578,17 +844,17 @@ int mfh_launch_potrf(mfh_handle* h, int count, const MfhCholBatch& B, int max_n,
 
 int mfh_launch_trsv(mfh_handle* h, int count, const MfhTrsvBatch& B, int max_n)
 {
-    BucketWs ws;
-    int st = prepare_buckets(h, count, &ws);
+    SortWs ws;
+    int st = device_sort(h, TrsvKey{B, max_n}, count, &ws);
     if (st) return st;
-    hipLaunchKernelGGL(k_bucketize_trsv, dim3((count + 255) / 256), dim3(256), 0, h->stream, B, count, ws, max_n);
-    MFH_HIP_CHECK(hipGetLastError());
-    int key = max_n > MFH_EXACT_MAX_N ? MFH_EXACT_MAX_N : max_n;
+    const int key = max_n > MFH_EXACT_MAX_N ? MFH_EXACT_MAX_N : max_n;
     for (int b = 0; b < nbuckets_for(key); ++b) {
-        int nb = kBucketBound(b);
-        int t = threads_for(nb);
-        if (t == 64) st = run_bucket(h, k_trsv_exact<64>, 64, nb, count, B, ws, b);
-        else st = run_bucket(h, k_trsv_exact<128>, 128, nb, count, B, ws, b);
+        const int nb = bucket_bound(b, key);
+        const int t = threads_for(nb);
+        const size_t lds = square_lds_bytes(nb);
+        const MfhQueue q = bucket_queue(ws, b);
+        if (t == 64) st = launch_bucket(h, k_trsv_exact<64>, 64, lds, count, B, q);
+        else st = launch_bucket(h, k_trsv_exact<128>, 128, lds, count, B, q);
         if (st) return st;
     }
     return MFH_OK;
@@ -596,18 +862,21 @@ int mfh_launch_trsv(mfh_handle* h, int count, const MfhTrsvBatch& B, int max_n)
 
 int mfh_launch_getrs(mfh_handle* h, int count, const MfhGetrsBatch& B, int max_n)
 {
-    BucketWs ws;
-    int st = prepare_buckets(h, count, &ws);
+    SortWs ws;
+    int st = device_sort(h, GetrsKey{B, max_n}, count, &ws);
     if (st) return st;
-    hipLaunchKernelGGL(k_bucketize_getrs, dim3((count + 255) / 256), dim3(256), 0, h->stream, B, count, ws, max_n);
-    MFH_HIP_CHECK(hipGetLastError());
-    int key = max_n > MFH_EXACT_MAX_N ? MFH_EXACT_MAX_N : max_n;
+    const int key = max_n > MFH_EXACT_MAX_N ? MFH_EXACT_MAX_N : max_n;
     for (int b = 0; b < nbuckets_for(key); ++b) {
-        int nb = kBucketBound(b);
-        int t = threads_for(nb);
-        if (t == 64) st = run_bucket(h, k_getrs_exact<64>, 64, nb, count, B, ws, b);
-        else st = run_bucket(h, k_getrs_exact<128>, 128, nb, count, B, ws, b);
+        const int nb = bucket_bound(b, key);
+        const int t = threads_for(nb);
+        const size_t lds = square_lds_bytes(nb);
+        const MfhQueue q = bucket_queue(ws, b);
+        if (t == 64) st = launch_bucket(h, k_getrs_exact<64>, 64, lds, count, B, q);
+        else st = launch_bucket(h, k_getrs_exact<128>, 128, lds, count, B, q);
         if (st) return st;
     }
     return MFH_OK;
 }
+
+// Workspace bytes a batch of count matrices needs (plans reserve this).
+size_t mfh_exact_ws_bytes(int count) { return sort_ws_bytes(count); }

@@ -110,6 +110,9 @@ struct MfhAdjointBatch {
     int* info;
 };
 
+// Device workspace bytes the exact kernels need for a batch of count.
+size_t mfh_exact_ws_bytes(int count);
+
 // Launchers (mfh_exact.hip, mfh_gemm.hip, mfh_adjoint.hip, mfh_mixed.hip).
 int mfh_launch_getrf(mfh_handle* h, int count, const MfhGetrfBatch& B, int max_m, int max_n);
 int mfh_launch_potrf(mfh_handle* h, int count, const MfhCholBatch& B, int max_n, bool posv);
