@@ -108,6 +108,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--count", type=int, default=100_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--math", choices=["fast", "exact"], default="fast",
+                    help="math mode of the headline run (the other mode is timed as a secondary line)")
+    ap.add_argument("--no-secondary", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (matrices are independent: "
@@ -134,33 +137,48 @@ def main() -> int:
     stream = torch.cuda.current_stream()
     handle = hip.Handle()
     handle.bind_stream(stream)
-    plan = hip.DposvPlan(n_t, a_ptrs, lda_t, x_ptrs, info, nmax, handle=handle)
 
-    for _ in range(args.warmup):
-        plan.execute()
-    torch.cuda.synchronize()
-    if int((info != 0).sum().item()) != 0:
-        log("factorization failures during warmup")
-        return 1
-    start = torch.cuda.Event(enable_timing=True)
-    stop = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    start.record(stream)
-    for _ in range(args.steps):
-        plan.execute()
-    stop.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms = start.elapsed_time(stop) / args.steps
-    log(f"{args.steps} steps: {ms:.4f} ms/step (events), wall {wall * 1e3 / args.steps:.4f} ms/step")
+    def run_mode(mode: int, steps: int, warmup: int):
+        handle.set_math_mode(mode)
+        plan = hip.DposvPlan(n_t, a_ptrs, lda_t, x_ptrs, info, nmax, handle=handle)
+        for _ in range(warmup):
+            plan.execute()
+        torch.cuda.synchronize()
+        if int((info != 0).sum().item()) != 0:
+            raise RuntimeError("factorization failures during warmup")
+        start = torch.cuda.Event(enable_timing=True)
+        stop = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        start.record(stream)
+        for _ in range(steps):
+            plan.execute()
+        stop.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        plan.close()
+        return start.elapsed_time(stop) / steps, wall * 1e3 / steps
+
+    main_mode = hip.MATH_FAST if args.math == "fast" else hip.MATH_EXACT
+    ms, wall_ms = run_mode(main_mode, args.steps, args.warmup)
+    log(f"{args.math}: {args.steps} steps: {ms:.4f} ms/step (events), wall {wall_ms:.4f} ms/step")
+    secondary = None
+    if not args.no_secondary:
+        other = hip.MATH_EXACT if main_mode == hip.MATH_FAST else hip.MATH_FAST
+        oms, _ = run_mode(other, max(3, args.steps // 4), 1)
+        oname = "exact" if other == hip.MATH_EXACT else "fast"
+        log(f"{oname}: {oms:.4f} ms/step")
+        secondary = {"math": oname, "value": round(args.count / (oms / 1e3), 1), "unit": "solves/s",
+                     "ms_per_step": round(oms, 5),
+                     "note": ("bit-identical to MatFlat's CPU results" if oname == "exact"
+                              else "FMA / right-looking, backward error O(n eps)")}
 
     value = args.count / (ms / 1e3)
     bytes_step = alg_bytes_posv(ns)
     achieved = bytes_step / (ms / 1e3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_potrf_exact<T,true> bucket launches (whole posv step)",
+                "kernel": "whole SPD-solve step: device size sort + per-bucket persistent kernels",
                 "alg_bytes_per_step": bytes_step}
     cpu = None
     if not args.no_cpu_baseline:
@@ -179,15 +197,16 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded N(0,1); A = B B^T + n I)",
-        "config": {"workload": "batched FP64 Cholesky + forward/back triangular solve (MatFlat order), "
+        "config": {"workload": "batched FP64 Cholesky + forward/back triangular solve, "
                                f"{args.count} SPD matrices, n~U{{{nmin}..{nmax}}}, lda=8*ceil(n/8), one rhs each, "
                                "one hipGraph replay per step",
                    "count": args.count, "n_min": nmin, "n_max": nmax, "seed": seed},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "math": args.math,
+        "secondary": secondary,
     }
     print(json.dumps(out), flush=True)
-    plan.close()
     return 0
 
 

@@ -71,6 +71,18 @@ const char* mfh_last_error(void);
 /* Library version as major*10000 + minor*100 + patch. */
 int mfh_version(void);
 
+/* Math mode of a handle.  MFH_MATH_EXACT (the default) reproduces MatFlat's
+ * floating-point operation order, so results are bit-identical to MatFlat's
+ * CPU results.  MFH_MATH_FAST lets the SPD factorization / solve kernels use
+ * fused multiply-adds and a right-looking, register-blocked schedule; results
+ * then agree with MatFlat to rounding (normwise backward error of order
+ * n*eps) instead of bit for bit.  No reference counterpart: MatFlat has one
+ * (sequential) CPU path. */
+#define MFH_MATH_EXACT 0
+#define MFH_MATH_FAST 1
+int mfh_set_math_mode(mfh_handle* handle, int mode);
+int mfh_get_math_mode(mfh_handle* handle, int* mode);
+
 /* ---- LU with partial pivoting -------------------------------------------
  * Replaces Factorization.Lu(int m, int n, double* a, int lda, int* piv)
  * (Factorization.Lu.cs:122-161, LuCore :296-350) for a batch.  Computes

@@ -22,6 +22,7 @@ NOTRANS, TRANS, CONJTRANS, CONJNOTRANS = 111, 112, 113, 114
 UPPER, LOWER = 121, 122
 NONUNIT, UNIT = 131, 132
 EXACT_MAX_N = 128
+MATH_EXACT, MATH_FAST = 0, 1
 BLOCKED_MAX_N = 512
 
 _i, _d, _p, _i64 = ctypes.c_int, ctypes.c_double, ctypes.c_void_p, ctypes.c_int64
@@ -34,6 +35,8 @@ SIGNATURES = {
     "mfh_get_stream": [_p, _p],
     "mfh_synchronize": [_p],
     "mfh_version": [],
+    "mfh_set_math_mode": [_p, _i],
+    "mfh_get_math_mode": [_p, _p],
     "mfh_dgetrf_vbatched": [_p, _i, _p, _p, _p, _p, _p, _p, _p, _i, _i],
     "mfh_dgetrf_strided_batched": [_p, _i, _i, _i, _p, _i, _i64, _p, _i64, _p, _p],
     "mfh_dpotrf_vbatched": [_p, _i, _p, _p, _p, _p, _i],

@@ -20,7 +20,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmatflat_hip.so")
-SOURCES = ["mfh_api.cpp", "mfh_exact.hip", "mfh_gemm.hip", "mfh_adjoint.hip", "mfh_mixed.hip"]
+SOURCES = ["mfh_api.cpp", "mfh_exact.hip", "mfh_fast.hip", "mfh_gemm.hip", "mfh_adjoint.hip", "mfh_mixed.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [

@@ -83,6 +83,22 @@ int mfh_set_stream(mfh_handle* h, void* stream)
     return MFH_OK;
 }
 
+int mfh_set_math_mode(mfh_handle* h, int mode)
+{
+    MFH_REQUIRE(h != nullptr, 1, "handle must not be null");
+    MFH_REQUIRE(mode == MFH_MATH_EXACT || mode == MFH_MATH_FAST, 2, "Invalid enum value.");
+    h->math_mode = mode;
+    return MFH_OK;
+}
+
+int mfh_get_math_mode(mfh_handle* h, int* mode)
+{
+    MFH_REQUIRE(h != nullptr, 1, "handle must not be null");
+    MFH_REQUIRE(mode != nullptr, 2, "mode must not be null");
+    *mode = h->math_mode;
+    return MFH_OK;
+}
+
 int mfh_get_stream(mfh_handle* h, void** stream)
 {
     MFH_REQUIRE(h != nullptr, 1, "handle must not be null");

@@ -835,8 +835,11 @@ int mfh_launch_potrf(mfh_handle* h, int count, const MfhCholBatch& B, int max_n,
     const int key = posv ? n_cap + 1 : n_cap;
     for (int b = 0; b < nbuckets_for(key); ++b) {
         const int nb_rows = bucket_bound(b, key) < 2 ? 2 : bucket_bound(b, key);
-        st = posv ? run_chol_bucket<true>(h, b, nb_rows, count, B, ws)
-                  : run_chol_bucket<false>(h, b, nb_rows, count, B, ws);
+        if (h->math_mode == MFH_MATH_FAST)
+            st = mfh_launch_chol_fast_bucket(h, B, bucket_queue(ws, b), nb_rows, count, posv);
+        else
+            st = posv ? run_chol_bucket<true>(h, b, nb_rows, count, B, ws)
+                      : run_chol_bucket<false>(h, b, nb_rows, count, B, ws);
         if (st) return st;
     }
     return MFH_OK;

@@ -7,6 +7,7 @@
 #include <string>
 
 #include "../../include/matflat_hip.h"
+#include "mfh_device.h"
 
 // Size buckets of the exact kernels: bucket b holds keys in (16b, 16(b+1)];
 // the key is the number of rows a workgroup handles (n + 1 for the SPD solve).
@@ -20,6 +21,7 @@ struct mfh_handle {
     void* ws = nullptr;  // device workspace
     size_t ws_bytes = 0;
     bool capturing = false;  // plans pre-size the workspace; no allocation while set
+    int math_mode = MFH_MATH_EXACT;
 };
 
 // Thread-local error text behind mfh_last_error().
@@ -112,6 +114,9 @@ struct MfhAdjointBatch {
 
 // Device workspace bytes the exact kernels need for a batch of count.
 size_t mfh_exact_ws_bytes(int count);
+
+// FAST-mode Cholesky / SPD solve for one size bucket (mfh_fast.hip).
+int mfh_launch_chol_fast_bucket(mfh_handle* h, const MfhCholBatch& B, MfhQueue q, int nr, long count, bool posv);
 
 // Launchers (mfh_exact.hip, mfh_gemm.hip, mfh_adjoint.hip, mfh_mixed.hip).
 int mfh_launch_getrf(mfh_handle* h, int count, const MfhGetrfBatch& B, int max_m, int max_n);

@@ -18,7 +18,8 @@ from dataclasses import dataclass
 import torch
 
 from . import _lib
-from ._lib import BLOCKED_MAX_N, EXACT_MAX_N, LOWER, NONUNIT, NOTRANS, TRANS, UNIT, UPPER  # noqa: F401
+from ._lib import (BLOCKED_MAX_N, EXACT_MAX_N, LOWER, MATH_EXACT, MATH_FAST, NONUNIT, NOTRANS,  # noqa: F401
+                   TRANS, UNIT, UPPER)
 
 
 class MatFlatException(Exception):
@@ -73,6 +74,15 @@ class Handle:
     def bind_stream(self, stream: torch.cuda.Stream | None = None) -> None:
         s = stream if stream is not None else torch.cuda.current_stream()
         _check(self.lib.mfh_set_stream(self.h, ctypes.c_void_p(s.cuda_stream)))
+
+    def set_math_mode(self, mode: int) -> None:
+        """MATH_EXACT (bit-identical to MatFlat) or MATH_FAST (FMA, blocked)."""
+        _check(self.lib.mfh_set_math_mode(self.h, int(mode)))
+
+    def math_mode(self) -> int:
+        m = ctypes.c_int()
+        _check(self.lib.mfh_get_math_mode(self.h, ctypes.byref(m)))
+        return m.value
 
     def synchronize(self) -> None:
         _check(self.lib.mfh_synchronize(self.h))

@@ -383,3 +383,50 @@ def test_matflat_signatures_on_device():
     assert c.cpu().tolist() == [7.0, 10.0, 15.0, 22.0]
     with pytest.raises(ArgumentException):
         Blas.MulMatMat(Transpose.NoTrans, Transpose.NoTrans, 2, 2, 2, aa, 2, aa, 2, aa, 2)
+
+
+# ------------------------------------------------------------------ FAST math mode
+@pytest.mark.parametrize("posv", [False, True])
+def test_fast_mode_cholesky_residuals(posv):
+    """FAST mode (FMA, right-looking) agrees with MatFlat to rounding: the
+    dpot01 residual ratio ||L L^T - A||_1 / (n ||A||_1 eps) stays <= 10,
+    L matches the oracle within 1e-12 relative, the strict upper triangle is
+    exactly zero, padding is untouched, and the solve has backward error
+    <= n*eps."""
+    from helpers import backward_error, chol_residual_ratio, view
+    hip = hipmod()
+    ns, ldas = chol_cases(21, 300, nmax=128)
+    ns[:6] = [1, 2, 8, 9, 100, 127]
+    ldas[:6] = [1, 3, 8, 12, 104, 127]
+    bufs = [spd_upper_buf(700 + i, n, l) for i, (n, l) in enumerate(zip(ns, ldas))]
+    xs = [np.random.default_rng(900 + i).standard_normal(n) for i, n in enumerate(ns)]
+    h = hip.Handle()
+    h.bind_stream()
+    h.set_math_mode(hip.MATH_FAST)
+    assert h.math_mode() == hip.MATH_FAST
+    A, X = Packed(bufs), Packed(xs)
+    info = i32(np.full(len(ns), 99))
+    if posv:
+        hip.dposv_vbatched(i32(ns), A.ptrs, i32(ldas), X.ptrs, info, max(ns), handle=h)
+    else:
+        hip.dpotrf_vbatched(i32(ns), A.ptrs, i32(ldas), info, max(ns), handle=h)
+    torch.cuda.synchronize()
+    assert np.all(info.cpu().numpy() == 0)
+    ga, gx = A.all(), X.all()
+    eps = np.finfo(np.float64).eps
+    for i, (n, l) in enumerate(zip(ns, ldas)):
+        full = view(bufs[i], n, n, l)
+        a_sym = np.triu(full) + np.triu(full, 1).T
+        lg = view(ga[i], n, n, l)
+        assert np.all(np.triu(lg, 1) == 0.0)
+        assert chol_residual_ratio(a_sym, lg) <= 10.0, (i, n)
+        want = bufs[i].copy()
+        wx = xs[i].copy()
+        oracle.dposv(n, want, l, wx)
+        lw = view(want, n, n, l)
+        assert np.max(np.abs(lg - lw)) <= 1e-12 * np.max(np.abs(lw)) * max(1, n), (i, n)
+        for j in range(n):
+            assert np.all(np.isnan(ga[i][j * l + n:(j + 1) * l]))
+        if posv:
+            assert backward_error(a_sym, gx[i], xs[i]) <= n * eps, (i, n)
+    h.close()
