@@ -83,6 +83,25 @@ int mfh_version(void);
 int mfh_set_math_mode(mfh_handle* handle, int mode);
 int mfh_get_math_mode(mfh_handle* handle, int* mode);
 
+/* Per-launch kernel timing (a profiling aid; no MatFlat counterpart).  With
+ * timing enabled, every kernel launch of the factorization / solve entry
+ * points on this handle (outside plan capture) is bracketed by HIP events on
+ * the handle's stream.  mfh_get_timing waits for the recorded launches,
+ * returns the first min(capacity, *n_records) of them in launch order --
+ * kind (MFH_KERNEL_*), rows (the size bucket's row bound: the largest
+ * number of rows a matrix of that launch holds, 0 for the size sort) and
+ * elapsed milliseconds -- and clears the record.  *n_records receives the
+ * number recorded. */
+#define MFH_KERNEL_SORT 1
+#define MFH_KERNEL_CHOL_EXACT 2
+#define MFH_KERNEL_CHOL_FAST 3
+#define MFH_KERNEL_GETRF_EXACT 4
+#define MFH_KERNEL_TRSV_EXACT 5
+#define MFH_KERNEL_GETRS_EXACT 6
+int mfh_set_timing(mfh_handle* handle, int enable);
+int mfh_get_timing(mfh_handle* handle, int capacity, int* n_records, int* kind, int* rows,
+                   float* ms);
+
 /* ---- LU with partial pivoting -------------------------------------------
  * Replaces Factorization.Lu(int m, int n, double* a, int lda, int* piv)
  * (Factorization.Lu.cs:122-161, LuCore :296-350) for a batch.  Computes

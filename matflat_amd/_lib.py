@@ -24,6 +24,10 @@ NONUNIT, UNIT = 131, 132
 EXACT_MAX_N = 128
 MATH_EXACT, MATH_FAST = 0, 1
 BLOCKED_MAX_N = 512
+KERNEL_SORT, KERNEL_CHOL_EXACT, KERNEL_CHOL_FAST, KERNEL_GETRF_EXACT, KERNEL_TRSV_EXACT, KERNEL_GETRS_EXACT = \
+    1, 2, 3, 4, 5, 6
+KERNEL_NAMES = {1: "size sort", 2: "k_chol_packed", 3: "k_chol_fast", 4: "k_getrf_exact", 5: "k_trsv_exact",
+                6: "k_getrs_exact"}
 
 _i, _d, _p, _i64 = ctypes.c_int, ctypes.c_double, ctypes.c_void_p, ctypes.c_int64
 
@@ -37,6 +41,8 @@ SIGNATURES = {
     "mfh_version": [],
     "mfh_set_math_mode": [_p, _i],
     "mfh_get_math_mode": [_p, _p],
+    "mfh_set_timing": [_p, _i],
+    "mfh_get_timing": [_p, _i, _p, _p, _p, _p],
     "mfh_dgetrf_vbatched": [_p, _i, _p, _p, _p, _p, _p, _p, _p, _i, _i],
     "mfh_dgetrf_strided_batched": [_p, _i, _i, _i, _p, _i, _i64, _p, _i64, _p, _p],
     "mfh_dpotrf_vbatched": [_p, _i, _p, _p, _p, _p, _i],

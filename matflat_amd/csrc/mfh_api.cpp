@@ -39,6 +39,42 @@ int mfh_ws_reserve(mfh_handle* h, size_t bytes)
     return MFH_OK;
 }
 
+namespace {
+int take_event(mfh_handle* h, hipEvent_t* ev)
+{
+    if (!h->spare_events.empty()) {
+        *ev = h->spare_events.back();
+        h->spare_events.pop_back();
+        return MFH_OK;
+    }
+    MFH_HIP_CHECK(hipEventCreate(ev));
+    return MFH_OK;
+}
+}  // namespace
+
+int mfh_timing_begin(mfh_handle* h, int kind, int rows)
+{
+    if (!h->timing || h->capturing) return MFH_OK;
+    MfhTimedLaunch t{nullptr, nullptr, kind, rows};
+    int st = take_event(h, &t.start);
+    if (st) return st;
+    st = take_event(h, &t.stop);
+    if (st) {
+        h->spare_events.push_back(t.start);
+        return st;
+    }
+    h->timed.push_back(t);
+    MFH_HIP_CHECK(hipEventRecord(t.start, h->stream));
+    return MFH_OK;
+}
+
+int mfh_timing_end(mfh_handle* h)
+{
+    if (!h->timing || h->capturing || h->timed.empty()) return MFH_OK;
+    MFH_HIP_CHECK(hipEventRecord(h->timed.back().stop, h->stream));
+    return MFH_OK;
+}
+
 #define MFH_REQUIRE(cond, pos, msg)                  \
     do {                                             \
         if (!(cond)) return mfh_fail(-(pos), (msg)); \
@@ -72,6 +108,12 @@ int mfh_destroy(mfh_handle* h)
         hipStreamSynchronize(h->stream);
         hipFree(h->ws);
     }
+    for (auto& t : h->timed) {
+        hipEventSynchronize(t.stop);
+        hipEventDestroy(t.start);
+        hipEventDestroy(t.stop);
+    }
+    for (auto ev : h->spare_events) hipEventDestroy(ev);
     delete h;
     return MFH_OK;
 }
@@ -97,6 +139,42 @@ int mfh_get_math_mode(mfh_handle* h, int* mode)
     MFH_REQUIRE(mode != nullptr, 2, "mode must not be null");
     *mode = h->math_mode;
     return MFH_OK;
+}
+
+int mfh_set_timing(mfh_handle* h, int enable)
+{
+    MFH_REQUIRE(h != nullptr, 1, "handle must not be null");
+    MFH_REQUIRE(enable == 0 || enable == 1, 2, "enable must be 0 or 1");
+    h->timing = enable != 0;
+    return MFH_OK;
+}
+
+int mfh_get_timing(mfh_handle* h, int capacity, int* n_records, int* kind, int* rows, float* ms)
+{
+    MFH_REQUIRE(h != nullptr, 1, "handle must not be null");
+    MFH_REQUIRE(capacity >= 0, 2, "capacity must be >= 0");
+    MFH_REQUIRE(n_records != nullptr, 3, "n_records must not be null");
+    MFH_REQUIRE(capacity == 0 || kind != nullptr, 4, "kind must not be null");
+    MFH_REQUIRE(capacity == 0 || rows != nullptr, 5, "rows must not be null");
+    MFH_REQUIRE(capacity == 0 || ms != nullptr, 6, "ms must not be null");
+    *n_records = (int)h->timed.size();
+    int st = MFH_OK;
+    for (size_t i = 0; i < h->timed.size(); ++i) {
+        MfhTimedLaunch& t = h->timed[i];
+        if ((int)i < capacity && st == MFH_OK) {
+            float e = 0.0f;
+            hipError_t he = hipEventSynchronize(t.stop);
+            if (he == hipSuccess) he = hipEventElapsedTime(&e, t.start, t.stop);
+            if (he != hipSuccess) st = mfh_fail(MFH_ERR_HIP, std::string("timing events: ") + hipGetErrorString(he));
+            kind[i] = t.kind;
+            rows[i] = t.rows;
+            ms[i] = e;
+        }
+        h->spare_events.push_back(t.start);
+        h->spare_events.push_back(t.stop);
+    }
+    h->timed.clear();
+    return st;
 }
 
 int mfh_get_stream(mfh_handle* h, void** stream)

@@ -35,7 +35,11 @@ __host__ __device__ inline int bucket_of(int key) { return (key + MFH_BUCKET_STE
 
 // ---------------------------------------------------------------- sorting
 // Workspace (ints): keys[count] items[count] hist[kKeys] cursor[kKeys]
-// bstart[NB] bcount[NB].
+// bstart[NB] bcount[NB] fstart[NF] fcount[NF].  The coarse buckets (16 keys
+// wide) feed the exact kernels, the fine ones (8 keys wide) the FAST kernels,
+// whose register tiles come in steps of 8 rows.
+constexpr int kFine = (kKeys + MFH_FINE_STEP - 1) / MFH_FINE_STEP;
+
 struct SortWs {
     int* keys;
     int* items;
@@ -43,9 +47,14 @@ struct SortWs {
     int* cursor;
     int* bstart;
     int* bcount;
+    int* fstart;
+    int* fcount;
 };
 
-size_t sort_ws_bytes(int count) { return sizeof(int) * (2 * (size_t)count + 2 * kKeys + 2 * MFH_NBUCKETS); }
+size_t sort_ws_bytes(int count)
+{
+    return sizeof(int) * (2 * (size_t)count + 2 * kKeys + 2 * MFH_NBUCKETS + 2 * kFine);
+}
 
 SortWs sort_ws(void* base, int count)
 {
@@ -57,6 +66,8 @@ SortWs sort_ws(void* base, int count)
     w.cursor = w.hist + kKeys;
     w.bstart = w.cursor + kKeys;
     w.bcount = w.bstart + MFH_NBUCKETS;
+    w.fstart = w.bcount + MFH_NBUCKETS;
+    w.fcount = w.fstart + kFine;
     return w;
 }
 
@@ -105,6 +116,14 @@ __global__ void k_sort_offsets(SortWs ws)
         ws.bstart[b] = ws.cursor[hi];
         ws.bcount[b] = c;
     }
+    for (int f = 0; f < kFine; ++f) {
+        const int lo = MFH_FINE_STEP * f;
+        const int hi = lo + MFH_FINE_STEP < kKeys - 1 ? lo + MFH_FINE_STEP : kKeys - 1;
+        int c = 0;
+        for (int k = lo + 1; k <= hi; ++k) c += ws.hist[k];
+        ws.fstart[f] = ws.cursor[hi];
+        ws.fcount[f] = c;
+    }
 }
 
 __global__ __launch_bounds__(256) void k_sort_scatter(int count, SortWs ws)
@@ -131,21 +150,26 @@ int device_sort(mfh_handle* h, const Op& op, int count, SortWs* out)
     if (st) return st;
     SortWs ws = sort_ws(h->ws, count);
     const unsigned blocks = (unsigned)((count + 255) / 256);
+    st = mfh_timing_begin(h, MFH_KERNEL_SORT, 0);
+    if (st) return st;
     hipLaunchKernelGGL(k_sort_reset, dim3(1), dim3(256), 0, h->stream, ws);
     hipLaunchKernelGGL(k_sort_hist<Op>, dim3(blocks), dim3(256), 0, h->stream, op, count, ws);
     hipLaunchKernelGGL(k_sort_offsets, dim3(1), dim3(64), 0, h->stream, ws);
     hipLaunchKernelGGL(k_sort_scatter, dim3(blocks), dim3(256), 0, h->stream, count, ws);
     MFH_HIP_CHECK(hipGetLastError());
+    st = mfh_timing_end(h);
+    if (st) return st;
     *out = ws;
     return MFH_OK;
 }
 
 MfhQueue bucket_queue(const SortWs& ws, int b) { return MfhQueue{ws.bstart + b, ws.bcount + b, ws.items}; }
+MfhQueue fine_queue(const SortWs& ws, int f) { return MfhQueue{ws.fstart + f, ws.fcount + f, ws.items}; }
 
 // Launches a persistent kernel for one bucket: as many workgroups as fit
 // on the device (by occupancy), capped by what the bucket could need.
 template <typename K, typename... Args>
-int launch_bucket(mfh_handle* h, K kernel, int threads, size_t lds, long max_groups, Args... args)
+int launch_bucket(mfh_handle* h, int kind, int rows, K kernel, int threads, size_t lds, long max_groups, Args... args)
 {
     MFH_HIP_CHECK(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     int per_cu = 0;
@@ -154,9 +178,11 @@ int launch_bucket(mfh_handle* h, K kernel, int threads, size_t lds, long max_gro
     long grid = (long)per_cu * h->num_cus;
     if (grid > max_groups) grid = max_groups;
     if (grid < 1) grid = 1;
+    int st = mfh_timing_begin(h, kind, rows);
+    if (st) return st;
     hipLaunchKernelGGL(kernel, dim3((unsigned)grid), dim3(threads), lds, h->stream, args...);
     MFH_HIP_CHECK(hipGetLastError());
-    return MFH_OK;
+    return mfh_timing_end(h);
 }
 
 int threads_for(int rows) { return MFH_WAVE * ((rows + MFH_WAVE - 1) / MFH_WAVE); }
@@ -794,7 +820,7 @@ int run_chol_bucket(mfh_handle* h, int b, int nb_rows, int count, const MfhCholB
     const MfhQueue q = bucket_queue(ws, b);
 #define MFH_CHOL_CASE(G_, S_)                                                                                 \
     if (sh.G == G_ && sh.S == S_)                                                                             \
-        return launch_bucket(h, k_chol_packed<G_, S_, POSV>, MFH_WAVE, lds, max_groups, B, q, nb_rows, per_mat);
+        return launch_bucket(h, MFH_KERNEL_CHOL_EXACT, nb_rows, k_chol_packed<G_, S_, POSV>, MFH_WAVE, lds, max_groups, B, q, nb_rows, per_mat);
     MFH_CHOL_CASE(8, 2)
     MFH_CHOL_CASE(16, 2)
     MFH_CHOL_CASE(16, 3)
@@ -819,8 +845,8 @@ int mfh_launch_getrf(mfh_handle* h, int count, const MfhGetrfBatch& B, int max_m
         const int t = threads_for(nb);
         const size_t lds = square_lds_bytes(nb);
         const MfhQueue q = bucket_queue(ws, b);
-        if (t == 64) st = launch_bucket(h, k_getrf_exact<64>, 64, lds, count, B, q);
-        else st = launch_bucket(h, k_getrf_exact<128>, 128, lds, count, B, q);
+        if (t == 64) st = launch_bucket(h, MFH_KERNEL_GETRF_EXACT, nb, k_getrf_exact<64>, 64, lds, count, B, q);
+        else st = launch_bucket(h, MFH_KERNEL_GETRF_EXACT, nb, k_getrf_exact<128>, 128, lds, count, B, q);
         if (st) return st;
     }
     return MFH_OK;
@@ -833,13 +859,18 @@ int mfh_launch_potrf(mfh_handle* h, int count, const MfhCholBatch& B, int max_n,
     if (st) return st;
     const int n_cap = max_n > MFH_EXACT_MAX_N ? MFH_EXACT_MAX_N : max_n;
     const int key = posv ? n_cap + 1 : n_cap;
+    if (h->math_mode == MFH_MATH_FAST) {
+        // fine bucket f holds row counts (8f, 8f + 8]: register tile of f + 1 row blocks
+        for (int f = 0; f < (key + MFH_FINE_STEP - 1) / MFH_FINE_STEP; ++f) {
+            st = mfh_launch_chol_fast_bucket(h, B, fine_queue(ws, f), f + 1, count, posv);
+            if (st) return st;
+        }
+        return MFH_OK;
+    }
     for (int b = 0; b < nbuckets_for(key); ++b) {
         const int nb_rows = bucket_bound(b, key) < 2 ? 2 : bucket_bound(b, key);
-        if (h->math_mode == MFH_MATH_FAST)
-            st = mfh_launch_chol_fast_bucket(h, B, bucket_queue(ws, b), nb_rows, count, posv);
-        else
-            st = posv ? run_chol_bucket<true>(h, b, nb_rows, count, B, ws)
-                      : run_chol_bucket<false>(h, b, nb_rows, count, B, ws);
+        st = posv ? run_chol_bucket<true>(h, b, nb_rows, count, B, ws)
+                  : run_chol_bucket<false>(h, b, nb_rows, count, B, ws);
         if (st) return st;
     }
     return MFH_OK;
@@ -856,8 +887,8 @@ int mfh_launch_trsv(mfh_handle* h, int count, const MfhTrsvBatch& B, int max_n)
         const int t = threads_for(nb);
         const size_t lds = square_lds_bytes(nb);
         const MfhQueue q = bucket_queue(ws, b);
-        if (t == 64) st = launch_bucket(h, k_trsv_exact<64>, 64, lds, count, B, q);
-        else st = launch_bucket(h, k_trsv_exact<128>, 128, lds, count, B, q);
+        if (t == 64) st = launch_bucket(h, MFH_KERNEL_TRSV_EXACT, nb, k_trsv_exact<64>, 64, lds, count, B, q);
+        else st = launch_bucket(h, MFH_KERNEL_TRSV_EXACT, nb, k_trsv_exact<128>, 128, lds, count, B, q);
         if (st) return st;
     }
     return MFH_OK;
@@ -874,8 +905,8 @@ int mfh_launch_getrs(mfh_handle* h, int count, const MfhGetrsBatch& B, int max_n
         const int t = threads_for(nb);
         const size_t lds = square_lds_bytes(nb);
         const MfhQueue q = bucket_queue(ws, b);
-        if (t == 64) st = launch_bucket(h, k_getrs_exact<64>, 64, lds, count, B, q);
-        else st = launch_bucket(h, k_getrs_exact<128>, 128, lds, count, B, q);
+        if (t == 64) st = launch_bucket(h, MFH_KERNEL_GETRS_EXACT, nb, k_getrs_exact<64>, 64, lds, count, B, q);
+        else st = launch_bucket(h, MFH_KERNEL_GETRS_EXACT, nb, k_getrs_exact<128>, 128, lds, count, B, q);
         if (st) return st;
     }
     return MFH_OK;

@@ -7,25 +7,30 @@
 // results match MatFlat to rounding (backward error of order n*eps) and are
 // checked against the oracle by residual in the tests.
 //
-// Design (one wavefront per matrix, n <= 135 rows):
-//  * right-looking Cholesky: at step k the column is finalised
-//    (L[k,k] = sqrt(a_kk), L[i,k] = a_ik / L[k,k]) and the whole trailing
-//    lower triangle takes the rank-1 update a_ij -= L[i,k] L[j,k] with FMA;
+// Design (one wavefront per matrix, at most 8*RB rows, RB <= 17):
+//  * right-looking Cholesky: at step k the whole trailing lower triangle
+//    takes the rank-1 update a_ij -= L[i,k] L[j,k] with FMA, and column k+1
+//    is finalised first (look-ahead) so its square root and its publication
+//    overlap the rest of the update;
 //  * the lower triangle lives in VGPRs, block-cyclic over an 8 x 8 lane
-//    grid: lane (lr, lc) owns elements (lr + 8a, lc + 8b), a >= b, so every
-//    update reads RB + CB column values from LDS for up to RB*(RB+1)/2 FMAs
-//    and the lanes stay busy until the trailing triangle is a few blocks;
-//  * LDS holds only the current column, the diagonal and the solve vector
-//    (a few KB), so occupancy is set by VGPRs, not by the matrix size;
+//    grid: lane (lr, lc) owns elements (lr + 8a, lc + 8b), a >= b.  The
+//    accumulators keep column k unscaled (a_ik, not a_ik / L[k,k]); only
+//    the published copy is scaled, and one pass at the end scales the
+//    registers, so a step carries no per-element selects;
+//  * a finished column is published in LDS in a permuted order (element i
+//    at (i % 8) * RBP + i / 8), so each lane reads the RB row values and RB
+//    column values it needs as 16-byte pairs, double-buffered by step
+//    parity so a step needs one barrier;
+//  * the diagonal element is broadcast with readlane (k is wave-uniform)
+//    and 1/sqrt comes from the hardware estimate plus two Newton steps;
 //  * the right-hand side of the SPD solve is carried as an extra row n of
-//    the triangle (its "L[n,k]" is the forward substitution), and the
-//    back substitution runs as a bottom-up column sweep from LDS;
+//    the triangle (its "L[n,k]" is the forward substitution), and the back
+//    substitution runs as a bottom-up column sweep through LDS;
 //  * loads of the upper triangle and stores of L (and of the zeroed strict
 //    upper triangle) touch each HBM byte once, eight contiguous doubles per
 //    lane group.
 #include "mfh_device.h"
 #include "mfh_internal.h"
-
 
 #pragma clang fp contract(fast)
 
@@ -37,66 +42,98 @@ struct Tri {
     static constexpr int idx(int a, int b) { return a * (a + 1) / 2 + b; }
 };
 
-// LDS per matrix (doubles): col[2][NR] diag[NR] rdiag[NR] y[NR] x[NR]
-// junk[64] + flags.  Lanes that must not store write to junk[lane], so
-// conditional stores stay branch-free.
-__host__ __device__ constexpr int fast_lds_doubles(int nr) { return 6 * nr + 66; }
+// Row stride of a permuted column buffer (even: pairs stay 16-byte aligned).
+template <int RB>
+constexpr int rbp() { return (RB + 1) & ~1; }
 
-// Per-wave state of one factorization, shared by the block steps below.
+// LDS per workgroup (doubles): col[2][8 * RBP], dgl[8RB], rdg[8RB],
+// yv[8RB], xv[8RB], one flag.
+template <int RB>
+constexpr int fast_lds_doubles() { return 16 * rbp<RB>() + 32 * RB + 2; }
+
+// Global-memory pointer: loads and stores through it use the global (not
+// flat) instructions, with a wave-uniform base and 32-bit lane offsets.
+typedef __attribute__((address_space(1))) double* GPtr;
+
 struct FastCtx {
-    double* colb;
-    double* dgl;
-    double* rdg;
-    double* junk;
+    double* col;  // two permuted column buffers
+    double* dgl;  // L[k,k]
+    double* rdg;  // 1 / L[k,k]
     int* failp;
-    int lane, lr, lc, n, N, NR;
+    int lane, lr, lc, n;
 };
 
-// Finalise column k, which lies in block column KB: the diagonal owner's
-// value (lane (k%8, k%8)) is broadcast, the 8 lanes with lc == k%8 scale
-// their column-k elements and publish them in colb[k & 1].  Returns false
-// when the diagonal is not positive.
-template <int RB, int KB>
-__device__ __forceinline__ bool fast_finalize(double (&acc)[Tri<RB>::size], const FastCtx& c, int k)
+// 1/sqrt(d) for a positive normal d: hardware estimate, then two Newton
+// steps y <- y + y (1 - d y^2) / 2 (each squares the relative error).
+__device__ __forceinline__ double fast_rsqrt(double d)
 {
-    const int src = (k & 7) * 9;  // lane (k%8) + 8*(k%8)
-    const double d = __shfl(acc[Tri<RB>::idx(KB, KB)], src, MFH_WAVE);
-    if (!(d > 0)) {
+    double y = __builtin_amdgcn_rsq(d);
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+        const double e = __builtin_fma(-(d * y), y, 1.0);
+        y = __builtin_fma(0.5 * y, e, y);
+    }
+    return y;
+}
+
+// v[a] for a >= KB from row r of a permuted column buffer.
+template <int RB, int KB>
+__device__ __forceinline__ void load_col(const double* cp, int r, double (&v)[RB])
+{
+    const double2* p = reinterpret_cast<const double2*>(cp + r * rbp<RB>());
+#pragma unroll
+    for (int a = KB & ~1; a < RB; a += 2) {
+        const double2 t = p[a / 2];
+        if (a >= KB) v[a] = t.x;
+        if (a + 1 < RB) v[a + 1] = t.y;
+    }
+}
+
+// Finalise column k (block column KB): broadcast the diagonal a_kk from
+// lane (k%8, k%8), record L[k,k] and its reciprocal, and let the 8 lanes
+// with lc == k%8 publish L[i,k] = a_ik / L[k,k] for their rows into cn.
+// Returns false when a_kk is not positive.
+template <int RB, int KB>
+__device__ __forceinline__ bool fast_finalize(const double (&acc)[Tri<RB>::size], const FastCtx& c, int k,
+                                              double* cn)
+{
+    const int src = (k & 7) * 9;
+    const double dv = acc[Tri<RB>::idx(KB, KB)];
+    const int lo = __builtin_amdgcn_readlane(__double2loint(dv), src);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(dv), src);
+    const double d = __hiloint2double(hi, lo);
+    if (!(d > 0.0)) {
         if (c.lane == 0) *c.failp = k + 1;
         return false;
     }
-    const double lkk = __builtin_sqrt(d);
-    const double rk = 1.0 / lkk;
+    const double y = fast_rsqrt(d);
     if (c.lane == 0) {
-        c.dgl[k] = lkk;
-        c.rdg[k] = rk;
+        c.dgl[k] = d * y;
+        c.rdg[k] = y;
     }
-    double* ck = c.colb + (k & 1) * c.NR;
-    const bool owner = c.lc == (k & 7);
+    if (c.lc == (k & 7)) {
+        double2* p = reinterpret_cast<double2*>(cn + c.lr * rbp<RB>());
 #pragma unroll
-    for (int a = KB; a < RB; ++a) {
-        const int i = c.lr + 8 * a;
-        const bool ok = owner && i > k && i < c.N;
-        const double t = acc[Tri<RB>::idx(a, KB)] * rk;
-        acc[Tri<RB>::idx(a, KB)] = ok ? t : acc[Tri<RB>::idx(a, KB)];
-        *(ok ? ck + i : c.junk + c.lane) = t;
+        for (int a = KB & ~1; a < RB; a += 2) {
+            const double x0 = (a >= KB) ? acc[Tri<RB>::idx(a >= KB ? a : KB, KB)] * y : 0.0;
+            const double x1 = (a + 1 < RB) ? acc[Tri<RB>::idx(a + 1 < RB ? a + 1 : KB, KB)] * y : 0.0;
+            p[a / 2] = make_double2(x0, x1);
+        }
     }
     return true;
 }
 
-// Rank-1 update with column k (block column KB) of every block (a, bb),
-// a >= bb >= KB, except block column SKIP (already updated by the look-ahead;
-// SKIP = -1 for none).
+// Rank-1 update of every block (a, bb), a >= bb >= KB, except block column
+// SKIP (already updated for the look-ahead; -1 for none).
 template <int RB, int KB, int SKIP>
 __device__ __forceinline__ void fast_update(double (&acc)[Tri<RB>::size], const double (&rv)[RB], const double (&cv)[RB])
 {
 #pragma unroll
-    for (int a = KB; a < RB; ++a)
+    for (int bb = KB; bb < RB; ++bb) {
+        if (bb == SKIP) continue;
 #pragma unroll
-        for (int bb = KB; bb <= a; ++bb) {
-            if (bb == SKIP) continue;
-            acc[Tri<RB>::idx(a, bb)] -= rv[a] * cv[bb];
-        }
+        for (int a = bb; a < RB; ++a) acc[Tri<RB>::idx(a, bb)] -= rv[a] * cv[bb];
+    }
 }
 
 template <int RB, int COL>
@@ -107,42 +144,32 @@ __device__ __forceinline__ void fast_update_col(double (&acc)[Tri<RB>::size], co
 }
 
 // Steps k = 8*KB .. min(8*KB + 8, n) - 1, then the next block column.
+// Column k was published into buffer k & 1 by the previous step.
 template <int RB, int KB>
 __device__ __forceinline__ bool fast_block_steps(double (&acc)[Tri<RB>::size], const FastCtx& c)
 {
+    constexpr int CB = 8 * rbp<RB>();
     const int kend = (8 * KB + 8 < c.n) ? 8 * KB + 8 : c.n;
     for (int k = 8 * KB; k < kend; ++k) {
-        const double* ck = c.colb + (k & 1) * c.NR;
+        const double* cp = c.col + (k & 1) * CB;
+        double* cn = c.col + ((k + 1) & 1) * CB;
         double rv[RB], cv[RB];
-#pragma unroll
-        for (int a = 0; a < RB; ++a) {
-            if (a < KB) {
-                rv[a] = 0.0;
-                cv[a] = 0.0;
-                continue;
-            }
-            // Rows need no mask: an element (i, j) with i <= k and j > k is
-            // in the (unused) upper part of a diagonal block, and rows i >= N
-            // are never stored.  Columns j <= k are finished and must not
-            // change, so their factor is zero.
-            const int i = c.lr + 8 * a;
-            rv[a] = ck[i < c.N ? i : c.N - 1];
-            const int j = c.lc + 8 * a;
-            const double y = ck[j < c.N ? j : c.N - 1];
-            cv[a] = (j > k) ? y : 0.0;
-        }
+        load_col<RB, KB>(cp, c.lr, rv);
+        load_col<RB, KB>(cp, c.lc, cv);
+        // Rows need no mask: an element (i, j) with i <= k < j lies in the
+        // unused upper part of a diagonal block, and rows beyond the matrix
+        // are never stored.  Finished columns j <= k must not change.
+        if (c.lc + 8 * KB <= k) cv[KB] = 0.0;
         bool ok = true;
-        // Look-ahead: finish column k+1 first (its block column is KB, or
-        // KB+1 at the last step of the block), then the rest of the update.
         if (k + 1 < c.n) {
             if ((k & 7) != 7) {
                 fast_update_col<RB, KB>(acc, rv, cv);
-                ok = fast_finalize<RB, KB>(acc, c, k + 1);
+                ok = fast_finalize<RB, KB>(acc, c, k + 1, cn);
                 fast_update<RB, KB, KB>(acc, rv, cv);
             } else {
                 if constexpr (KB + 1 < RB) {
                     fast_update_col<RB, KB + 1>(acc, rv, cv);
-                    ok = fast_finalize<RB, KB + 1>(acc, c, k + 1);
+                    ok = fast_finalize<RB, KB + 1>(acc, c, k + 1, cn);
                     fast_update<RB, KB, KB + 1>(acc, rv, cv);
                 }
             }
@@ -158,7 +185,9 @@ __device__ __forceinline__ bool fast_block_steps(double (&acc)[Tri<RB>::size], c
     return true;
 }
 
-// Back substitution steps j = 8*KB+7 .. 8*KB (descending), then block KB-1.
+// Back substitution L^T x = y, columns j = min(8*KB+7, n-1) .. 8*KB
+// (descending), then block KB-1: x_j = y_j / L[j,j], then y_r -= L[j,r] x_j
+// for r < j by the 8 lanes that hold row j.
 template <int RB, int KB>
 __device__ __forceinline__ void fast_back_steps(const double (&acc)[Tri<RB>::size], const FastCtx& c, double* yv,
                                                 double* xv)
@@ -167,14 +196,12 @@ __device__ __forceinline__ void fast_back_steps(const double (&acc)[Tri<RB>::siz
     for (int j = jhi; j >= 8 * KB; --j) {
         const double xj = yv[j] * c.rdg[j];
         if (c.lane == 0) xv[j] = xj;
-        const bool mine = c.lr == (j & 7);
+        if (c.lr == (j & 7)) {
 #pragma unroll
-        for (int bb = 0; bb <= KB; ++bb) {
-            const int r = c.lc + 8 * bb;
-            const bool ok = mine && r < j;
-            const int rc = r < j ? r : 0;
-            const double nv = yv[rc] - acc[Tri<RB>::idx(KB, bb)] * xj;
-            *(ok ? yv + r : c.junk + c.lane) = nv;
+            for (int bb = 0; bb <= KB; ++bb) {
+                const int r = c.lc + 8 * bb;
+                if (r < j) yv[r] -= acc[Tri<RB>::idx(KB, bb)] * xj;
+            }
         }
         __syncthreads();
     }
@@ -194,23 +221,22 @@ __device__ __forceinline__ void fast_back_from(const double (&acc)[Tri<RB>::size
 }
 
 template <int RB, bool POSV>
-__global__ __launch_bounds__(64) void k_chol_fast(MfhCholBatch B, MfhQueue q, int NR)
+__global__ __launch_bounds__(64) void k_chol_fast(MfhCholBatch B, MfhQueue q)
 {
     extern __shared__ double lds[];
+    constexpr int NR = 8 * RB;
+    constexpr int T = Tri<RB>::size;
     FastCtx c;
-    c.colb = lds;  // column buffers, double-buffered by step parity
-    c.dgl = lds + 2 * NR;
-    c.rdg = lds + 3 * NR;
-    double* yv = lds + 4 * NR;
-    double* xv = lds + 5 * NR;
-    c.junk = lds + 6 * NR;
-    c.failp = (int*)(lds + 6 * NR + 64);
+    c.col = lds;
+    c.dgl = lds + 16 * rbp<RB>();
+    c.rdg = c.dgl + NR;
+    double* yv = c.rdg + NR;
+    double* xv = yv + NR;
+    c.failp = (int*)(xv + NR);
     c.lane = threadIdx.x;
     c.lr = c.lane & 7;
     c.lc = c.lane >> 3;
-    c.NR = NR;
     const int lr = c.lr, lc = c.lc, lane = c.lane;
-    constexpr int T = Tri<RB>::size;
 
     for (int it = blockIdx.x;; it += gridDim.x) {
         // One matrix per wavefront: its id and sizes are wave-uniform, which
@@ -218,16 +244,13 @@ __global__ __launch_bounds__(64) void k_chol_fast(MfhCholBatch B, MfhQueue q, in
         const int b = __builtin_amdgcn_readfirstlane(mfh_item(q, it));
         if (b < 0) break;
         const int n = __builtin_amdgcn_readfirstlane(B.n[b]);
-        const int N = POSV ? n + 1 : n;  // rows of the triangle
         const int lda = __builtin_amdgcn_readfirstlane(B.lda[b]);
         c.n = n;
-        c.N = N;
-        double* A = B.a[b];
-        const double* X = POSV ? B.x[b] : nullptr;
+        GPtr A = (GPtr)B.a[b];
 
-        // Load element (i, j), i >= j: the upper-triangle entry A[j + i*lda];
-        // row n of the SPD solve holds b.  Addresses are clamped into the
-        // matrix and the value selected, so the loads are unconditional.
+        // Load element (i, j), i >= j: the upper-triangle entry A[j + i*lda].
+        // Offsets are clamped into the matrix so every load is unconditional
+        // (32-bit offsets from a wave-uniform base), and the value selected.
         double acc[T];
 #pragma unroll
         for (int a = 0; a < RB; ++a) {
@@ -237,39 +260,65 @@ __global__ __launch_bounds__(64) void k_chol_fast(MfhCholBatch B, MfhQueue q, in
             for (int bb = 0; bb <= a; ++bb) {
                 const int j = lc + 8 * bb;
                 const int jc = j < ic ? j : ic;
-                double v = A[jc + (size_t)ic * lda];
-                v = (i < n && j <= i) ? v : 0.0;
-                if (POSV && a == (n >> 3)) {
-                    const double xb = X[j < n ? j : n - 1];
-                    v = (i == n && j < n) ? xb : v;
+                const double v = A[(unsigned)(jc + ic * lda)];
+                acc[Tri<RB>::idx(a, bb)] = (i < n && j <= i) ? v : 0.0;
+            }
+        }
+        if (POSV) {
+            // Row n of the triangle holds b (lanes with lr == n % 8, block n / 8).
+            GPtr X = (GPtr)B.x[b];
+            double xb[RB];
+#pragma unroll
+            for (int bb = 0; bb < RB; ++bb) {
+                const int j = lc + 8 * bb;
+                xb[bb] = X[(unsigned)(j < n ? j : n - 1)];
+            }
+            const bool rown = lr == (n & 7);
+#pragma unroll
+            for (int a = 0; a < RB; ++a) {
+                if (a != (n >> 3)) continue;
+#pragma unroll
+                for (int bb = 0; bb <= a; ++bb) {
+                    const int j = lc + 8 * bb;
+                    acc[Tri<RB>::idx(a, bb)] = (rown && j < n) ? xb[bb] : acc[Tri<RB>::idx(a, bb)];
                 }
-                acc[Tri<RB>::idx(a, bb)] = v;
             }
         }
         if (lane == 0) *c.failp = 0;
-        bool ok = fast_finalize<RB, 0>(acc, c, 0);
+        bool ok = fast_finalize<RB, 0>(acc, c, 0, c.col);
         __syncthreads();
         if (ok) ok = fast_block_steps<RB, 0>(acc, c);
         const int fail = __builtin_amdgcn_readfirstlane(*(volatile int*)c.failp);
 
         if (fail) {
+            // FAST mode leaves a matrix that is not positive definite unchanged.
             if (lane == 0) B.info[b] = fail;
         } else {
+            // Scale the columns: a_ij / L[j,j] (the diagonal is stored from dgl).
+#pragma unroll
+            for (int bb = 0; bb < RB; ++bb) {
+                const double r = c.rdg[lc + 8 * bb];
+#pragma unroll
+                for (int a = bb; a < RB; ++a) acc[Tri<RB>::idx(a, bb)] *= r;
+            }
             // Store L, the diagonal and the zeroed strict upper triangle: in
             // a diagonal block each lane writes its own (i, j); off-diagonal
-            // blocks write L at (i, j) and 0 at (j, i).
+            // blocks write L at (i, j) and 0 at (j, i).  One exec-masked
+            // region per block row.
 #pragma unroll
             for (int a = 0; a < RB; ++a) {
                 const int i = lr + 8 * a;
+                if (i < n) {
 #pragma unroll
-                for (int bb = 0; bb <= a; ++bb) {
-                    const int j = lc + 8 * bb;
-                    const double v = acc[Tri<RB>::idx(a, bb)];
-                    if (bb == a) {
-                        if (i < n && j < n) A[i + (size_t)j * lda] = (i > j) ? v : (i == j ? c.dgl[i] : 0.0);
-                    } else if (i < n) {
-                        A[i + (size_t)j * lda] = v;
-                        A[j + (size_t)i * lda] = 0.0;
+                    for (int bb = 0; bb < a; ++bb) {
+                        const int j = lc + 8 * bb;
+                        A[(unsigned)(i + j * lda)] = acc[Tri<RB>::idx(a, bb)];
+                        A[(unsigned)(j + i * lda)] = 0.0;
+                    }
+                    const int j = lc + 8 * a;
+                    if (j < n) {
+                        const double v = acc[Tri<RB>::idx(a, a)];
+                        A[(unsigned)(i + j * lda)] = (i > j) ? v : (i == j ? c.dgl[i] : 0.0);
                     }
                 }
             }
@@ -277,19 +326,20 @@ __global__ __launch_bounds__(64) void k_chol_fast(MfhCholBatch B, MfhQueue q, in
             if (POSV) {
                 // y = row n of the triangle, then the Lower/Trans sweep
                 // (Blas.SolveTriangular.cs:224-231 order) with x in xv.
+                if (lr == (n & 7)) {
 #pragma unroll
-                for (int a = 0; a < RB; ++a) {
-                    if (a != (n >> 3)) continue;
+                    for (int a = 0; a < RB; ++a) {
+                        if (a != (n >> 3)) continue;
 #pragma unroll
-                    for (int bb = 0; bb <= a; ++bb) {
-                        const int j = lc + 8 * bb;
-                        const bool okw = lr == (n & 7) && j < n;
-                        *(okw ? yv + j : c.junk + lane) = acc[Tri<RB>::idx(a, bb)];
+                        for (int bb = 0; bb <= a; ++bb) {
+                            const int j = lc + 8 * bb;
+                            if (j < n) yv[j] = acc[Tri<RB>::idx(a, bb)];
+                        }
                     }
                 }
                 __syncthreads();
                 fast_back_from<RB, RB - 1>(acc, c, yv, xv, (n - 1) >> 3);
-                double* Xo = B.x[b];
+                GPtr Xo = (GPtr)B.x[b];
                 for (int r = lane; r < n; r += 64) Xo[r] = xv[r];
             }
         }
@@ -298,49 +348,50 @@ __global__ __launch_bounds__(64) void k_chol_fast(MfhCholBatch B, MfhQueue q, in
 }
 
 template <int RB, bool POSV>
-int launch_fast(mfh_handle* h, const MfhCholBatch& B, MfhQueue q, int nr, long count)
+int launch_fast(mfh_handle* h, const MfhCholBatch& B, MfhQueue q, long count)
 {
-    const size_t lds = (size_t)fast_lds_doubles(nr) * 8;
+    const size_t lds = (size_t)fast_lds_doubles<RB>() * 8;
     int per_cu = 0;
     MFH_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_chol_fast<RB, POSV>, 64, lds));
     if (per_cu < 1) per_cu = 1;
     long grid = (long)per_cu * h->num_cus;
     if (grid > count) grid = count;
     if (grid < 1) grid = 1;
-    hipLaunchKernelGGL((k_chol_fast<RB, POSV>), dim3((unsigned)grid), dim3(64), lds, h->stream, B, q, nr);
+    int st = mfh_timing_begin(h, MFH_KERNEL_CHOL_FAST, 8 * RB);
+    if (st) return st;
+    hipLaunchKernelGGL((k_chol_fast<RB, POSV>), dim3((unsigned)grid), dim3(64), lds, h->stream, B, q);
     MFH_HIP_CHECK(hipGetLastError());
-    return MFH_OK;
+    return mfh_timing_end(h);
 }
 
 template <bool POSV>
-int dispatch_fast(mfh_handle* h, const MfhCholBatch& B, MfhQueue q, int nr, long count)
+int dispatch_fast(mfh_handle* h, const MfhCholBatch& B, MfhQueue q, int rb, long count)
 {
-    const int rb = (nr + 7) / 8;
     switch (rb) {
-    case 1: return launch_fast<1, POSV>(h, B, q, nr, count);
-    case 2: return launch_fast<2, POSV>(h, B, q, nr, count);
-    case 3: return launch_fast<3, POSV>(h, B, q, nr, count);
-    case 4: return launch_fast<4, POSV>(h, B, q, nr, count);
-    case 5: return launch_fast<5, POSV>(h, B, q, nr, count);
-    case 6: return launch_fast<6, POSV>(h, B, q, nr, count);
-    case 7: return launch_fast<7, POSV>(h, B, q, nr, count);
-    case 8: return launch_fast<8, POSV>(h, B, q, nr, count);
-    case 9: return launch_fast<9, POSV>(h, B, q, nr, count);
-    case 10: return launch_fast<10, POSV>(h, B, q, nr, count);
-    case 11: return launch_fast<11, POSV>(h, B, q, nr, count);
-    case 12: return launch_fast<12, POSV>(h, B, q, nr, count);
-    case 13: return launch_fast<13, POSV>(h, B, q, nr, count);
-    case 14: return launch_fast<14, POSV>(h, B, q, nr, count);
-    case 15: return launch_fast<15, POSV>(h, B, q, nr, count);
-    case 16: return launch_fast<16, POSV>(h, B, q, nr, count);
-    case 17: return launch_fast<17, POSV>(h, B, q, nr, count);
+    case 1: return launch_fast<1, POSV>(h, B, q, count);
+    case 2: return launch_fast<2, POSV>(h, B, q, count);
+    case 3: return launch_fast<3, POSV>(h, B, q, count);
+    case 4: return launch_fast<4, POSV>(h, B, q, count);
+    case 5: return launch_fast<5, POSV>(h, B, q, count);
+    case 6: return launch_fast<6, POSV>(h, B, q, count);
+    case 7: return launch_fast<7, POSV>(h, B, q, count);
+    case 8: return launch_fast<8, POSV>(h, B, q, count);
+    case 9: return launch_fast<9, POSV>(h, B, q, count);
+    case 10: return launch_fast<10, POSV>(h, B, q, count);
+    case 11: return launch_fast<11, POSV>(h, B, q, count);
+    case 12: return launch_fast<12, POSV>(h, B, q, count);
+    case 13: return launch_fast<13, POSV>(h, B, q, count);
+    case 14: return launch_fast<14, POSV>(h, B, q, count);
+    case 15: return launch_fast<15, POSV>(h, B, q, count);
+    case 16: return launch_fast<16, POSV>(h, B, q, count);
+    case 17: return launch_fast<17, POSV>(h, B, q, count);
     default: return mfh_fail(MFH_ERR_UNSUPPORTED, "fast Cholesky: too many rows");
     }
 }
 
 }  // namespace
 
-int mfh_launch_chol_fast_bucket(mfh_handle* h, const MfhCholBatch& B, MfhQueue q, int nr, long count, bool posv)
+int mfh_launch_chol_fast_bucket(mfh_handle* h, const MfhCholBatch& B, MfhQueue q, int rb, long count, bool posv)
 {
-    return posv ? dispatch_fast<true>(h, B, q, nr, count) : dispatch_fast<false>(h, B, q, nr, count);
+    return posv ? dispatch_fast<true>(h, B, q, rb, count) : dispatch_fast<false>(h, B, q, rb, count);
 }

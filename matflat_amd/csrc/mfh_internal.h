@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include <string>
+#include <vector>
 
 #include "../../include/matflat_hip.h"
 #include "mfh_device.h"
@@ -13,6 +14,14 @@
 // the key is the number of rows a workgroup handles (n + 1 for the SPD solve).
 #define MFH_NBUCKETS 9
 #define MFH_BUCKET_STEP 16
+// The FAST kernels use finer buckets: (8f, 8f + 8] rows.
+#define MFH_FINE_STEP 8
+
+// One kernel launch bracketed by events (mfh_set_timing).
+struct MfhTimedLaunch {
+    hipEvent_t start, stop;
+    int kind, rows;
+};
 
 struct mfh_handle {
     hipStream_t stream = nullptr;
@@ -22,6 +31,9 @@ struct mfh_handle {
     size_t ws_bytes = 0;
     bool capturing = false;  // plans pre-size the workspace; no allocation while set
     int math_mode = MFH_MATH_EXACT;
+    bool timing = false;                  // bracket launches with events
+    std::vector<MfhTimedLaunch> timed;    // recorded launches not yet read
+    std::vector<hipEvent_t> spare_events; // reusable events
 };
 
 // Thread-local error text behind mfh_last_error().
@@ -30,6 +42,12 @@ int mfh_fail(int status, const std::string& msg);
 
 // Grows the handle workspace to at least bytes (device memory).
 int mfh_ws_reserve(mfh_handle* h, size_t bytes);
+
+// Per-launch timing: when h->timing is set (and no plan is being captured),
+// mfh_timing_begin records a start event of a new record on the handle's
+// stream and mfh_timing_end the stop event of the newest record.
+int mfh_timing_begin(mfh_handle* h, int kind, int rows);
+int mfh_timing_end(mfh_handle* h);
 
 #define MFH_HIP_CHECK(expr)                                                          \
     do {                                                                             \
@@ -115,8 +133,9 @@ struct MfhAdjointBatch {
 // Device workspace bytes the exact kernels need for a batch of count.
 size_t mfh_exact_ws_bytes(int count);
 
-// FAST-mode Cholesky / SPD solve for one size bucket (mfh_fast.hip).
-int mfh_launch_chol_fast_bucket(mfh_handle* h, const MfhCholBatch& B, MfhQueue q, int nr, long count, bool posv);
+// FAST-mode Cholesky / SPD solve for one fine size bucket (mfh_fast.hip):
+// every matrix of the queue has at most 8 * rb rows (n, or n + 1 for posv).
+int mfh_launch_chol_fast_bucket(mfh_handle* h, const MfhCholBatch& B, MfhQueue q, int rb, long count, bool posv);
 
 // Launchers (mfh_exact.hip, mfh_gemm.hip, mfh_adjoint.hip, mfh_mixed.hip).
 int mfh_launch_getrf(mfh_handle* h, int count, const MfhGetrfBatch& B, int max_m, int max_n);

@@ -84,6 +84,20 @@ class Handle:
         _check(self.lib.mfh_get_math_mode(self.h, ctypes.byref(m)))
         return m.value
 
+    def set_timing(self, enable: bool) -> None:
+        """Bracket every kernel launch on this handle with HIP events."""
+        _check(self.lib.mfh_set_timing(self.h, 1 if enable else 0))
+
+    def timing(self) -> list[tuple[int, int, float]]:
+        """Waits for and returns the timed launches as (kind, rows, ms), then clears them."""
+        cap = 4096
+        n = ctypes.c_int()
+        kind = (ctypes.c_int * cap)()
+        rows = (ctypes.c_int * cap)()
+        ms = (ctypes.c_float * cap)()
+        _check(self.lib.mfh_get_timing(self.h, cap, ctypes.byref(n), kind, rows, ms))
+        return [(kind[i], rows[i], ms[i]) for i in range(min(cap, n.value))]
+
     def synchronize(self) -> None:
         _check(self.lib.mfh_synchronize(self.h))
 
