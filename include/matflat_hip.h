@@ -83,6 +83,14 @@ int mfh_version(void);
 int mfh_set_math_mode(mfh_handle* handle, int mode);
 int mfh_get_math_mode(mfh_handle* handle, int* mode);
 
+/* Tuning of the FAST Cholesky / SPD-solve kernels (no MatFlat counterpart):
+ * matrices of up to 8 * row_blocks rows (n, or n + 1 for the SPD solve;
+ * row_blocks 1..17) run on `waves` wavefronts per matrix: 1 or 2, or 0 for
+ * the library's built-in table.  matflat_amd/autotune.py measures both and
+ * prints the table. */
+int mfh_set_fast_waves(mfh_handle* handle, int row_blocks, int waves);
+int mfh_get_fast_waves(mfh_handle* handle, int row_blocks, int* waves);
+
 /* Per-launch kernel timing (a profiling aid; no MatFlat counterpart).  With
  * timing enabled, every kernel launch of the factorization / solve entry
  * points on this handle (outside plan capture) is bracketed by HIP events on

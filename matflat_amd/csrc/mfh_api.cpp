@@ -133,6 +133,25 @@ int mfh_set_math_mode(mfh_handle* h, int mode)
     return MFH_OK;
 }
 
+int mfh_set_fast_waves(mfh_handle* h, int row_blocks, int waves)
+{
+    MFH_REQUIRE(h != nullptr, 1, "handle must not be null");
+    MFH_REQUIRE(row_blocks >= 1 && row_blocks <= 17, 2, "row_blocks must be in [1, 17]");
+    MFH_REQUIRE(waves >= 0 && waves <= 2 && !(waves == 2 && row_blocks < 2), 3,
+                "waves must be 0 (default), 1 or 2 (2 needs row_blocks >= 2)");
+    h->fast_waves[row_blocks] = waves;
+    return MFH_OK;
+}
+
+int mfh_get_fast_waves(mfh_handle* h, int row_blocks, int* waves)
+{
+    MFH_REQUIRE(h != nullptr, 1, "handle must not be null");
+    MFH_REQUIRE(row_blocks >= 1 && row_blocks <= 17, 2, "row_blocks must be in [1, 17]");
+    MFH_REQUIRE(waves != nullptr, 3, "waves must not be null");
+    *waves = h->fast_waves[row_blocks] > 0 ? h->fast_waves[row_blocks] : mfh_fast_default_waves(row_blocks);
+    return MFH_OK;
+}
+
 int mfh_get_math_mode(mfh_handle* h, int* mode)
 {
     MFH_REQUIRE(h != nullptr, 1, "handle must not be null");

@@ -327,7 +327,7 @@ struct CholKey {
         int bad = 0;
         if (n < 1 || n > max_n || n > MFH_EXACT_MAX_N) bad = -3;
         else if (B.a[i] == nullptr) bad = -4;
-        else if (B.lda[i] < n) bad = -5;
+        else if (B.lda[i] < n || (long long)B.lda[i] * n > MFH_MAX_MATRIX_ELEMS) bad = -5;
         else if (posv && B.x[i] == nullptr) bad = -6;
         if (bad) {
             B.info[i] = bad;

@@ -14,6 +14,10 @@
 // the key is the number of rows a workgroup handles (n + 1 for the SPD solve).
 #define MFH_NBUCKETS 9
 #define MFH_BUCKET_STEP 16
+// Largest lda * n of one matrix (2 GiB of doubles): kernels address a matrix
+// with 32-bit byte offsets from its base pointer.
+#define MFH_MAX_MATRIX_ELEMS (1LL << 28)
+
 // The FAST kernels use finer buckets: (8f, 8f + 8] rows.
 #define MFH_FINE_STEP 8
 
@@ -31,6 +35,7 @@ struct mfh_handle {
     size_t ws_bytes = 0;
     bool capturing = false;  // plans pre-size the workspace; no allocation while set
     int math_mode = MFH_MATH_EXACT;
+    int fast_waves[18] = {};              // FAST Cholesky waves per matrix by row blocks (0 = default)
     bool timing = false;                  // bracket launches with events
     std::vector<MfhTimedLaunch> timed;    // recorded launches not yet read
     std::vector<hipEvent_t> spare_events; // reusable events
@@ -132,6 +137,10 @@ struct MfhAdjointBatch {
 
 // Device workspace bytes the exact kernels need for a batch of count.
 size_t mfh_exact_ws_bytes(int count);
+
+// Built-in choice of wavefronts per matrix (1 or 2) of the FAST Cholesky
+// kernel for matrices of up to 8 * rb rows.
+int mfh_fast_default_waves(int rb);
 
 // FAST-mode Cholesky / SPD solve for one fine size bucket (mfh_fast.hip):
 // every matrix of the queue has at most 8 * rb rows (n, or n + 1 for posv).

@@ -84,6 +84,15 @@ class Handle:
         _check(self.lib.mfh_get_math_mode(self.h, ctypes.byref(m)))
         return m.value
 
+    def set_fast_waves(self, row_blocks: int, waves: int) -> None:
+        """FAST Cholesky: wavefronts per matrix (1, 2; 0 = built-in) for up to 8*row_blocks rows."""
+        _check(self.lib.mfh_set_fast_waves(self.h, int(row_blocks), int(waves)))
+
+    def fast_waves(self, row_blocks: int) -> int:
+        w = ctypes.c_int()
+        _check(self.lib.mfh_get_fast_waves(self.h, int(row_blocks), ctypes.byref(w)))
+        return w.value
+
     def set_timing(self, enable: bool) -> None:
         """Bracket every kernel launch on this handle with HIP events."""
         _check(self.lib.mfh_set_timing(self.h, 1 if enable else 0))

@@ -386,8 +386,9 @@ def test_matflat_signatures_on_device():
 
 
 # ------------------------------------------------------------------ FAST math mode
+@pytest.mark.parametrize("waves", [0, 1, 2])
 @pytest.mark.parametrize("posv", [False, True])
-def test_fast_mode_cholesky_residuals(posv):
+def test_fast_mode_cholesky_residuals(posv, waves):
     """FAST mode (FMA, right-looking) agrees with MatFlat to rounding: the
     dpot01 residual ratio ||L L^T - A||_1 / (n ||A||_1 eps) stays <= 10,
     L matches the oracle within 1e-12 relative, the strict upper triangle is
@@ -404,6 +405,10 @@ def test_fast_mode_cholesky_residuals(posv):
     h.bind_stream()
     h.set_math_mode(hip.MATH_FAST)
     assert h.math_mode() == hip.MATH_FAST
+    if waves:
+        for rb in range(2, 18):
+            h.set_fast_waves(rb, waves)
+            assert h.fast_waves(rb) == waves
     A, X = Packed(bufs), Packed(xs)
     info = i32(np.full(len(ns), 99))
     if posv:
@@ -430,3 +435,39 @@ def test_fast_mode_cholesky_residuals(posv):
         if posv:
             assert backward_error(a_sym, gx[i], xs[i]) <= n * eps, (i, n)
     h.close()
+
+
+@pytest.mark.parametrize("waves", [1, 2])
+def test_fast_mode_not_positive_definite(waves):
+    """FAST mode reports the failing pivot like MatFlat (info = j+1 for a
+    clearly indefinite leading block) and leaves such a matrix unchanged;
+    its neighbours in the batch are factored normally."""
+    hip = hipmod()
+    ns = [5, 40, 70, 90, 12]
+    bad = {1: 17, 3: 60}  # matrix index -> row whose diagonal is made negative
+    bufs = []
+    for i, n in enumerate(ns):
+        b = spd_upper_buf(1300 + i, n, n)
+        if i in bad:
+            j = bad[i]
+            b[j + j * n] = -1.0e3 * n
+        bufs.append(b)
+    h = hip.Handle()
+    h.bind_stream()
+    h.set_math_mode(hip.MATH_FAST)
+    for rb in range(2, 18):
+        h.set_fast_waves(rb, waves)
+    A = Packed(bufs)
+    info = i32(np.full(len(ns), 99))
+    hip.dpotrf_vbatched(i32(ns), A.ptrs, i32(ns), info, max(ns), handle=h)
+    torch.cuda.synchronize()
+    got = info.cpu().numpy()
+    ga = A.all()
+    for i, n in enumerate(ns):
+        if i in bad:
+            assert got[i] == bad[i] + 1, (i, got)
+            assert_bits_equal(ga[i], bufs[i])
+        else:
+            assert got[i] == 0
+    h.close()
+
