@@ -106,6 +106,8 @@ int mfh_get_fast_waves(mfh_handle* handle, int row_blocks, int* waves);
 #define MFH_KERNEL_GETRF_EXACT 4
 #define MFH_KERNEL_TRSV_EXACT 5
 #define MFH_KERNEL_GETRS_EXACT 6
+#define MFH_KERNEL_GESV_MIXED 7
+#define MFH_KERNEL_GESV_F64 8
 int mfh_set_timing(mfh_handle* handle, int enable);
 int mfh_get_timing(mfh_handle* handle, int capacity, int* n_records, int* kind, int* rows,
                    float* ms);

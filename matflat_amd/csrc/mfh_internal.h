@@ -146,6 +146,9 @@ int mfh_fast_default_waves(int rb);
 // every matrix of the queue has at most 8 * rb rows (n, or n + 1 for posv).
 int mfh_launch_chol_fast_bucket(mfh_handle* h, const MfhCholBatch& B, MfhQueue q, int rb, long count, bool posv);
 
+// Workspace bytes of the blocked LU solvers (mfh_mixed.hip).
+size_t mfh_gesv_ws_bytes(int count, int max_n, bool mixed);
+
 // Launchers (mfh_exact.hip, mfh_gemm.hip, mfh_adjoint.hip, mfh_mixed.hip).
 int mfh_launch_getrf(mfh_handle* h, int count, const MfhGetrfBatch& B, int max_m, int max_n);
 int mfh_launch_potrf(mfh_handle* h, int count, const MfhCholBatch& B, int max_n, bool posv);

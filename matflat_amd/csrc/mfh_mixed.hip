@@ -1,13 +1,795 @@
-// mfh_mixed.hip -- blocked LU for 128 <= n <= 512: FP32/BF16 factorization
-// with FP64 iterative refinement, and the plain FP64 comparison path.
+// mfh_mixed.hip -- batched LU solve for n <= 512: the mixed-precision solver
+// (FP32 factorization, BF16-split MFMA trailing updates for n >= 256, FP64
+// iterative refinement) and the plain FP64 blocked solver it is compared
+// with.  Neither has a single MatFlat counterpart: they compose
+// Factorization.Lu (Factorization.Lu.cs:122-161) with the LU solve of
+// Factorization.LuInverse.cs:169-192, and are judged by backward error, not
+// bit-identity.
+//
+// One workgroup of 512 threads (8 wavefronts) per matrix, persistent over the
+// batch through an atomic work counter:
+//  * A (FP64, column-major) is copied into a row-major workspace W in the
+//    factorization precision T, through an LDS-tiled transpose;
+//  * right-looking blocked LU with partial pivoting, panel width NB (32 for
+//    FP32, 16 for FP64): the panel lives in registers (one row per thread),
+//    each column takes one block-wide argmax and one row exchange through
+//    LDS; the pivots are applied to the other columns by row swaps (W is
+//    row-major, so a swap moves two contiguous rows); U12 = L11^-1 A12 by
+//    column-parallel forward substitution, kept transposed in LDS; the
+//    trailing update A22 -= L21 U12 runs on MFMA:
+//      - FP64: v_mfma_f64_16x16x4f64, 16x16 output tiles, K = 16 in 4 steps;
+//      - FP32, n >= 256: v_mfma_f32_16x16x32_bf16 on a three-term BF16
+//        split of both operands (x = x1 + x2 + x3), six products summed
+//        smallest first, FP32 accumulation: FP32-level accuracy from BF16
+//        matrix cores;
+//      - FP32, n < 256: FP32 FMA on the vector units;
+//  * the solve applies the row permutation, then blocked forward / backward
+//    substitution (64-row blocks: a block-parallel GEMV, then one wavefront
+//    solving the diagonal block with register-resident rows);
+//  * mixed precision: x0 = (LU)^-1 P b, then while the normwise backward
+//    error eta = ||b - A x||_inf / (||A||_inf ||x||_inf + ||b||_inf) exceeds
+//    tol_per_n * n and fewer than max_iter corrections were made,
+//    x += (LU)^-1 P (b - A x) -- residual, update and test in FP64, reading
+//    the original FP64 A.
 #include "mfh_device.h"
 #include "mfh_internal.h"
+
+namespace {
+
+constexpr int BT = 512;  // threads per workgroup
+constexpr int NWAVE = BT / 64;
+
+typedef float float4_t __attribute__((ext_vector_type(4)));
+typedef double double4_t __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+
+template <typename T>
+struct Prec;
+// NB: panel width; BS: triangular-solve block (rows held in registers by
+// one wavefront during the diagonal-block solve).
+template <>
+struct Prec<float> {
+    static constexpr int NB = 32;
+    static constexpr int BS = 64;
+};
+template <>
+struct Prec<double> {
+    static constexpr int NB = 16;
+    static constexpr int BS = 32;
+};
+
+struct GesvArgs {
+    const int* n;
+    const double* const* a;
+    const int* lda;
+    const double* const* b;
+    double* const* x;
+    int* iters;  // mixed only
+    double* berr;
+    int* info;
+    int count;
+    int max_n;
+    int max_iter;
+    double tol_per_n;
+    char* ws;          // per-matrix workspaces, wstride bytes apart
+    long long wstride;
+    int ldw;           // row stride of W (elements)
+    int* counter;      // work counter (zeroed before launch)
+};
+
+// LDS layout (bytes), for row stride L = ldw:
+//   ut    [L][NB] T       U12 transposed (k contiguous); also the copy tile
+//   l11   [NB][NB] T
+//   prow, jrow [NB] T     pivot row / row j during an exchange
+//   xv, rv [L] double     solution, residual / right-hand side
+//   yv    [L] T           solve vector
+//   perm  [L] int         row permutation (position -> original row)
+//   pivs  [NB] int
+//   red   [NWAVE] (double key, int idx), scal [8] double
+template <typename T>
+struct Lds {
+    T* ut;
+    T* l11;
+    T* prow;
+    T* jrow;
+    double* xv;
+    double* rv;
+    T* yv;
+    int* perm;
+    int* pivs;
+    double* rkey;
+    int* ridx;
+    double* scal;
+    int* flag;
+};
+
+// The ut region also holds the 32 x 33 staging tile of the input copy.
+__host__ __device__ inline size_t ut_elems(int ldw, int nb)
+{
+    const size_t e = (size_t)ldw * nb;
+    return e > 32 * 33 ? e : 32 * 33;
+}
+
+template <typename T>
+__host__ __device__ size_t gesv_lds_bytes(int ldw)
+{
+    constexpr int NB = Prec<T>::NB;
+    size_t s = 0;
+    s += ut_elems(ldw, NB) * sizeof(T);
+    s += (size_t)NB * NB * sizeof(T);
+    s += 2 * (size_t)NB * sizeof(T);
+    s = (s + 15) & ~(size_t)15;
+    s += 2 * (size_t)ldw * sizeof(double);
+    s += (size_t)ldw * sizeof(T);
+    s = (s + 15) & ~(size_t)15;
+    s += (size_t)ldw * sizeof(int);
+    s += (size_t)NB * sizeof(int);
+    s = (s + 15) & ~(size_t)15;
+    s += NWAVE * sizeof(double) + NWAVE * sizeof(int) + 8 * sizeof(double) + 16;
+    return s;
+}
+
+template <typename T>
+__device__ __forceinline__ Lds<T> carve(char* base, int ldw)
+{
+    constexpr int NB = Prec<T>::NB;
+    Lds<T> L;
+    size_t s = 0;
+    L.ut = (T*)(base + s);
+    s += ut_elems(ldw, NB) * sizeof(T);
+    L.l11 = (T*)(base + s);
+    s += (size_t)NB * NB * sizeof(T);
+    L.prow = (T*)(base + s);
+    s += NB * sizeof(T);
+    L.jrow = (T*)(base + s);
+    s += NB * sizeof(T);
+    s = (s + 15) & ~(size_t)15;
+    L.xv = (double*)(base + s);
+    s += (size_t)ldw * sizeof(double);
+    L.rv = (double*)(base + s);
+    s += (size_t)ldw * sizeof(double);
+    L.yv = (T*)(base + s);
+    s += (size_t)ldw * sizeof(T);
+    s = (s + 15) & ~(size_t)15;
+    L.perm = (int*)(base + s);
+    s += (size_t)ldw * sizeof(int);
+    L.pivs = (int*)(base + s);
+    s += NB * sizeof(int);
+    s = (s + 15) & ~(size_t)15;
+    L.rkey = (double*)(base + s);
+    s += NWAVE * sizeof(double);
+    L.scal = (double*)(base + s);
+    s += 8 * sizeof(double);
+    L.ridx = (int*)(base + s);
+    s += NWAVE * sizeof(int);
+    L.flag = (int*)(base + s);
+    return L;
+}
+
+__device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
+
+// A value every lane already holds, made provably uniform for the compiler:
+// loops and branches that contain barriers must depend only on such values
+// (a branch the compiler believes divergent may be restructured so that the
+// lanes of one wavefront reach the barriers in different orders).
+__device__ __forceinline__ double uniform(double v)
+{
+    const int lo = __builtin_amdgcn_readfirstlane(__double2loint(v));
+    const int hi = __builtin_amdgcn_readfirstlane(__double2hiint(v));
+    return __hiloint2double(hi, lo);
+}
+
+// Block-wide max of a non-negative double (all threads get the result).
+template <typename T>
+__device__ __forceinline__ double block_max(const Lds<T>& L, double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(v, off, 64);
+        v = o > v ? o : v;
+    }
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) L.rkey[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double m = L.rkey[0];
+#pragma unroll
+    for (int w = 1; w < NWAVE; ++w) m = L.rkey[w] > m ? L.rkey[w] : m;
+    return uniform(m);
+}
+
+// ------------------------------------------------------------------- copy
+// W[r][c] = (T) A[r + c*lda], through 32 x 32 LDS tiles (both sides coalesced).
+template <typename T>
+__device__ __forceinline__ void copy_in(const double* A, int lda, T* W, int ldw, int n, T* tile)
+{
+    const int t = threadIdx.x;
+    const int nt = (n + 31) / 32;
+    for (int tt = 0; tt < nt * nt; ++tt) {
+        const int r0 = (tt % nt) * 32, c0 = (tt / nt) * 32;
+        // read: 32 rows of 16 columns per pass
+        {
+            const int rr = t & 31, cc = t >> 5;  // cc in 0..15
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int r = r0 + rr, c = c0 + cc + 16 * h;
+                T v = 0;
+                if (r < n && c < n) v = (T)A[r + (size_t)c * lda];
+                tile[rr * 33 + cc + 16 * h] = v;
+            }
+        }
+        __syncthreads();
+        {
+            const int cc = t & 31, rr = t >> 5;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int r = r0 + rr + 16 * h, c = c0 + cc;
+                if (r < n && c < n) W[(size_t)r * ldw + c] = tile[(rr + 16 * h) * 33 + cc];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// --------------------------------------------------------------- panel
+// Factor columns k0 .. k0+nb-1 of rows k0 .. n-1 (one row per thread, in
+// registers), record the pivots in pivs[] and perm, write the panel back,
+// and leave the unit-lower L11 in l11.  Returns the first zero pivot
+// (1-based) or 0.
+template <typename T>
+__device__ __forceinline__ int panel(T* W, int ldw, int n, int k0, int nb, const Lds<T>& L)
+{
+    constexpr int NB = Prec<T>::NB;
+    const int t = threadIdx.x;
+    const int r = k0 + t;
+    const bool mine = r < n;
+    T P[NB];
+    {
+        // clamped addresses, selected values: branch-free loads
+        const int rc = mine ? r : n - 1;
+#pragma unroll
+        for (int c = 0; c < NB; ++c) {
+            const T v = W[(size_t)rc * ldw + (k0 + c < n ? k0 + c : n - 1)];
+            P[c] = (mine && c < nb) ? v : (T)0;
+        }
+    }
+    int zero = 0;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        if (j >= nb) break;
+        // pivot: largest |a_rj| over r >= k0+j, lowest row on ties, NaN last
+        double key = (mine && r >= k0 + j) ? mfh_pivot_key((double)P[j]) : -2.0;
+        int idx = r;
+        mfh_wave_argmax(key, idx);
+        if ((t & 63) == 0) {
+            L.rkey[t >> 6] = key;
+            L.ridx[t >> 6] = idx;
+        }
+        __syncthreads();
+        double bk = L.rkey[0];
+        int p = L.ridx[0];
+#pragma unroll
+        for (int w = 1; w < NWAVE; ++w) mfh_pivot_combine(bk, p, L.rkey[w], L.ridx[w]);
+        if (bk < -1.5) p = k0 + j;  // unreachable (row k0+j exists); keeps p in range
+        // exchange rows k0+j and p
+        if (r == p) {
+#pragma unroll
+            for (int c = 0; c < NB; ++c) L.prow[c] = P[c];
+        }
+        if (r == k0 + j) {
+#pragma unroll
+            for (int c = 0; c < NB; ++c) L.jrow[c] = P[c];
+        }
+        if (t == 0) L.pivs[j] = p;
+        __syncthreads();
+        if (r == p && p != k0 + j) {
+#pragma unroll
+            for (int c = 0; c < NB; ++c) P[c] = L.jrow[c];
+        }
+        if (r == k0 + j) {
+#pragma unroll
+            for (int c = 0; c < NB; ++c) P[c] = L.prow[c];
+        }
+        const T d = L.prow[j];
+        if (d == (T)0 && zero == 0) zero = k0 + j + 1;
+        if (mine && r > k0 + j) {
+            const T l = (d != (T)0) ? P[j] / d : P[j];
+            P[j] = l;
+#pragma unroll
+            for (int c = j + 1; c < NB; ++c) P[c] -= l * L.prow[c];
+        }
+    }
+    if (mine) {
+        if (nb == NB) {
+#pragma unroll
+            for (int c = 0; c < NB; ++c) W[(size_t)r * ldw + k0 + c] = P[c];
+        } else {
+            for (int c = 0; c < nb; ++c) {
+                T v = P[0];
+#pragma unroll
+                for (int q = 1; q < NB; ++q) v = (c == q) ? P[q] : v;
+                W[(size_t)r * ldw + k0 + c] = v;
+            }
+        }
+    }
+    if (t < nb) {
+#pragma unroll
+        for (int c = 0; c < NB; ++c) L.l11[t * NB + c] = P[c];
+    }
+    __syncthreads();
+    return zero;
+}
+
+// Apply the panel's row swaps to the columns outside it (each thread owns
+// columns, so the swap sequence needs no barrier), and to perm (thread 0).
+template <typename T>
+__device__ __forceinline__ void swap_rows(T* W, int ldw, int n, int k0, int nb, const Lds<T>& L)
+{
+    const int t = threadIdx.x;
+    for (int c = t; c < n; c += BT) {
+        if (c >= k0 && c < k0 + nb) continue;
+        for (int j = 0; j < nb; ++j) {
+            const int p = L.pivs[j];
+            if (p != k0 + j) {
+                T* a = W + (size_t)(k0 + j) * ldw + c;
+                T* b = W + (size_t)p * ldw + c;
+                const T va = *a, vb = *b;
+                *a = vb;
+                *b = va;
+            }
+        }
+    }
+    if (t == 0) {
+        for (int j = 0; j < nb; ++j) {
+            const int p = L.pivs[j];
+            const int v = L.perm[k0 + j];
+            L.perm[k0 + j] = L.perm[p];
+            L.perm[p] = v;
+        }
+    }
+    __syncthreads();
+}
+
+// U12 = L11^-1 A12 (columns k0+NB .. n-1), written to W and, transposed, to
+// ut.  Only full panels have a trailing part, so nb == NB here and every load
+// and store is unconditional (per-row conditions would split the code into
+// many blocks and defeat register allocation).
+template <typename T>
+__device__ __forceinline__ void trsm_u12(T* W, int ldw, int n, int k0, const Lds<T>& L)
+{
+    constexpr int NB = Prec<T>::NB;
+    const int c0 = k0 + NB;
+    // n <= BT: one column per thread
+    const int c = c0 + (int)threadIdx.x;
+    if (c < n) {
+        T u[NB];
+#pragma unroll
+        for (int i = 0; i < NB; ++i) u[i] = W[(size_t)(k0 + i) * ldw + c];
+        // column-oriented substitution; the compiler fence (which takes the
+        // L11 address) keeps the loads of one column in flight at a time
+#pragma unroll
+        for (int j = 0; j < NB - 1; ++j) {
+#pragma unroll
+            for (int i = j + 1; i < NB; ++i) u[i] -= L.l11[i * NB + j] * u[j];
+            asm volatile("" ::"v"(L.l11) : "memory");
+        }
+#pragma unroll
+        for (int i = 0; i < NB; ++i) {
+            W[(size_t)(k0 + i) * ldw + c] = u[i];
+            L.ut[(size_t)(c - c0) * NB + i] = u[i];
+        }
+    }
+    __syncthreads();
+}
+
+// ------------------------------------------------------- trailing updates
+// A22 -= L21 U12 over 16 x 16 output tiles, one wavefront per tile.
+__device__ __forceinline__ void trailing_f64(double* W, int ldw, int n, int k0, const Lds<double>& L)
+{
+    constexpr int NB = Prec<double>::NB;
+    const int R0 = k0 + NB;
+    const int m = n - R0;
+    const int nt = (m + 15) / 16;
+    const int lane = threadIdx.x & 63, i = lane & 15, kq = lane >> 4;
+    for (int tile = wave_id(); tile < nt * nt; tile += NWAVE) {
+        const int row0 = R0 + (tile / nt) * 16, col0 = R0 + (tile % nt) * 16;
+        const int ar = row0 + i, bc = col0 + i;
+        // lane supplies k = 4*kq + s to MFMA step s, for both operands
+        double av[4], bv[4];
+        if (ar < n) {
+            const double4_t t4 = *(const double4_t*)(W + (size_t)ar * ldw + k0 + 4 * kq);
+#pragma unroll
+            for (int s = 0; s < 4; ++s) av[s] = t4[s];
+        } else {
+#pragma unroll
+            for (int s = 0; s < 4; ++s) av[s] = 0.0;
+        }
+        if (bc < n) {
+            const double4_t t4 = *(const double4_t*)(L.ut + (size_t)(bc - R0) * NB + 4 * kq);
+#pragma unroll
+            for (int s = 0; s < 4; ++s) bv[s] = t4[s];
+        } else {
+#pragma unroll
+            for (int s = 0; s < 4; ++s) bv[s] = 0.0;
+        }
+        double4_t acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int s = 0; s < 4; ++s) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[s], bv[s], acc, 0, 0, 0);
+        // item r of lane l is row (l/16) + 4r, column l%16
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = row0 + kq + 4 * r, col = col0 + i;
+            if (row < n && col < n) W[(size_t)row * ldw + col] -= acc[r];
+        }
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ void split3(float x, __bf16& h1, __bf16& h2, __bf16& h3)
+{
+    h1 = (__bf16)x;
+    const float r1 = x - (float)h1;
+    h2 = (__bf16)r1;
+    const float r2 = r1 - (float)h2;
+    h3 = (__bf16)r2;
+}
+
+__device__ __forceinline__ void trailing_bf16x6(float* W, int ldw, int n, int k0, const Lds<float>& L)
+{
+    constexpr int NB = Prec<float>::NB;  // 32 = K of one MFMA
+    const int R0 = k0 + NB;
+    const int m = n - R0;
+    const int nt = (m + 15) / 16;
+    const int lane = threadIdx.x & 63, i = lane & 15, k8 = lane >> 4;
+    for (int tile = wave_id(); tile < nt * nt; tile += NWAVE) {
+        const int row0 = R0 + (tile / nt) * 16, col0 = R0 + (tile % nt) * 16;
+        const int ar = row0 + i, bc = col0 + i;
+        float a[8], b[8];
+        if (ar < n) {
+            const float4_t* p = (const float4_t*)(W + (size_t)ar * ldw + k0 + 8 * k8);
+            const float4_t u0 = p[0], u1 = p[1];
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                a[s] = u0[s];
+                a[4 + s] = u1[s];
+            }
+        } else {
+#pragma unroll
+            for (int s = 0; s < 8; ++s) a[s] = 0.0f;
+        }
+        if (bc < n) {
+            const float4_t* p = (const float4_t*)(L.ut + (size_t)(bc - R0) * NB + 8 * k8);
+            const float4_t u0 = p[0], u1 = p[1];
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                b[s] = u0[s];
+                b[4 + s] = u1[s];
+            }
+        } else {
+#pragma unroll
+            for (int s = 0; s < 8; ++s) b[s] = 0.0f;
+        }
+        bf16x8_t a1, a2, a3, b1, b2, b3;
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            __bf16 x1, x2, x3;
+            split3(a[s], x1, x2, x3);
+            a1[s] = x1;
+            a2[s] = x2;
+            a3[s] = x3;
+            split3(b[s], x1, x2, x3);
+            b1[s] = x1;
+            b2[s] = x2;
+            b3[s] = x3;
+        }
+        float4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a3, b1, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, b2, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b3, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, b1, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b2, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1, acc, 0, 0, 0);
+        // item r of lane l is row 4*(l/16) + r, column l%16
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = row0 + 4 * k8 + r, col = col0 + i;
+            if (row < n && col < n) W[(size_t)row * ldw + col] -= acc[r];
+        }
+    }
+    __syncthreads();
+}
+
+// FP32 FMA on the vector units: each lane computes 4 outputs of a 16 x 16 tile.
+__device__ __forceinline__ void trailing_f32_valu(float* W, int ldw, int n, int k0, const Lds<float>& L)
+{
+    constexpr int NB = Prec<float>::NB;
+    const int R0 = k0 + NB;
+    const int m = n - R0;
+    const int nt = (m + 15) / 16;
+    const int lane = threadIdx.x & 63, i = lane & 15, k8 = lane >> 4;
+    for (int tile = wave_id(); tile < nt * nt; tile += NWAVE) {
+        const int row0 = R0 + (tile / nt) * 16, col0 = R0 + (tile % nt) * 16;
+        const int col = col0 + i;
+        if (col >= n) continue;
+        const float* u = L.ut + (size_t)(col - R0) * NB;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = row0 + 4 * k8 + r;
+            if (row >= n) continue;
+            const float* l = W + (size_t)row * ldw + k0;
+            float s = 0.0f;
+#pragma unroll
+            for (int k = 0; k < NB; ++k) s = __builtin_fmaf(l[k], u[k], s);
+            W[(size_t)row * ldw + col] -= s;
+        }
+    }
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------- solve
+// yv <- U^-1 L^-1 yv with the factors in W (row-major, unit-lower L), in
+// blocks of BS rows: a block-parallel GEMV with the finished part (BT/BS
+// threads per row), then wavefront 0 solves the diagonal block with its rows
+// in registers.
+template <typename T>
+__device__ __forceinline__ void lu_solve(const T* W, int ldw, int n, const Lds<T>& L)
+{
+    constexpr int BS = Prec<T>::BS;
+    constexpr int TPR = BT / BS;
+    const int t = threadIdx.x, lane = t & 63;
+    const int w = wave_id();
+    const int nblk = (n + BS - 1) / BS;
+    // forward
+    for (int blk = 0; blk < nblk; ++blk) {
+        const int r0 = blk * BS;
+        const int rows = n - r0 < BS ? n - r0 : BS;
+        if (r0 > 0) {
+            const int r = r0 + t / TPR, q = t % TPR;
+            double s = 0.0;
+            if (r < r0 + rows)
+                for (int c = q; c < r0; c += TPR) s += (double)W[(size_t)r * ldw + c] * (double)L.yv[c];
+#pragma unroll
+            for (int off = 1; off < TPR; off <<= 1) s += __shfl_xor(s, off, 64);
+            if (q == 0 && r < r0 + rows) L.yv[r] = (T)((double)L.yv[r] - s);
+            __syncthreads();
+        }
+        if (w == 0) {
+            T row[BS];
+            const int r = r0 + lane;
+            const int rr = r < n ? r : n - 1;
+#pragma unroll
+            for (int j = 0; j < BS; ++j) {
+                const T v = W[(size_t)rr * ldw + (r0 + j < n ? r0 + j : n - 1)];
+                row[j] = (lane < rows && j < lane) ? v : (T)0;
+            }
+            T y = (lane < rows) ? L.yv[rr] : (T)0;
+#pragma unroll
+            for (int j = 0; j < BS; ++j) {
+                const T yj = __shfl(y, j, 64);
+                y -= row[j] * yj;
+            }
+            if (lane < rows) L.yv[r] = y;
+        }
+        __syncthreads();
+    }
+    // backward
+    for (int blk = nblk - 1; blk >= 0; --blk) {
+        const int r0 = blk * BS;
+        const int rows = n - r0 < BS ? n - r0 : BS;
+        const int c1 = r0 + rows;
+        if (c1 < n) {
+            const int r = r0 + t / TPR, q = t % TPR;
+            double s = 0.0;
+            if (r < c1)
+                for (int c = c1 + q; c < n; c += TPR) s += (double)W[(size_t)r * ldw + c] * (double)L.yv[c];
+#pragma unroll
+            for (int off = 1; off < TPR; off <<= 1) s += __shfl_xor(s, off, 64);
+            if (q == 0 && r < c1) L.yv[r] = (T)((double)L.yv[r] - s);
+            __syncthreads();
+        }
+        if (w == 0) {
+            T row[BS];
+            const int r = r0 + lane;
+            const int rr = r < n ? r : n - 1;
+#pragma unroll
+            for (int j = 0; j < BS; ++j) {
+                const T v = W[(size_t)rr * ldw + (r0 + j < n ? r0 + j : n - 1)];
+                row[j] = (lane < rows && j < rows && j >= lane) ? v : (T)0;
+            }
+            T y = (lane < rows) ? L.yv[rr] : (T)0;
+#pragma unroll
+            for (int j = BS - 1; j >= 0; --j) {
+                if (lane == j) y = (row[j] != (T)0) ? y / row[j] : y;
+                const T yj = __shfl(y, j, 64);
+                if (lane < j) y -= row[j] * yj;
+            }
+            if (lane < rows) L.yv[r] = y;
+        }
+        __syncthreads();
+    }
+}
+
+// rv <- b - A x (FP64, column-major A); returns ||rv||_inf (all threads).
+template <typename T>
+__device__ __forceinline__ double residual(const double* A, int lda, const double* b, int n, const Lds<T>& L)
+{
+    double m = 0.0;
+    for (int r = threadIdx.x; r < n; r += BT) {
+        double s = b[r];
+        for (int c = 0; c < n; ++c) s = __builtin_fma(-A[r + (size_t)c * lda], L.xv[c], s);
+        L.rv[r] = s;
+        const double a = __builtin_fabs(s);
+        m = a > m ? a : m;
+    }
+    return block_max(L, m);
+}
+
+// One matrix (bi), all threads of the workgroup.
+template <typename T, bool MIXED>
+__device__ __forceinline__ void gesv_matrix(const GesvArgs& G, const Lds<T>& L, int bi, int n, int lda)
+{
+    constexpr int NB = Prec<T>::NB;
+    const int ldw = G.ldw;
+    const int t = threadIdx.x;
+    const double* A = G.a[bi];
+    const double* bvec = G.b[bi];
+    double* xo = G.x[bi];
+    T* W = (T*)(G.ws + (long long)bi * G.wstride);
+
+    // ||A||_inf and ||b||_inf
+    double an = 0.0, bn = 0.0;
+    for (int r = t; r < n; r += BT) {
+        double s = 0.0;
+        for (int c = 0; c < n; ++c) s += __builtin_fabs(A[r + (size_t)c * lda]);
+        an = s > an ? s : an;
+        const double v = __builtin_fabs(bvec[r]);
+        bn = v > bn ? v : bn;
+    }
+    const double anorm = block_max(L, an);
+    const double bnorm = block_max(L, bn);
+
+    copy_in(A, lda, W, ldw, n, L.ut);
+    for (int r = t; r < n; r += BT) L.perm[r] = r;
+    __syncthreads();
+
+    int zero = 0;
+    for (int k0 = 0; k0 < n; k0 += NB) {
+        const int nb = n - k0 < NB ? n - k0 : NB;
+        const int z = panel<T>(W, ldw, n, k0, nb, L);
+        if (zero == 0) zero = z;
+        swap_rows<T>(W, ldw, n, k0, nb, L);
+        if (k0 + nb < n) {
+            trsm_u12<T>(W, ldw, n, k0, L);
+            if constexpr (MIXED) {
+                if (n >= 256) trailing_bf16x6(W, ldw, n, k0, L);
+                else trailing_f32_valu(W, ldw, n, k0, L);
+            } else {
+                trailing_f64(W, ldw, n, k0, L);
+            }
+        }
+    }
+
+    // x0 = (LU)^-1 P b
+    for (int r = t; r < n; r += BT) L.yv[r] = (T)bvec[L.perm[r]];
+    __syncthreads();
+    lu_solve<T>(W, ldw, n, L);
+    for (int r = t; r < n; r += BT) L.xv[r] = (double)L.yv[r];
+    __syncthreads();
+    int it = 0;
+    double eta = 0.0;
+    for (;;) {
+        double xm = 0.0;
+        for (int r = t; r < n; r += BT) {
+            const double v = __builtin_fabs(L.xv[r]);
+            xm = v > xm ? v : xm;
+        }
+        const double xnorm = block_max(L, xm);
+        const double rnorm = residual<T>(A, lda, bvec, n, L);
+        const double den = anorm * xnorm + bnorm;
+        eta = uniform(den > 0.0 ? rnorm / den : (rnorm > 0.0 ? 1.0 : 0.0));
+        if (!MIXED || !(eta > G.tol_per_n * n) || it >= G.max_iter) break;
+        // d = (LU)^-1 P r ; x += d
+        __syncthreads();
+        for (int r = t; r < n; r += BT) L.yv[r] = (T)L.rv[L.perm[r]];
+        __syncthreads();
+        lu_solve<T>(W, ldw, n, L);
+        for (int r = t; r < n; r += BT) L.xv[r] += (double)L.yv[r];
+        __syncthreads();
+        ++it;
+    }
+    for (int r = t; r < n; r += BT) xo[r] = L.xv[r];
+    if (t == 0) {
+        G.berr[bi] = eta;
+        if (MIXED) G.iters[bi] = it;
+        int inf = 0;
+        if (zero) inf = zero;
+        else if (MIXED && eta > G.tol_per_n * n) inf = n + 1;
+        G.info[bi] = inf;
+    }
+}
+
+// The persistent loop has a single back edge and every branch around a
+// barrier is on a wave-uniform value.
+template <typename T, bool MIXED>
+__global__ __launch_bounds__(BT) void k_gesv(GesvArgs G)
+{
+    extern __shared__ __attribute__((aligned(16))) char lds_raw[];
+    const Lds<T> L = carve<T>(lds_raw, G.ldw);
+    const int t = threadIdx.x;
+    for (;;) {
+        if (t == 0) *L.flag = atomicAdd(G.counter, 1);
+        __syncthreads();
+        const int bi = __builtin_amdgcn_readfirstlane(*L.flag);
+        __syncthreads();
+        if (bi >= G.count) break;
+        const int n = __builtin_amdgcn_readfirstlane(G.n[bi]);
+        const int lda = __builtin_amdgcn_readfirstlane(G.lda[bi]);
+        int bad = 0;
+        if (n < 1 || n > G.max_n) bad = -3;
+        else if (G.a[bi] == nullptr) bad = -4;
+        else if (lda < n || (long long)lda * n > MFH_MAX_MATRIX_ELEMS) bad = -5;
+        else if (G.b[bi] == nullptr) bad = -6;
+        else if (G.x[bi] == nullptr) bad = -7;
+        bad = __builtin_amdgcn_readfirstlane(bad);
+        if (bad == 0) gesv_matrix<T, MIXED>(G, L, bi, n, lda);
+        else if (t == 0) G.info[bi] = bad;
+        __syncthreads();
+    }
+}
+
+__global__ void k_zero_counter(int* c) { *c = 0; }
+
+template <typename T, bool MIXED>
+int launch_gesv(mfh_handle* h, GesvArgs G)
+{
+    const size_t lds = gesv_lds_bytes<T>(G.ldw);
+    MFH_HIP_CHECK(hipFuncSetAttribute((const void*)k_gesv<T, MIXED>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)lds));
+    int per_cu = 0;
+    MFH_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_gesv<T, MIXED>, BT, lds));
+    if (per_cu < 1) per_cu = 1;
+    long grid = (long)per_cu * h->num_cus;
+    if (grid > G.count) grid = G.count;
+    hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, h->stream, G.counter);
+    int st = mfh_timing_begin(h, MIXED ? MFH_KERNEL_GESV_MIXED : MFH_KERNEL_GESV_F64, G.max_n);
+    if (st) return st;
+    hipLaunchKernelGGL((k_gesv<T, MIXED>), dim3((unsigned)grid), dim3(BT), lds, h->stream, G);
+    MFH_HIP_CHECK(hipGetLastError());
+    return mfh_timing_end(h);
+}
+
+}  // namespace
+
+size_t mfh_gesv_ws_bytes(int count, int max_n, bool mixed)
+{
+    const size_t ldw = (size_t)((max_n + 7) & ~7);
+    const size_t per = ((ldw * max_n * (mixed ? 4 : 8)) + 255) & ~(size_t)255;
+    return 256 + per * (size_t)count;
+}
 
 int mfh_launch_gesv_mixed(mfh_handle* h, int count, const int* n, const double* const* a, const int* lda,
                           const double* const* b, double* const* x, int max_iter, double tol_per_n, int* iters,
                           double* berr, int* info, int max_n, bool mixed)
 {
-    (void)h; (void)count; (void)n; (void)a; (void)lda; (void)b; (void)x; (void)max_iter; (void)tol_per_n;
-    (void)iters; (void)berr; (void)info; (void)max_n; (void)mixed;
-    return mfh_fail(MFH_ERR_UNSUPPORTED, "blocked mixed-precision solver not built yet");
+    const int ldw = (max_n + 7) & ~7;
+    const size_t per = (((size_t)ldw * max_n * (mixed ? 4 : 8)) + 255) & ~(size_t)255;
+    int st = mfh_ws_reserve(h, mfh_gesv_ws_bytes(count, max_n, mixed));
+    if (st) return st;
+    GesvArgs G;
+    G.n = n;
+    G.a = a;
+    G.lda = lda;
+    G.b = b;
+    G.x = x;
+    G.iters = iters;
+    G.berr = berr;
+    G.info = info;
+    G.count = count;
+    G.max_n = max_n;
+    G.max_iter = max_iter;
+    G.tol_per_n = tol_per_n;
+    G.counter = (int*)h->ws;
+    G.ws = (char*)h->ws + 256;
+    G.wstride = (long long)per;
+    G.ldw = ldw;
+    return mixed ? launch_gesv<float, true>(h, G) : launch_gesv<double, false>(h, G);
 }
