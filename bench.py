@@ -167,6 +167,117 @@ def copy_calibration(mib: int = 256):
     log(f"copy calibration: {mib} MiB read + {mib} MiB written")
 
 
+MIXED_SIZES = (128, 256, 384, 512)
+
+
+def make_conditioned_batch(count: int, seed: int, chunk: int = 200):
+    """The mixed-precision workload: n uniform over {128, 256, 384, 512},
+    kappa = 10^U(2,6), sigma_i = kappa^(-i/(n-1)), A = U diag(sigma) V^T with
+    Haar U, V (QR of N(0,1) with the R-diagonal sign fix), lda = n, b ~ N(0,1),
+    all in FP64 on the GPU.  Returns (buf, offsets, ns, b, boffsets)."""
+    import torch
+
+    rng = np.random.default_rng(seed)
+    ns = rng.choice(np.array(MIXED_SIZES), size=count).astype(np.int32)
+    kappa = 10.0 ** rng.uniform(2.0, 6.0, size=count)
+    order = np.argsort(ns, kind="stable")
+    sizes = ns.astype(np.int64) ** 2
+    offsets = np.zeros(count, dtype=np.int64)
+    offsets[order] = np.concatenate([[0], np.cumsum(sizes[order])[:-1]])
+    buf = torch.empty(int(sizes.sum()), dtype=torch.float64, device="cuda")
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+
+    def haar(c, n):
+        q, r = torch.linalg.qr(torch.randn((c, n, n), dtype=torch.float64, device="cuda", generator=g))
+        return q * torch.sign(torch.diagonal(r, dim1=1, dim2=2)).unsqueeze(1)
+
+    for n in MIXED_SIZES:
+        idx = np.flatnonzero(ns == n)
+        for s0 in range(0, idx.size, chunk):
+            sub = idx[s0:s0 + chunk]
+            c = sub.size
+            expo = torch.arange(n, dtype=torch.float64, device="cuda") / (n - 1)
+            sig = torch.from_numpy(kappa[sub]).to("cuda").unsqueeze(1) ** (-expo.unsqueeze(0))
+            u, v = haar(c, n), haar(c, n)
+            a = torch.bmm(u * sig.unsqueeze(1), v.transpose(1, 2).contiguous())
+            start = int(offsets[sub[0]])
+            buf[start:start + c * n * n] = a.transpose(1, 2).contiguous().reshape(-1)  # column-major
+            del u, v, a
+    boff = np.concatenate([[0], np.cumsum(ns.astype(np.int64))[:-1]])
+    b = torch.randn(int(ns.sum()), dtype=torch.float64, device="cuda", generator=g)
+    return buf, offsets, ns, b, boff
+
+
+def mixed_leg(steps: int, warmup: int, count: int = 8000, seed: int = 3):
+    """FP32/BF16-factor + FP64-refinement LU solve against the plain FP64
+    blocked LU solve on the same batch, both to backward error <= 1e-15 n."""
+    import torch
+
+    from matflat_amd import _lib, hip
+
+    log(f"mixed precision: generating {count} matrices n in {MIXED_SIZES}, cond 1e2..1e6")
+    buf, offsets, ns, b, boff = make_conditioned_batch(count, seed)
+    torch.cuda.synchronize()
+    dev = torch.device("cuda")
+    n_t = torch.from_numpy(ns).to(dev)
+    a_ptrs = (buf.data_ptr() + 8 * torch.from_numpy(offsets).to(dev)).contiguous()
+    b_ptrs = (b.data_ptr() + 8 * torch.from_numpy(boff).to(dev)).contiguous()
+    x = torch.zeros_like(b)
+    x_ptrs = (x.data_ptr() + 8 * torch.from_numpy(boff).to(dev)).contiguous()
+    info = torch.zeros(count, dtype=torch.int32, device=dev)
+    iters = torch.zeros(count, dtype=torch.int32, device=dev)
+    berr = torch.zeros(count, dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream()
+    handle = hip.Handle()
+    handle.bind_stream(stream)
+    max_n = max(MIXED_SIZES)
+
+    def call(mixed):
+        if mixed:
+            hip.dgesv_mixed_vbatched(n_t, a_ptrs, n_t, b_ptrs, x_ptrs, 10, 1e-15, iters, berr, info, max_n,
+                                     handle=handle)
+        else:
+            hip.dgesv_vbatched(n_t, a_ptrs, n_t, b_ptrs, x_ptrs, berr, info, max_n, handle=handle)
+
+    res = {}
+    flops = float(np.sum(2.0 / 3.0 * ns.astype(np.float64) ** 3 + 1.5 * ns.astype(np.float64) ** 2))
+    for name, mixed in (("fp64", False), ("mixed", True)):
+        for _ in range(warmup):
+            call(mixed)
+        torch.cuda.synchronize()
+        start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        start.record(stream)
+        for _ in range(steps):
+            call(mixed)
+        stop.record(stream)
+        torch.cuda.synchronize()
+        ms = start.elapsed_time(stop) / steps
+        inf = info.cpu().numpy()
+        be = berr.cpu().numpy()
+        ratio = be / ns
+        entry = {"ms_per_step": round(ms, 4), "solves_per_s": round(count / (ms / 1e3), 1),
+                 "hpl_tflops": round(flops / (ms / 1e3) / 1e12, 2),
+                 "max_berr_over_n": float(ratio.max()), "all_info_zero": bool(np.all(inf == 0)),
+                 "meets_1e-15n": bool(np.all(ratio <= 1e-15))}
+        if mixed:
+            it = iters.cpu().numpy()
+            entry.update({"mean_iters": round(float(it.mean()), 3), "max_iters": int(it.max())})
+        log(f"  {name}: {ms:.3f} ms/step, {entry['solves_per_s']:.0f} solves/s, {entry['hpl_tflops']} TFLOP/s (HPL "
+            f"count), max berr/n {entry['max_berr_over_n']:.2e}" +
+            (f", iters mean {entry['mean_iters']} max {entry['max_iters']}" if mixed else ""))
+        res[name] = entry
+    handle.close()
+    res["speedup_mixed_over_fp64"] = round(res["fp64"]["ms_per_step"] / res["mixed"]["ms_per_step"], 3)
+    res["workload"] = (f"{count} LU solves, n uniform over {list(MIXED_SIZES)}, A = U diag(s) V^T (Haar U, V), "
+                       "cond 10^U(2,6), b ~ N(0,1); refine to ||b-Ax||/(||A|| ||x|| + ||b||) <= 1e-15 n or 10 "
+                       "iterations")
+    res["seed"] = seed
+    del buf, b, x
+    torch.cuda.empty_cache()
+    return res
+
+
 def cpu_baseline(buf_host, offsets, ns, ldas, xoff, b_host, max_seconds=20.0):
     import oracle
 
@@ -200,6 +311,8 @@ def main() -> int:
     ap.add_argument("--math", choices=["fast", "exact"], default="fast",
                     help="math mode of the headline run (the other mode is timed as a secondary line)")
     ap.add_argument("--no-secondary", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="skip the mixed-precision LU-solve comparison (n = 128..512)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 256 MiB device copy (reference for HBM byte counters)")
     args = ap.parse_args()
@@ -274,6 +387,9 @@ def main() -> int:
     if not args.no_cpu_baseline:
         log("cpu baseline")
         cpu = cpu_baseline(buf_host, offsets, ns, ldas, xoff, b_host)
+    mixed = None
+    if not args.no_mixed:
+        mixed = mixed_leg(max(3, args.steps // 4), 1)
     out = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -295,6 +411,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "math": args.math,
         "secondary": secondary,
+        "mixed_precision": mixed,
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -34,6 +34,9 @@
 #include "mfh_device.h"
 #include "mfh_internal.h"
 
+// Neither solver is bit-identical to MatFlat: multiply-adds may fuse.
+#pragma clang fp contract(fast)
+
 namespace {
 
 constexpr int BT = 512;  // threads per workgroup
@@ -80,34 +83,39 @@ struct GesvArgs {
 // LDS layout (bytes), for row stride L = ldw:
 //   ut    [L][NB] T       U12 transposed (k contiguous); also the copy tile
 //   l11   [NB][NB] T
-//   prow, jrow [NB] T     pivot row / row j during an exchange
+//   cand  [2][NWAVE][NB] T, jrow [2][NB] T, ckey/cidx [2][NWAVE]: pivot search
 //   xv, rv [L] double     solution, residual / right-hand side
 //   yv    [L] T           solve vector
 //   perm  [L] int         row permutation (position -> original row)
-//   pivs  [NB] int
+//   pivs  [NB] int, mapk / mapv [64] int, mapn
 //   red   [NWAVE] (double key, int idx), scal [8] double
 template <typename T>
 struct Lds {
     T* ut;
     T* l11;
-    T* prow;
-    T* jrow;
+    T* cand;     // [2][NWAVE][NB]: each wavefront's pivot candidate row, by column parity
+    T* jrow;     // [2][NB]: row k0+j before the exchange, by column parity
+    double* ckey;  // [2][NWAVE] candidate keys
+    int* cidx;     // [2][NWAVE] candidate rows
     double* xv;
     double* rv;
     T* yv;
     int* perm;
     int* pivs;
+    int* mapk;  // rows touched by a panel's swaps (at most 64) ...
+    int* mapv;  // ... and the row whose content each receives
+    int* mapn;  // number of such rows
     double* rkey;
     int* ridx;
     double* scal;
     int* flag;
 };
 
-// The ut region also holds the 32 x 33 staging tile of the input copy.
+// The ut region also holds the 128 x 33 staging tile of the input copy.
 __host__ __device__ inline size_t ut_elems(int ldw, int nb)
 {
     const size_t e = (size_t)ldw * nb;
-    return e > 32 * 33 ? e : 32 * 33;
+    return e > 128 * 33 ? e : 128 * 33;
 }
 
 template <typename T>
@@ -117,13 +125,16 @@ __host__ __device__ size_t gesv_lds_bytes(int ldw)
     size_t s = 0;
     s += ut_elems(ldw, NB) * sizeof(T);
     s += (size_t)NB * NB * sizeof(T);
-    s += 2 * (size_t)NB * sizeof(T);
+    s += 2 * (size_t)NWAVE * NB * sizeof(T) + 2 * (size_t)NB * sizeof(T);
+    s = (s + 15) & ~(size_t)15;
+    s += 2 * NWAVE * sizeof(double) + 2 * NWAVE * sizeof(int);
     s = (s + 15) & ~(size_t)15;
     s += 2 * (size_t)ldw * sizeof(double);
     s += (size_t)ldw * sizeof(T);
     s = (s + 15) & ~(size_t)15;
     s += (size_t)ldw * sizeof(int);
     s += (size_t)NB * sizeof(int);
+    s += 2 * 64 * sizeof(int) + 16;
     s = (s + 15) & ~(size_t)15;
     s += NWAVE * sizeof(double) + NWAVE * sizeof(int) + 8 * sizeof(double) + 16;
     return s;
@@ -139,10 +150,15 @@ __device__ __forceinline__ Lds<T> carve(char* base, int ldw)
     s += ut_elems(ldw, NB) * sizeof(T);
     L.l11 = (T*)(base + s);
     s += (size_t)NB * NB * sizeof(T);
-    L.prow = (T*)(base + s);
-    s += NB * sizeof(T);
+    L.cand = (T*)(base + s);
+    s += 2 * (size_t)NWAVE * NB * sizeof(T);
     L.jrow = (T*)(base + s);
-    s += NB * sizeof(T);
+    s += 2 * (size_t)NB * sizeof(T);
+    s = (s + 15) & ~(size_t)15;
+    L.ckey = (double*)(base + s);
+    s += 2 * NWAVE * sizeof(double);
+    L.cidx = (int*)(base + s);
+    s += 2 * NWAVE * sizeof(int);
     s = (s + 15) & ~(size_t)15;
     L.xv = (double*)(base + s);
     s += (size_t)ldw * sizeof(double);
@@ -155,6 +171,12 @@ __device__ __forceinline__ Lds<T> carve(char* base, int ldw)
     s += (size_t)ldw * sizeof(int);
     L.pivs = (int*)(base + s);
     s += NB * sizeof(int);
+    L.mapk = (int*)(base + s);
+    s += 64 * sizeof(int);
+    L.mapv = (int*)(base + s);
+    s += 64 * sizeof(int);
+    L.mapn = (int*)(base + s);
+    s += 16;
     s = (s + 15) & ~(size_t)15;
     L.rkey = (double*)(base + s);
     s += NWAVE * sizeof(double);
@@ -198,43 +220,154 @@ __device__ __forceinline__ double block_max(const Lds<T>& L, double v)
 }
 
 // ------------------------------------------------------------------- copy
-// W[r][c] = (T) A[r + c*lda], through 32 x 32 LDS tiles (both sides coalesced).
+// W[r][c] = (T) A[r + c*lda] through 128 x 32 LDS tiles (both sides
+// coalesced, eight independent loads per thread and tile), with the row sums
+// of |A| on the way.  Returns ||A||_inf (wave-uniform).
 template <typename T>
-__device__ __forceinline__ void copy_in(const double* A, int lda, T* W, int ldw, int n, T* tile)
+__device__ __forceinline__ double copy_in(const double* A, int lda, T* W, int ldw, int n, const Lds<T>& L)
 {
+    T* tile = L.ut;
     const int t = threadIdx.x;
-    const int nt = (n + 31) / 32;
-    for (int tt = 0; tt < nt * nt; ++tt) {
-        const int r0 = (tt % nt) * 32, c0 = (tt / nt) * 32;
-        // read: 32 rows of 16 columns per pass
-        {
-            const int rr = t & 31, cc = t >> 5;  // cc in 0..15
+    const int rr = t & 127, cq = t >> 7;  // read: row, column quarter
+    const int cc = t & 31, rq = t >> 5;   // write: column, row sixteenth
+    double best = 0.0;
+    for (int r0 = 0; r0 < n; r0 += 128) {
+        const int r = r0 + rr;
+        double rowsum = 0.0;
+        for (int c0 = 0; c0 < n; c0 += 32) {
+            double v[8];
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const int r = r0 + rr, c = c0 + cc + 16 * h;
-                T v = 0;
-                if (r < n && c < n) v = (T)A[r + (size_t)c * lda];
-                tile[rr * 33 + cc + 16 * h] = v;
+            for (int h = 0; h < 8; ++h) {
+                const int c = c0 + cq + 4 * h;
+                v[h] = (r < n && c < n) ? A[r + (size_t)c * lda] : 0.0;
             }
+#pragma unroll
+            for (int h = 0; h < 8; ++h) {
+                rowsum += __builtin_fabs(v[h]);
+                tile[rr * 33 + cq + 4 * h] = (T)v[h];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int h = 0; h < 8; ++h) {
+                const int rl = rq + 16 * h;
+                const int r2 = r0 + rl, c2 = c0 + cc;
+                if (r2 < n && c2 < n) W[(size_t)r2 * ldw + c2] = tile[rl * 33 + cc];
+            }
+            __syncthreads();
         }
+        // four partial sums per row (one per column quarter)
+        double* part = (double*)tile;
+        part[cq * 128 + rr] = rowsum;
         __syncthreads();
-        {
-            const int cc = t & 31, rr = t >> 5;
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const int r = r0 + rr + 16 * h, c = c0 + cc;
-                if (r < n && c < n) W[(size_t)r * ldw + c] = tile[(rr + 16 * h) * 33 + cc];
-            }
+        if (cq == 0 && r < n) {
+            const double sum = part[rr] + part[128 + rr] + part[256 + rr] + part[384 + rr];
+            best = sum > best ? sum : best;
         }
         __syncthreads();
     }
+    return block_max(L, best);
 }
 
 // --------------------------------------------------------------- panel
+// Pivot search key of row r for a column value v: the row order is partial
+// pivoting's (largest |v| in single precision, lowest row on ties, NaN below
+// every number, non-candidates below NaN), packed so that one unsigned 64-bit
+// maximum finds it.
+template <typename T>
+__device__ __forceinline__ unsigned long long pivot_pack(bool cand, T v, int r)
+{
+    const float a = __builtin_fabsf((float)v);
+    unsigned hi = 0;
+    if (cand) hi = (a != a) ? 1u : (__float_as_uint(a) + 2u);
+    return ((unsigned long long)hi << 32) | (unsigned)(~r);
+}
+
+#define MFH_DPP_MAX_U64(v, CTRL, RMASK)                                                                     \
+    do {                                                                                                  \
+        const unsigned lo_ = (unsigned)(v), hi_ = (unsigned)((v) >> 32);                                  \
+        const unsigned olo_ = __builtin_amdgcn_update_dpp(lo_, lo_, CTRL, RMASK, 0xF, false);             \
+        const unsigned ohi_ = __builtin_amdgcn_update_dpp(hi_, hi_, CTRL, RMASK, 0xF, false);             \
+        const unsigned long long o_ = ((unsigned long long)ohi_ << 32) | olo_;                            \
+        (v) = o_ > (v) ? o_ : (v);                                                                        \
+    } while (0)
+
+// Wave-wide maximum through DPP row shifts and row broadcasts (no LDS
+// round trips); every lane receives it.
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
+{
+    MFH_DPP_MAX_U64(v, 0x111, 0xF);  // row_shr:1
+    MFH_DPP_MAX_U64(v, 0x112, 0xF);  // row_shr:2
+    MFH_DPP_MAX_U64(v, 0x114, 0xF);  // row_shr:4
+    MFH_DPP_MAX_U64(v, 0x118, 0xF);  // row_shr:8  -> lane 15 of each row: row maximum
+    MFH_DPP_MAX_U64(v, 0x142, 0xA);  // row_bcast:15 into rows 1 and 3
+    MFH_DPP_MAX_U64(v, 0x143, 0xC);  // row_bcast:31 into rows 2 and 3 -> lane 63: maximum
+    const unsigned lo = __builtin_amdgcn_readlane((unsigned)v, 63);
+    const unsigned hi = __builtin_amdgcn_readlane((unsigned)(v >> 32), 63);
+    return ((unsigned long long)hi << 32) | lo;
+}
+
+// Column J of the panel (J a template parameter, so every register index is
+// static; a runtime column loop would index the row registers dynamically).
+// One barrier per column: before it, each wavefront's argmax winner
+// publishes its whole row and row k0+J publishes itself (double-buffered by
+// column parity); after it every thread picks the pivot row and does its part
+// of the exchange and the elimination.
+template <typename T, int J>
+__device__ __forceinline__ void panel_cols(T (&P)[Prec<T>::NB], int r, bool mine, int n, int k0, int nb,
+                                           const Lds<T>& L, int& zero)
+{
+    constexpr int NB = Prec<T>::NB;
+    if constexpr (J < NB) {
+        if (J < nb) {
+            const int t = threadIdx.x, w = t >> 6;
+            T* cand = L.cand + (J & 1) * NWAVE * NB;
+            T* jrow = L.jrow + (J & 1) * NB;
+            unsigned long long* ckey = (unsigned long long*)(L.ckey + (J & 1) * NWAVE);
+            const unsigned long long key = wave_max_u64(pivot_pack(mine && r >= k0 + J, P[J], r));
+            if (r == (int)~(unsigned)key) {
+#pragma unroll
+                for (int c = 0; c < NB; ++c) cand[w * NB + c] = P[c];
+                ckey[w] = key;
+            }
+            if (r == k0 + J) {
+#pragma unroll
+                for (int c = 0; c < NB; ++c) jrow[c] = P[c];
+            }
+            __syncthreads();
+            unsigned long long best = ckey[0];
+#pragma unroll
+            for (int ww = 1; ww < NWAVE; ++ww) best = ckey[ww] > best ? ckey[ww] : best;
+            int p = (int)~(unsigned)best;  // thread p - k0 holds row p: wavefront (p - k0) / 64
+            if ((best >> 32) == 0 || p < k0 + J || p >= n) p = k0 + J;  // keeps every row index in range
+            const T* prow = cand + ((p - k0) >> 6) * NB;
+            T pr[NB];
+#pragma unroll
+            for (int c = 0; c < NB; ++c) pr[c] = prow[c];
+            if (t == 0) L.pivs[J] = p;
+            if (r == p && p != k0 + J) {
+#pragma unroll
+                for (int c = 0; c < NB; ++c) P[c] = jrow[c];
+            }
+            if (r == k0 + J) {
+#pragma unroll
+                for (int c = 0; c < NB; ++c) P[c] = pr[c];
+            }
+            const T d = pr[J];
+            if (d == (T)0 && zero == 0) zero = k0 + J + 1;
+            if (mine && r > k0 + J) {
+                const T l = (d != (T)0) ? P[J] / d : P[J];
+                P[J] = l;
+#pragma unroll
+                for (int c = J + 1; c < NB; ++c) P[c] -= l * pr[c];
+            }
+            panel_cols<T, J + 1>(P, r, mine, n, k0, nb, L, zero);
+        }
+    }
+}
+
 // Factor columns k0 .. k0+nb-1 of rows k0 .. n-1 (one row per thread, in
-// registers), record the pivots in pivs[] and perm, write the panel back,
-// and leave the unit-lower L11 in l11.  Returns the first zero pivot
-// (1-based) or 0.
+// registers), record the pivots in pivs[], write the panel back, and leave
+// the unit-lower L11 in l11.  Returns the first zero pivot (1-based) or 0.
 template <typename T>
 __device__ __forceinline__ int panel(T* W, int ldw, int n, int k0, int nb, const Lds<T>& L)
 {
@@ -253,51 +386,7 @@ __device__ __forceinline__ int panel(T* W, int ldw, int n, int k0, int nb, const
         }
     }
     int zero = 0;
-#pragma unroll
-    for (int j = 0; j < NB; ++j) {
-        if (j >= nb) break;
-        // pivot: largest |a_rj| over r >= k0+j, lowest row on ties, NaN last
-        double key = (mine && r >= k0 + j) ? mfh_pivot_key((double)P[j]) : -2.0;
-        int idx = r;
-        mfh_wave_argmax(key, idx);
-        if ((t & 63) == 0) {
-            L.rkey[t >> 6] = key;
-            L.ridx[t >> 6] = idx;
-        }
-        __syncthreads();
-        double bk = L.rkey[0];
-        int p = L.ridx[0];
-#pragma unroll
-        for (int w = 1; w < NWAVE; ++w) mfh_pivot_combine(bk, p, L.rkey[w], L.ridx[w]);
-        if (bk < -1.5) p = k0 + j;  // unreachable (row k0+j exists); keeps p in range
-        // exchange rows k0+j and p
-        if (r == p) {
-#pragma unroll
-            for (int c = 0; c < NB; ++c) L.prow[c] = P[c];
-        }
-        if (r == k0 + j) {
-#pragma unroll
-            for (int c = 0; c < NB; ++c) L.jrow[c] = P[c];
-        }
-        if (t == 0) L.pivs[j] = p;
-        __syncthreads();
-        if (r == p && p != k0 + j) {
-#pragma unroll
-            for (int c = 0; c < NB; ++c) P[c] = L.jrow[c];
-        }
-        if (r == k0 + j) {
-#pragma unroll
-            for (int c = 0; c < NB; ++c) P[c] = L.prow[c];
-        }
-        const T d = L.prow[j];
-        if (d == (T)0 && zero == 0) zero = k0 + j + 1;
-        if (mine && r > k0 + j) {
-            const T l = (d != (T)0) ? P[j] / d : P[j];
-            P[j] = l;
-#pragma unroll
-            for (int c = j + 1; c < NB; ++c) P[c] -= l * L.prow[c];
-        }
-    }
+    panel_cols<T, 0>(P, r, mine, n, k0, nb, L, zero);
     if (mine) {
         if (nb == NB) {
 #pragma unroll
@@ -319,32 +408,57 @@ __device__ __forceinline__ int panel(T* W, int ldw, int n, int k0, int nb, const
     return zero;
 }
 
-// Apply the panel's row swaps to the columns outside it (each thread owns
-// columns, so the swap sequence needs no barrier), and to perm (thread 0).
+// Apply the panel's row swaps to the columns outside it and to perm.  The
+// swap sequence (k0+j <-> pivs[j], in order) only moves rows of the set
+// S = {k0 .. k0+nb-1} u {pivs[j]} (at most 64 rows): wavefront 0 composes it
+// into a map (row mapk[i] receives the old content of row mapv[i]) with
+// ballots, then every thread moves its column through registers: all loads
+// first, then the stores, so the memory accesses are independent.
 template <typename T>
 __device__ __forceinline__ void swap_rows(T* W, int ldw, int n, int k0, int nb, const Lds<T>& L)
 {
     const int t = threadIdx.x;
-    for (int c = t; c < n; c += BT) {
-        if (c >= k0 && c < k0 + nb) continue;
+    if (t < 64) {
+        const int lane = t;
+        int key = lane < nb ? k0 + lane : -1;  // keys never move; values do
+        int val = key;
+        int cnt = nb;
         for (int j = 0; j < nb; ++j) {
             const int p = L.pivs[j];
-            if (p != k0 + j) {
-                T* a = W + (size_t)(k0 + j) * ldw + c;
-                T* b = W + (size_t)p * ldw + c;
-                const T va = *a, vb = *b;
-                *a = vb;
-                *b = va;
+            if (p == k0 + j) continue;
+            const unsigned long long m = __ballot(key == p);
+            int b;
+            if (m == 0) {
+                b = cnt++;
+                if (lane == b) key = val = p;
+            } else {
+                b = __builtin_ctzll(m);
             }
+            const int va = __shfl(val, j, 64), vb = __shfl(val, b, 64);
+            if (lane == j) val = vb;
+            if (lane == b) val = va;
         }
+        const int oldperm = lane < cnt ? L.perm[val] : 0;
+        if (lane < cnt) {
+            L.mapk[lane] = key;
+            L.mapv[lane] = val;
+        }
+        if (lane == 0) *L.mapn = cnt;
+        // perm: position key now holds the original row that was at val
+        __builtin_amdgcn_wave_barrier();
+        if (lane < cnt) L.perm[key] = oldperm;
     }
-    if (t == 0) {
-        for (int j = 0; j < nb; ++j) {
-            const int p = L.pivs[j];
-            const int v = L.perm[k0 + j];
-            L.perm[k0 + j] = L.perm[p];
-            L.perm[p] = v;
-        }
+    __syncthreads();
+    const int cnt = __builtin_amdgcn_readfirstlane(*L.mapn);
+    const int c = t;  // n <= BT: one column per thread
+    if (c < n && (c < k0 || c >= k0 + nb)) {
+        T v[64];
+#pragma unroll
+        for (int i = 0; i < 64; ++i)
+            if (i < cnt) v[i] = W[(size_t)L.mapv[i] * ldw + c];
+#pragma unroll
+        for (int i = 0; i < 64; ++i)
+            if (i < cnt && L.mapk[i] != L.mapv[i]) W[(size_t)L.mapk[i] * ldw + c] = v[i];
     }
     __syncthreads();
 }
@@ -382,43 +496,81 @@ __device__ __forceinline__ void trsm_u12(T* W, int ldw, int n, int k0, const Lds
 }
 
 // ------------------------------------------------------- trailing updates
-// A22 -= L21 U12 over 16 x 16 output tiles, one wavefront per tile.
-__device__ __forceinline__ void trailing_f64(double* W, int ldw, int n, int k0, const Lds<double>& L)
+// A22 -= L21 U12.  Each wavefront takes a 32 x 32 block (2 x 2 MFMA tiles of
+// 16 x 16) per step: the block's C values, L21 rows and U12 columns are all
+// loaded up front (independent loads in flight), then the MFMAs run.
+struct F64Blk {
+    double a[2][4], b[2][4], c[2][2][4];
+};
+
+__device__ __forceinline__ void f64_load(F64Blk& B, const double* W, int ldw, int n, int k0, int R0, int row0,
+                                         int col0, const Lds<double>& L)
 {
     constexpr int NB = Prec<double>::NB;
+    const int lane = threadIdx.x & 63, i = lane & 15, kq = lane >> 4;
+    // lane supplies k = 4*kq + s to MFMA step s, for both operands
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int ar = row0 + 16 * h + i;
+        const double4_t ta = *(const double4_t*)(W + (size_t)(ar < n ? ar : n - 1) * ldw + k0 + 4 * kq);
+        const int bc = col0 + 16 * h + i;
+        const double4_t tb = *(const double4_t*)(L.ut + (size_t)((bc < n ? bc : n - 1) - R0) * NB + 4 * kq);
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            B.a[h][s] = ar < n ? ta[s] : 0.0;
+            B.b[h][s] = bc < n ? tb[s] : 0.0;
+        }
+    }
+    // item r of lane l is row (l/16) + 4r, column l%16
+#pragma unroll
+    for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = row0 + 16 * ti + kq + 4 * r, col = col0 + 16 * tj + i;
+                B.c[ti][tj][r] = W[(size_t)(row < n ? row : n - 1) * ldw + (col < n ? col : n - 1)];
+            }
+}
+
+__device__ __forceinline__ void f64_compute_store(const F64Blk& B, double* W, int ldw, int n, int row0, int col0)
+{
+    const int lane = threadIdx.x & 63, i = lane & 15, kq = lane >> 4;
+#pragma unroll
+    for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < 2; ++tj) {
+            double4_t acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int s = 0; s < 4; ++s) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(B.a[ti][s], B.b[tj][s], acc, 0, 0, 0);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = row0 + 16 * ti + kq + 4 * r, col = col0 + 16 * tj + i;
+                if (row < n && col < n) W[(size_t)row * ldw + col] = B.c[ti][tj][r] - acc[r];
+            }
+        }
+}
+
+// Software-pipelined like the BF16 variant below.
+__device__ __forceinline__ void trailing_f64(double* W, int ldw, int n, int k0, const Lds<double>& L)
+{
+    constexpr int NB = Prec<double>::NB;  // 16 = 4 MFMA steps of K = 4
     const int R0 = k0 + NB;
     const int m = n - R0;
-    const int nt = (m + 15) / 16;
-    const int lane = threadIdx.x & 63, i = lane & 15, kq = lane >> 4;
-    for (int tile = wave_id(); tile < nt * nt; tile += NWAVE) {
-        const int row0 = R0 + (tile / nt) * 16, col0 = R0 + (tile % nt) * 16;
-        const int ar = row0 + i, bc = col0 + i;
-        // lane supplies k = 4*kq + s to MFMA step s, for both operands
-        double av[4], bv[4];
-        if (ar < n) {
-            const double4_t t4 = *(const double4_t*)(W + (size_t)ar * ldw + k0 + 4 * kq);
-#pragma unroll
-            for (int s = 0; s < 4; ++s) av[s] = t4[s];
-        } else {
-#pragma unroll
-            for (int s = 0; s < 4; ++s) av[s] = 0.0;
-        }
-        if (bc < n) {
-            const double4_t t4 = *(const double4_t*)(L.ut + (size_t)(bc - R0) * NB + 4 * kq);
-#pragma unroll
-            for (int s = 0; s < 4; ++s) bv[s] = t4[s];
-        } else {
-#pragma unroll
-            for (int s = 0; s < 4; ++s) bv[s] = 0.0;
-        }
-        double4_t acc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-        for (int s = 0; s < 4; ++s) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[s], bv[s], acc, 0, 0, 0);
-        // item r of lane l is row (l/16) + 4r, column l%16
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int row = row0 + kq + 4 * r, col = col0 + i;
-            if (row < n && col < n) W[(size_t)row * ldw + col] -= acc[r];
+    const int nt = (m + 31) / 32;
+    const int nblk = nt * nt;
+    int blk = wave_id();
+    if (blk < nblk) {
+        F64Blk cur;
+        f64_load(cur, W, ldw, n, k0, R0, R0 + (blk / nt) * 32, R0 + (blk % nt) * 32, L);
+        for (;;) {
+            const int nxt = blk + NWAVE;
+            F64Blk nb;
+            if (nxt < nblk) f64_load(nb, W, ldw, n, k0, R0, R0 + (nxt / nt) * 32, R0 + (nxt % nt) * 32, L);
+            f64_compute_store(cur, W, ldw, n, R0 + (blk / nt) * 32, R0 + (blk % nt) * 32);
+            if (nxt >= nblk) break;
+            cur = nb;
+            blk = nxt;
         }
     }
     __syncthreads();
@@ -433,66 +585,106 @@ __device__ __forceinline__ void split3(float x, __bf16& h1, __bf16& h2, __bf16& 
     h3 = (__bf16)r2;
 }
 
+__device__ __forceinline__ void split8(const float (&x)[8], bf16x8_t& h1, bf16x8_t& h2, bf16x8_t& h3)
+{
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+        __bf16 a, b, c;
+        split3(x[s], a, b, c);
+        h1[s] = a;
+        h2[s] = b;
+        h3[s] = c;
+    }
+}
+
+// Operands of one 32 x 32 block: L21 rows (a), U12 columns (b) and C.
+struct Bf16Blk {
+    float a[2][8], b[2][8], c[2][2][4];
+};
+
+__device__ __forceinline__ void bf16_load(Bf16Blk& B, const float* W, int ldw, int n, int k0, int R0, int row0,
+                                          int col0, const Lds<float>& L)
+{
+    constexpr int NB = Prec<float>::NB;
+    const int lane = threadIdx.x & 63, i = lane & 15, k8 = lane >> 4;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int ar = row0 + 16 * h + i;
+        const float4_t* pa = (const float4_t*)(W + (size_t)(ar < n ? ar : n - 1) * ldw + k0 + 8 * k8);
+        const float4_t a0 = pa[0], a1 = pa[1];
+        const int bc = col0 + 16 * h + i;
+        const float4_t* pb = (const float4_t*)(L.ut + (size_t)((bc < n ? bc : n - 1) - R0) * NB + 8 * k8);
+        const float4_t b0 = pb[0], b1 = pb[1];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            B.a[h][s] = ar < n ? a0[s] : 0.0f;
+            B.a[h][4 + s] = ar < n ? a1[s] : 0.0f;
+            B.b[h][s] = bc < n ? b0[s] : 0.0f;
+            B.b[h][4 + s] = bc < n ? b1[s] : 0.0f;
+        }
+    }
+    // item r of lane l is row 4*(l/16) + r, column l%16
+#pragma unroll
+    for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = row0 + 16 * ti + 4 * k8 + r, col = col0 + 16 * tj + i;
+                B.c[ti][tj][r] = W[(size_t)(row < n ? row : n - 1) * ldw + (col < n ? col : n - 1)];
+            }
+}
+
+__device__ __forceinline__ void bf16_compute_store(const Bf16Blk& B, float* W, int ldw, int n, int row0, int col0)
+{
+    const int lane = threadIdx.x & 63, i = lane & 15, k8 = lane >> 4;
+    bf16x8_t a1[2], a2[2], a3[2], b1[2], b2[2], b3[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        split8(B.a[h], a1[h], a2[h], a3[h]);
+        split8(B.b[h], b1[h], b2[h], b3[h]);
+    }
+#pragma unroll
+    for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < 2; ++tj) {
+            // six products, smallest first, FP32 accumulation
+            float4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a3[ti], b1[tj], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2[ti], b2[tj], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1[ti], b3[tj], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2[ti], b1[tj], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1[ti], b2[tj], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1[ti], b1[tj], acc, 0, 0, 0);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = row0 + 16 * ti + 4 * k8 + r, col = col0 + 16 * tj + i;
+                if (row < n && col < n) W[(size_t)row * ldw + col] = B.c[ti][tj][r] - acc[r];
+            }
+        }
+}
+
+// Software-pipelined: the next block's operands are loaded before the
+// current block is computed and stored (blocks never overlap).
 __device__ __forceinline__ void trailing_bf16x6(float* W, int ldw, int n, int k0, const Lds<float>& L)
 {
     constexpr int NB = Prec<float>::NB;  // 32 = K of one MFMA
     const int R0 = k0 + NB;
     const int m = n - R0;
-    const int nt = (m + 15) / 16;
-    const int lane = threadIdx.x & 63, i = lane & 15, k8 = lane >> 4;
-    for (int tile = wave_id(); tile < nt * nt; tile += NWAVE) {
-        const int row0 = R0 + (tile / nt) * 16, col0 = R0 + (tile % nt) * 16;
-        const int ar = row0 + i, bc = col0 + i;
-        float a[8], b[8];
-        if (ar < n) {
-            const float4_t* p = (const float4_t*)(W + (size_t)ar * ldw + k0 + 8 * k8);
-            const float4_t u0 = p[0], u1 = p[1];
-#pragma unroll
-            for (int s = 0; s < 4; ++s) {
-                a[s] = u0[s];
-                a[4 + s] = u1[s];
-            }
-        } else {
-#pragma unroll
-            for (int s = 0; s < 8; ++s) a[s] = 0.0f;
-        }
-        if (bc < n) {
-            const float4_t* p = (const float4_t*)(L.ut + (size_t)(bc - R0) * NB + 8 * k8);
-            const float4_t u0 = p[0], u1 = p[1];
-#pragma unroll
-            for (int s = 0; s < 4; ++s) {
-                b[s] = u0[s];
-                b[4 + s] = u1[s];
-            }
-        } else {
-#pragma unroll
-            for (int s = 0; s < 8; ++s) b[s] = 0.0f;
-        }
-        bf16x8_t a1, a2, a3, b1, b2, b3;
-#pragma unroll
-        for (int s = 0; s < 8; ++s) {
-            __bf16 x1, x2, x3;
-            split3(a[s], x1, x2, x3);
-            a1[s] = x1;
-            a2[s] = x2;
-            a3[s] = x3;
-            split3(b[s], x1, x2, x3);
-            b1[s] = x1;
-            b2[s] = x2;
-            b3[s] = x3;
-        }
-        float4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a3, b1, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, b2, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b3, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, b1, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b2, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1, acc, 0, 0, 0);
-        // item r of lane l is row 4*(l/16) + r, column l%16
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int row = row0 + 4 * k8 + r, col = col0 + i;
-            if (row < n && col < n) W[(size_t)row * ldw + col] -= acc[r];
+    const int nt = (m + 31) / 32;
+    const int nblk = nt * nt;
+    int blk = wave_id();
+    if (blk < nblk) {
+        Bf16Blk cur;
+        bf16_load(cur, W, ldw, n, k0, R0, R0 + (blk / nt) * 32, R0 + (blk % nt) * 32, L);
+        for (;;) {
+            const int nxt = blk + NWAVE;
+            Bf16Blk nb;
+            if (nxt < nblk) bf16_load(nb, W, ldw, n, k0, R0, R0 + (nxt / nt) * 32, R0 + (nxt % nt) * 32, L);
+            bf16_compute_store(cur, W, ldw, n, R0 + (blk / nt) * 32, R0 + (blk % nt) * 32);
+            if (nxt >= nblk) break;
+            cur = nb;
+            blk = nxt;
         }
     }
     __syncthreads();
@@ -526,6 +718,23 @@ __device__ __forceinline__ void trailing_f32_valu(float* W, int ldw, int n, int 
 }
 
 // ---------------------------------------------------------------- solve
+// sum over c = c0, c0+step, ... < c1 of row[c] * y[c], with four
+// independent partial sums (several loads in flight).
+template <typename T>
+__device__ __forceinline__ double gemv_row(const T* row, const T* y, int c0, int c1, int step)
+{
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int c = c0;
+    for (; c + 3 * step < c1; c += 4 * step) {
+        s0 += (double)row[c] * (double)y[c];
+        s1 += (double)row[c + step] * (double)y[c + step];
+        s2 += (double)row[c + 2 * step] * (double)y[c + 2 * step];
+        s3 += (double)row[c + 3 * step] * (double)y[c + 3 * step];
+    }
+    for (; c < c1; c += step) s0 += (double)row[c] * (double)y[c];
+    return (s0 + s1) + (s2 + s3);
+}
+
 // yv <- U^-1 L^-1 yv with the factors in W (row-major, unit-lower L), in
 // blocks of BS rows: a block-parallel GEMV with the finished part (BT/BS
 // threads per row), then wavefront 0 solves the diagonal block with its rows
@@ -545,8 +754,7 @@ __device__ __forceinline__ void lu_solve(const T* W, int ldw, int n, const Lds<T
         if (r0 > 0) {
             const int r = r0 + t / TPR, q = t % TPR;
             double s = 0.0;
-            if (r < r0 + rows)
-                for (int c = q; c < r0; c += TPR) s += (double)W[(size_t)r * ldw + c] * (double)L.yv[c];
+            if (r < r0 + rows) s = gemv_row(W + (size_t)r * ldw, L.yv, q, r0, TPR);
 #pragma unroll
             for (int off = 1; off < TPR; off <<= 1) s += __shfl_xor(s, off, 64);
             if (q == 0 && r < r0 + rows) L.yv[r] = (T)((double)L.yv[r] - s);
@@ -579,8 +787,7 @@ __device__ __forceinline__ void lu_solve(const T* W, int ldw, int n, const Lds<T
         if (c1 < n) {
             const int r = r0 + t / TPR, q = t % TPR;
             double s = 0.0;
-            if (r < c1)
-                for (int c = c1 + q; c < n; c += TPR) s += (double)W[(size_t)r * ldw + c] * (double)L.yv[c];
+            if (r < c1) s = gemv_row(W + (size_t)r * ldw, L.yv, c1 + q, n, TPR);
 #pragma unroll
             for (int off = 1; off < TPR; off <<= 1) s += __shfl_xor(s, off, 64);
             if (q == 0 && r < c1) L.yv[r] = (T)((double)L.yv[r] - s);
@@ -614,8 +821,18 @@ __device__ __forceinline__ double residual(const double* A, int lda, const doubl
 {
     double m = 0.0;
     for (int r = threadIdx.x; r < n; r += BT) {
-        double s = b[r];
-        for (int c = 0; c < n; ++c) s = __builtin_fma(-A[r + (size_t)c * lda], L.xv[c], s);
+        // eight independent partial sums keep several column loads in flight
+        double ps[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) ps[u] = 0.0;
+        ps[0] = b[r];
+        int c = 0;
+        for (; c + 8 <= n; c += 8) {
+#pragma unroll
+            for (int u = 0; u < 8; ++u) ps[u] = __builtin_fma(-A[r + (size_t)(c + u) * lda], L.xv[c + u], ps[u]);
+        }
+        for (; c < n; ++c) ps[0] = __builtin_fma(-A[r + (size_t)c * lda], L.xv[c], ps[0]);
+        const double s = ((ps[0] + ps[1]) + (ps[2] + ps[3])) + ((ps[4] + ps[5]) + (ps[6] + ps[7]));
         L.rv[r] = s;
         const double a = __builtin_fabs(s);
         m = a > m ? a : m;
@@ -635,19 +852,14 @@ __device__ __forceinline__ void gesv_matrix(const GesvArgs& G, const Lds<T>& L, 
     double* xo = G.x[bi];
     T* W = (T*)(G.ws + (long long)bi * G.wstride);
 
-    // ||A||_inf and ||b||_inf
-    double an = 0.0, bn = 0.0;
+    // ||b||_inf; ||A||_inf comes with the copy
+    double bn = 0.0;
     for (int r = t; r < n; r += BT) {
-        double s = 0.0;
-        for (int c = 0; c < n; ++c) s += __builtin_fabs(A[r + (size_t)c * lda]);
-        an = s > an ? s : an;
         const double v = __builtin_fabs(bvec[r]);
         bn = v > bn ? v : bn;
     }
-    const double anorm = block_max(L, an);
     const double bnorm = block_max(L, bn);
-
-    copy_in(A, lda, W, ldw, n, L.ut);
+    const double anorm = copy_in(A, lda, W, ldw, n, L);
     for (int r = t; r < n; r += BT) L.perm[r] = r;
     __syncthreads();
 
