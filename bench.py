@@ -278,6 +278,193 @@ def mixed_leg(steps: int, warmup: int, count: int = 8000, seed: int = 3):
     return res
 
 
+def timed_calls(call, restore, steps: int, warmup: int, stream):
+    """Mean ms of `call` over `steps` runs, each bracketed by its own HIP events
+    on `stream`; `restore` (untimed, may be None) runs before every call."""
+    import torch
+
+    for _ in range(warmup):
+        if restore:
+            restore()
+        call()
+    torch.cuda.synchronize()
+    total = 0.0
+    for _ in range(steps):
+        if restore:
+            restore()
+        start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        start.record(stream)
+        call()
+        stop.record(stream)
+        stop.synchronize()
+        total += start.elapsed_time(stop)
+    return total / steps
+
+
+def hbm_entry(alg_bytes: float, ms: float) -> dict:
+    gbs = alg_bytes / (ms / 1e3) / 1e9
+    return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "alg_bytes": alg_bytes}
+
+
+def lu_leg(steps: int, warmup: int, count: int = 1024, n: int = 64, seed: int = 1):
+    """Exact (bit-identical to MatFlat) partial-pivot LU of `count` column-major
+    n x n matrices, lda = n, N(0,1) entries; each timed call factors a fresh
+    copy of the batch."""
+    import torch
+
+    from matflat_amd import hip
+
+    dev = torch.device("cuda")
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    src = torch.randn(count * n * n, dtype=torch.float64, device=dev, generator=g)
+    a = torch.empty_like(src)
+    piv = torch.zeros(count * n, dtype=torch.int32, device=dev)
+    sign = torch.zeros(count, dtype=torch.int32, device=dev)
+    info = torch.zeros(count, dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream()
+    handle = hip.Handle()
+    handle.bind_stream(stream)
+    handle.set_math_mode(hip.MATH_EXACT)
+    ms = timed_calls(lambda: hip.dgetrf_strided_batched(count, n, n, a, n, n * n, piv, n, sign, info, handle=handle),
+                     lambda: a.copy_(src), steps, warmup, stream)
+    torch.cuda.synchronize()
+    ok = bool(torch.all(info == 0).item())
+    handle.close()
+    alg = count * (16.0 * n * n + 4 * n + 4)
+    res = {"workload": f"{count} LU factorizations (partial pivoting), n = {n}, lda = {n}, N(0,1), exact mode "
+                       "(bit-identical to MatFlat's Factorization.Lu)",
+           "ms_per_call": round(ms, 5), "factorizations_per_s": round(count / (ms / 1e3), 1),
+           "all_info_zero": ok, "roofline": hbm_entry(alg, ms)}
+    log(f"  LU n={n} x{count}: {ms:.4f} ms/call, {res['factorizations_per_s']:.0f} LU/s, "
+        f"{res['roofline']['achieved']} GB/s")
+    return res
+
+
+def adjoint_leg(steps: int, warmup: int, count: int = 50_000, n: int = 32, seed: int = 4):
+    """Forward (exact-mode SPD solve) and hand-written backward of the batched
+    SPD solve, n = 32, A = B B^T + n I."""
+    import torch
+
+    from matflat_amd import hip
+
+    dev = torch.device("cuda")
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    bm = torch.randn((count, n, n), dtype=torch.float64, device=dev, generator=g)
+    src = torch.bmm(bm, bm.transpose(1, 2).contiguous()) + n * torch.eye(n, dtype=torch.float64, device=dev)
+    del bm
+    b0 = torch.randn((count, n), dtype=torch.float64, device=dev, generator=g)
+    w = torch.randn((count, n), dtype=torch.float64, device=dev, generator=g)
+    a, x = torch.empty_like(src), torch.empty_like(b0)
+    abar, bbar = torch.empty_like(src), torch.empty_like(b0)
+    ns = torch.full((count,), n, dtype=torch.int32, device=dev)
+    step = torch.arange(count, dtype=torch.int64, device=dev)
+
+    def ptrs(t, per):
+        return (t.data_ptr() + 8 * per * step).contiguous()
+
+    info = torch.zeros(count, dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream()
+    handle = hip.Handle()
+    handle.bind_stream(stream)
+    handle.set_math_mode(hip.MATH_EXACT)
+    pa, px, pw, pab, pbb = ptrs(a, n * n), ptrs(x, n), ptrs(w, n), ptrs(abar, n * n), ptrs(bbar, n)
+
+    def restore():
+        a.copy_(src)
+        x.copy_(b0)
+
+    fwd = timed_calls(lambda: hip.dposv_vbatched(ns, pa, ns, px, info, n, handle=handle), restore, steps, warmup,
+                      stream)
+    ok_f = bool(torch.all(info == 0).item())
+    bwd = timed_calls(lambda: hip.dposv_backward_vbatched(ns, pa, ns, px, pw, pab, ns, pbb, info, n, handle=handle),
+                      None, steps, warmup, stream)
+    ok_b = bool(torch.all(info == 0).item())
+    handle.close()
+    fb = count * 8.0 * (n * (n + 1) / 2 + n * n + 2 * n)
+    bb = count * 8.0 * (2 * n * n + 3 * n)
+    res = {"workload": f"{count} SPD solves n = {n} (exact mode) and their adjoint (dL/dA, dL/db for L = w.x)",
+           "forward_ms": round(fwd, 5), "backward_ms": round(bwd, 5),
+           "forward_plus_backward_per_s": round(count / ((fwd + bwd) / 1e3), 1), "all_info_zero": ok_f and ok_b,
+           "forward_roofline": hbm_entry(fb, fwd), "backward_roofline": hbm_entry(bb, bwd)}
+    log(f"  adjoint n={n} x{count}: forward {fwd:.4f} ms ({res['forward_roofline']['achieved']} GB/s), backward "
+        f"{bwd:.4f} ms ({res['backward_roofline']['achieved']} GB/s)")
+    return res
+
+
+def gemm_leg(steps: int, warmup: int, count: int = 200_000, seed: int = 5, complex_: bool = False):
+    """C = alpha op(A) op(B) + beta C over `count` products with m, n, k iid
+    U{8..128}, op uniform over {N, T} ({N, T, C} for complex), leading
+    dimensions padded to a multiple of 4, alpha, beta ~ U(-1, 1), exact mode
+    (MatFlat's summation order)."""
+    import torch
+
+    from matflat_amd import hip
+
+    rng = np.random.default_rng(seed + (100 if complex_ else 0))
+    m, n, k = (rng.integers(8, 129, size=count).astype(np.int64) for _ in range(3))
+    ops = np.array([111, 112, 113]) if complex_ else np.array([111, 112])
+    ta, tb = rng.choice(ops, size=count), rng.choice(ops, size=count)
+    ra = np.where(ta == 111, m, k)
+    rb = np.where(tb == 111, k, n)
+    lda, ldb, ldc = (ra + 3) // 4 * 4, (rb + 3) // 4 * 4, (m + 3) // 4 * 4
+    size_a = lda * np.where(ta == 111, k, m)
+    size_b = ldb * np.where(tb == 111, n, k)
+    size_c = ldc * n
+    w = 2 if complex_ else 1
+    dev = torch.device("cuda")
+
+    def slab(sizes):
+        off = np.concatenate([[0], np.cumsum(sizes)[:-1]]) * w
+        g = torch.Generator(device=dev)
+        g.manual_seed(int(rng.integers(1 << 30)))
+        t = torch.randn(int(sizes.sum()) * w, dtype=torch.float64, device=dev, generator=g)
+        return t, (t.data_ptr() + 8 * torch.from_numpy(off).to(dev)).contiguous()
+
+    A, pa = slab(size_a)
+    Bt, pb = slab(size_b)
+    C, pc = slab(size_c)
+    alpha = torch.from_numpy(rng.uniform(-1, 1, size=count * w)).to(dev)
+    beta = torch.from_numpy(rng.uniform(-1, 1, size=count * w)).to(dev)
+
+    def i32(v):
+        return torch.from_numpy(np.ascontiguousarray(v, dtype=np.int32)).to(dev)
+
+    args = [i32(ta), i32(tb), i32(m), i32(n), i32(k), alpha, pa, i32(lda), pb, i32(ldb), beta, pc, i32(ldc)]
+    info = torch.zeros(count, dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream()
+    handle = hip.Handle()
+    handle.bind_stream(stream)
+    mf, nf, kf = m.astype(np.float64), n.astype(np.float64), k.astype(np.float64)
+    flops = (8.0 if complex_ else 2.0) * mf * nf * kf
+    alg = 8.0 * w * (mf * kf + kf * nf + 2 * mf * nf)
+    # per product: the slower of its FP64 flops at 78.6 TFLOP/s and its bytes at 8 TB/s
+    ceiling_ms = float(np.sum(np.maximum(flops / 78.6e12, alg / 8e12))) * 1e3
+    name = "complex128" if complex_ else "fp64"
+    res = {"workload": f"{count} {name} products C = alpha op(A) op(B) + beta C, m, n, k ~ U{{8..128}}, "
+                       f"op in {{{'N, T, C' if complex_ else 'N, T'}}}, ld padded to 4, alpha, beta ~ U(-1, 1)",
+           "model_ceiling_ms": round(ceiling_ms, 4), "alg_bytes": float(alg.sum()), "flops": float(flops.sum())}
+    for mode_name, mode in (("fast", hip.MATH_FAST), ("exact", hip.MATH_EXACT)):
+        handle.set_math_mode(mode)
+        ms = timed_calls(lambda: hip.gemm_vbatched(*args, info, 128, 128, 128, complex_=complex_, handle=handle),
+                         None, steps, warmup, stream)
+        ok = bool(torch.all(info == 0).item())
+        res[mode_name] = {"ms_per_call": round(ms, 5), "products_per_s": round(count / (ms / 1e3), 1),
+                          "tflops": round(float(flops.sum()) / (ms / 1e3) / 1e12, 2), "all_info_zero": ok,
+                          "roofline": hbm_entry(float(alg.sum()), ms),
+                          "frac_of_model_ceiling": round(ceiling_ms / ms, 4),
+                          "note": ("FP64 matrix cores, within 4 k eps of MatFlat" if mode == hip.MATH_FAST
+                                   else "MatFlat's summation order, bit-identical")}
+        log(f"  gemm {name} {mode_name} x{count}: {ms:.4f} ms/call, {res[mode_name]['tflops']} TFLOP/s, "
+            f"ceiling {ceiling_ms:.3f} ms ({res[mode_name]['frac_of_model_ceiling']:.3f})")
+    handle.close()
+    del A, Bt, C
+    torch.cuda.empty_cache()
+    return res
+
+
 def cpu_baseline(buf_host, offsets, ns, ldas, xoff, b_host, max_seconds=20.0):
     import oracle
 
@@ -313,6 +500,8 @@ def main() -> int:
     ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the mixed-precision LU-solve comparison (n = 128..512)")
+    ap.add_argument("--no-legs", action="store_true",
+                    help="skip the LU (n = 64), adjoint (n = 32) and GEMM legs")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 256 MiB device copy (reference for HBM byte counters)")
     args = ap.parse_args()
@@ -390,6 +579,14 @@ def main() -> int:
     mixed = None
     if not args.no_mixed:
         mixed = mixed_leg(max(3, args.steps // 4), 1)
+    legs = None
+    if not args.no_legs:
+        del buf, b
+        torch.cuda.empty_cache()
+        log("other configurations")
+        legs = {"lu_n64": lu_leg(args.steps, 2), "spd_solve_adjoint_n32": adjoint_leg(args.steps, 2),
+                "gemm_fp64": gemm_leg(max(3, args.steps // 4), 1),
+                "gemm_complex128": gemm_leg(max(3, args.steps // 4), 1, complex_=True)}
     out = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -412,6 +609,7 @@ def main() -> int:
         "math": args.math,
         "secondary": secondary,
         "mixed_precision": mixed,
+        "other_configs": legs,
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -172,15 +172,16 @@ int mfh_dgetrs_vbatched(mfh_handle* h, int count, const int* n, const double* co
  * MatFlat's.  A product whose c pointer equals its a or b pointer gets
  * info = -1 (MatFlat's "In-place operation is not supported").
  * m, n, k <= 4096 (larger k is fine; sizes are not staged whole).
- * Complex data is interleaved (re, im) doubles; alpha/beta point to host
- * arrays of count (re, im) pairs... see mfh_zgemm_vbatched. */
+ * alpha and beta are device arrays of count doubles, one per product. */
 int mfh_dgemm_vbatched(mfh_handle* h, int count, const int* transa, const int* transb,
                        const int* m, const int* n, const int* k, const double* alpha,
                        const double* const* a, const int* lda, const double* const* b,
                        const int* ldb, const double* beta, double* const* c, const int* ldc,
                        int* info, int max_m, int max_n, int max_k);
-/* alpha and beta: device arrays of count doubles (dgemm) or count (re, im)
- * pairs (zgemm), one per product. */
+/* Complex version: a, b and c hold interleaved (re, im) doubles (the layout
+ * of System.Numerics.Complex and hipDoubleComplex); alpha and beta are device
+ * arrays of count (re, im) pairs, one per product; transa/transb may also be
+ * MFH_CONJTRANS (113) or MFH_CONJNOTRANS (114). */
 int mfh_zgemm_vbatched(mfh_handle* h, int count, const int* transa, const int* transb,
                        const int* m, const int* n, const int* k, const double* alpha,
                        const double* const* a, const int* lda, const double* const* b,
@@ -191,7 +192,7 @@ int mfh_zgemm_vbatched(mfh_handle* h, int count, const int* transa, const int* t
  * No single MatFlat counterpart: it composes Factorization.Lu and the LU
  * solve of LuInverse (above) in lower precision and refines in FP64.
  * Factors each A (FP64, n <= MFH_BLOCKED_MAX_N) in FP32 with partial
- * pivoting (BF16x3 split MFMA trailing update when n >= 256, FP32
+ * pivoting (BF16x6 split MFMA trailing update when n >= 256, FP32
  * accumulation), then iterates x <- x + A_lo^-1 (b - A x) with the residual,
  * the update and the test in FP64 until
  * ||b - A x||_inf / (||A||_inf ||x||_inf + ||b||_inf) <= tol_per_n * n or

@@ -19,11 +19,6 @@
 
 namespace {
 
-__global__ void k_reset_adj(int* p, int n)
-{
-    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
-}
-
 __device__ __forceinline__ void fold(int i, int c, double p, double& s, double& pend)
 {
     if (i & 1) {
@@ -37,22 +32,20 @@ __device__ __forceinline__ void fold(int i, int c, double p, double& s, double& 
 }
 
 template <int T>
-__global__ __launch_bounds__(T) void k_posv_backward(MfhAdjointBatch B, int count, int* counter, int max_n)
+__global__ __launch_bounds__(T) void k_posv_backward(MfhAdjointBatch B, int count, int max_n)
 {
     extern __shared__ double lds[];
-    __shared__ int s_item;
     const int tid = threadIdx.x;
-    for (;;) {
-        if (tid == 0) s_item = atomicAdd(counter, 1);
-        __syncthreads();
-        const int b = s_item;
-        __syncthreads();
-        if (b >= count) break;
-        const int n = B.n[b];
-        const int lda = B.lda[b];
-        const int ldab = B.ldabar[b];
-        if (n < 1 || n > T || n > max_n || lda < n || ldab < n || B.l[b] == nullptr || B.x[b] == nullptr ||
-            B.xbar[b] == nullptr || B.abar[b] == nullptr || B.bbar[b] == nullptr) {
+    // Persistent workgroups walk the batch with a fixed stride; every value
+    // that steers a loop or branch around a barrier is wave-uniform.
+    for (int b = blockIdx.x; b < count; b += gridDim.x) {
+        const int n = __builtin_amdgcn_readfirstlane(B.n[b]);
+        const int lda = __builtin_amdgcn_readfirstlane(B.lda[b]);
+        const int ldab = __builtin_amdgcn_readfirstlane(B.ldabar[b]);
+        const int bad = __builtin_amdgcn_readfirstlane(
+            (n < 1 || n > T || n > max_n || lda < n || ldab < n || B.l[b] == nullptr || B.x[b] == nullptr ||
+             B.xbar[b] == nullptr || B.abar[b] == nullptr || B.bbar[b] == nullptr) ? 1 : 0);
+        if (bad) {
             if (tid == 0) B.info[b] = -3;
             continue;
         }
@@ -116,7 +109,7 @@ __global__ __launch_bounds__(T) void k_posv_backward(MfhAdjointBatch B, int coun
 }
 
 template <int T>
-int launch_t(mfh_handle* h, int count, const MfhAdjointBatch& B, int max_n, int* counter)
+int launch_t(mfh_handle* h, int count, const MfhAdjointBatch& B, int max_n)
 {
     const size_t lds = (size_t)max_n * max_n * 8 + (size_t)max_n * 16 + 64;
     MFH_HIP_CHECK(hipFuncSetAttribute((const void*)k_posv_backward<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -126,7 +119,7 @@ int launch_t(mfh_handle* h, int count, const MfhAdjointBatch& B, int max_n, int*
     if (per_cu < 1) per_cu = 1;
     long grid = (long)per_cu * h->num_cus;
     if (grid > count) grid = count;
-    hipLaunchKernelGGL(k_posv_backward<T>, dim3((unsigned)grid), dim3(T), lds, h->stream, B, count, counter, max_n);
+    hipLaunchKernelGGL(k_posv_backward<T>, dim3((unsigned)grid), dim3(T), lds, h->stream, B, count, max_n);
     MFH_HIP_CHECK(hipGetLastError());
     return MFH_OK;
 }
@@ -135,11 +128,6 @@ int launch_t(mfh_handle* h, int count, const MfhAdjointBatch& B, int max_n, int*
 
 int mfh_launch_posv_backward(mfh_handle* h, int count, const MfhAdjointBatch& B, int max_n)
 {
-    int st = mfh_ws_reserve(h, 256);
-    if (st) return st;
-    int* counter = (int*)h->ws;
-    hipLaunchKernelGGL(k_reset_adj, dim3(1), dim3(64), 0, h->stream, counter, 1);
-    MFH_HIP_CHECK(hipGetLastError());
-    if (max_n <= 64) return launch_t<64>(h, count, B, max_n, counter);
-    return launch_t<128>(h, count, B, max_n, counter);
+    if (max_n <= 64) return launch_t<64>(h, count, B, max_n);
+    return launch_t<128>(h, count, B, max_n);
 }

@@ -13,8 +13,9 @@
 //    (k-major), so each operand byte is read from L2 once per tile and each
 //    LDS value feeds 4 products;
 //  * the tiles of all products form one work list (device-side prefix sum
-//    of per-product tile counts) that persistent workgroups drain through an
-//    atomic counter, so one launch covers any mix of shapes.
+//    of per-product tile counts, then a tile -> product map) that
+//    persistent workgroups walk with a fixed stride, so one launch covers
+//    any mix of shapes with no atomics and no search per tile.
 // No product is ever contracted into an FMA (-ffp-contract=off), so with
 // alpha = 1 and beta = 0 the output is bit-identical to MatFlat's.
 #include "mfh_device.h"
@@ -30,8 +31,8 @@ constexpr int KC = 16;
 constexpr int GT = 64;  // threads per workgroup
 
 struct GemmWs {
-    int* tiles;    // per-product tile count, then exclusive scan (count + 1)
-    int* counter;  // [0] work counter
+    int* tiles;  // per-product tile count, then exclusive scan (count + 1)
+    int* map;    // tile -> product (tiles[count] entries)
 };
 
 __device__ inline bool is_trans(int t) { return t == MFH_TRANS || t == MFH_CONJTRANS; }
@@ -40,7 +41,6 @@ __device__ inline bool is_conj(int t) { return t == MFH_CONJTRANS || t == MFH_CO
 __global__ void k_gemm_validate(MfhGemmBatch B, int count, GemmWs ws, int max_m, int max_n, int cplx)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) ws.counter[0] = 0;
     if (i >= count) return;
     const int ta = B.transa[i], tb = B.transb[i];
     const int m = B.m[i], n = B.n[i], k = B.k[i];
@@ -91,15 +91,12 @@ __global__ __launch_bounds__(1024) void k_scan(int* v, int count)
     if (t == 1023) v[count] = part[1023];
 }
 
-__device__ inline int find_product(const int* scan, int count, int tile)
+// map[t] = p for every tile t of product p.
+__global__ void k_gemm_map(GemmWs ws, int count)
 {
-    int lo = 0, hi = count - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (scan[mid] <= tile) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= count) return;
+    for (int t = ws.tiles[p]; t < ws.tiles[p + 1]; ++t) ws.map[t] = p;
 }
 
 // Loads a KC x 32 chunk of op(X) into LDS as S[kk][r] (k-major).  op(X) is
@@ -129,17 +126,11 @@ __global__ __launch_bounds__(GT) void k_dgemm_exact(MfhGemmBatch B, int count, G
 {
     __shared__ double As[KC][TM + 1];
     __shared__ double Bs[KC][TN + 1];
-    __shared__ int s_tile;
     const int t = threadIdx.x;
     const int tr = t % 8, tc = t / 8;
     const int total = ws.tiles[count];
-    for (;;) {
-        if (t == 0) s_tile = atomicAdd(ws.counter, 1);
-        __syncthreads();
-        const int tile = s_tile;
-        __syncthreads();
-        if (tile >= total) break;
-        const int p = find_product(ws.tiles, count, tile);
+    for (int tile = blockIdx.x; tile < total; tile += gridDim.x) {
+        const int p = ws.map[tile];
         const int local = tile - ws.tiles[p];
         const int m = B.m[p], n = B.n[p], k = B.k[p];
         const int mt = (m + TM - 1) / TM;
@@ -283,17 +274,11 @@ __global__ __launch_bounds__(GT) void k_zgemm_exact(MfhGemmBatch B, int count, G
 {
     __shared__ Cplx As[KC][TM + 1];
     __shared__ Cplx Bs[KC][TN + 1];
-    __shared__ int s_tile;
     const int t = threadIdx.x;
     const int tr = t % 8, tc = t / 8;
     const int total = ws.tiles[count];
-    for (;;) {
-        if (t == 0) s_tile = atomicAdd(ws.counter, 1);
-        __syncthreads();
-        const int tile = s_tile;
-        __syncthreads();
-        if (tile >= total) break;
-        const int p = find_product(ws.tiles, count, tile);
+    for (int tile = blockIdx.x; tile < total; tile += gridDim.x) {
+        const int p = ws.map[tile];
         const int local = tile - ws.tiles[p];
         const int m = B.m[p], n = B.n[p], k = B.k[p];
         const int mt = (m + TM - 1) / TM;
@@ -350,25 +335,320 @@ __global__ __launch_bounds__(GT) void k_zgemm_exact(MfhGemmBatch B, int count, G
     }
 }
 
+// ---- FAST mode: FP64 matrix cores ------------------------------------------
+// With MFH_MATH_FAST the products run on v_mfma_f64_16x16x4f64 instead of
+// MatFlat's dot order; results agree with MatFlat to rounding (|error| <=
+// k eps |op(A)| |op(B)| per element).  One wavefront owns a 32x32 tile of C
+// and computes its transpose D = C^T = op(B)^T op(A)^T as 2x2 (or fewer, at
+// the matrix edge) 16x16 blocks, so that each lane's result column is a run
+// of 16 consecutive rows of C and the stores coalesce.  Operands go straight
+// from global memory into the MFMA operand registers -- lane l holds
+// element (row l%16, k l/16) of its 16x4 fragment -- with no LDS and no
+// barriers: the four wavefronts of a workgroup are independent workers on
+// the strided tile list.  Rows outside the matrix load a clamped address
+// and contribute zero; k runs in steps of 4 with a masked tail step.
+typedef double mfh_d4 __attribute__((ext_vector_type(4)));
+
+constexpr int FW = 4;  // wavefronts per workgroup
+
+// One operand side: NB fragments of 16 rows each.  Element (r, kk) of op(X)
+// is X[kk + r*ld] when kmajor, else X[r + kk*ld].
+template <typename V, int NB>
+struct Side {
+    const V* p[NB];  // this lane's element at the current k step
+    bool ok[NB];     // lane row inside the matrix
+    long inc;        // elements per k step of 4
+};
+
+template <typename V, int NB>
+__device__ __forceinline__ Side<V, NB> side_at(const V* X, int ld, bool kmajor, int r0, int rows, int lane)
+{
+    Side<V, NB> S;
+    const int kl = lane >> 4;
+#pragma unroll
+    for (int f = 0; f < NB; ++f) {
+        int r = r0 + 16 * f + (lane & 15);
+        S.ok[f] = r < rows;
+        r = r < rows ? r : rows - 1;
+        S.p[f] = X + (kmajor ? ((long)kl + (long)r * ld) : ((long)r + (long)kl * ld));
+    }
+    S.inc = kmajor ? 4 : 4L * ld;
+    return S;
+}
+
+__device__ __forceinline__ double masked(double v, bool ok) { return ok ? v : 0.0; }
+__device__ __forceinline__ Cplx masked(Cplx v, bool ok, double sgn)
+{
+    return Cplx{ok ? v.re : 0.0, ok ? sgn * v.im : 0.0};
+}
+
+// acc[bi][bj] += op(B)^T-fragment(bj) x op(A)^T-fragment(bi) over k steps
+// [s0, s0 + steps) of 4; `tail` valid k values (1..3) in the last step when
+// nonzero.
+template <int NBI, int NBJ, int U>
+__device__ __forceinline__ void dmain(Side<double, NBI>& SA, Side<double, NBJ>& SB, mfh_d4 (&acc)[NBI][NBJ], int steps)
+{
+    int s = 0;
+    for (; s + U <= steps; s += U) {
+        double av[U][NBI], bv[U][NBJ];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+#pragma unroll
+            for (int f = 0; f < NBI; ++f) av[u][f] = masked(SA.p[f][u * SA.inc], SA.ok[f]);
+#pragma unroll
+            for (int f = 0; f < NBJ; ++f) bv[u][f] = masked(SB.p[f][u * SB.inc], SB.ok[f]);
+        }
+#pragma unroll
+        for (int f = 0; f < NBI; ++f) SA.p[f] += U * SA.inc;
+#pragma unroll
+        for (int f = 0; f < NBJ; ++f) SB.p[f] += U * SB.inc;
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int i = 0; i < NBI; ++i)
+#pragma unroll
+                for (int j = 0; j < NBJ; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[u][j], av[u][i], acc[i][j], 0, 0, 0);
+    }
+    for (; s < steps; ++s) {
+        double av[NBI], bv[NBJ];
+#pragma unroll
+        for (int f = 0; f < NBI; ++f) av[f] = masked(SA.p[f][0], SA.ok[f]);
+#pragma unroll
+        for (int f = 0; f < NBJ; ++f) bv[f] = masked(SB.p[f][0], SB.ok[f]);
+#pragma unroll
+        for (int f = 0; f < NBI; ++f) SA.p[f] += SA.inc;
+#pragma unroll
+        for (int f = 0; f < NBJ; ++f) SB.p[f] += SB.inc;
+#pragma unroll
+        for (int i = 0; i < NBI; ++i)
+#pragma unroll
+            for (int j = 0; j < NBJ; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[j], av[i], acc[i][j], 0, 0, 0);
+    }
+}
+
+template <int NBI, int NBJ>
+__device__ __forceinline__ void dtile(const MfhGemmBatch& B, int p, int i0, int j0, int lane)
+{
+    const int m = B.m[p], n = B.n[p], k = B.k[p];
+    const bool ta = B.transa[p] == MFH_TRANS, tb = B.transb[p] == MFH_TRANS;
+    const int lda = B.lda[p], ldb = B.ldb[p];
+    // op(A)(i, kk): NoTrans A[i + kk*lda], Trans A[kk + i*lda];
+    // op(B)(kk, j): NoTrans B[kk + j*ldb], Trans B[j + kk*ldb].
+    Side<double, NBI> SA = side_at<double, NBI>(B.a[p], lda, ta, i0, m, lane);
+    Side<double, NBJ> SB = side_at<double, NBJ>(B.b[p], ldb, !tb, j0, n, lane);
+    mfh_d4 acc[NBI][NBJ];
+#pragma unroll
+    for (int i = 0; i < NBI; ++i)
+#pragma unroll
+        for (int j = 0; j < NBJ; ++j) acc[i][j] = mfh_d4{0.0, 0.0, 0.0, 0.0};
+    const int full = k >> 2, tail = k & 3;
+    dmain<NBI, NBJ, 4>(SA, SB, acc, full);
+    if (tail) {
+        const bool kin = (lane >> 4) < tail;
+        double av[NBI], bv[NBJ];
+#pragma unroll
+        for (int f = 0; f < NBI; ++f) av[f] = masked(kin ? SA.p[f][0] : 0.0, SA.ok[f]);
+#pragma unroll
+        for (int f = 0; f < NBJ; ++f) bv[f] = masked(kin ? SB.p[f][0] : 0.0, SB.ok[f]);
+#pragma unroll
+        for (int i = 0; i < NBI; ++i)
+#pragma unroll
+            for (int j = 0; j < NBJ; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[j], av[i], acc[i][j], 0, 0, 0);
+    }
+    const double alpha = B.alpha[p], beta = B.beta[p];
+    double* C = B.c[p];
+    const long ldc = B.ldc[p];
+#pragma unroll
+    for (int i = 0; i < NBI; ++i) {
+        const int gi = i0 + 16 * i + (lane & 15);
+#pragma unroll
+        for (int j = 0; j < NBJ; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int gj = j0 + 16 * j + (lane >> 4) + 4 * r;
+                if (gi < m && gj < n) {
+                    double* cp = C + gi + gj * ldc;
+                    double v = alpha * acc[i][j][r];
+                    if (beta != 0.0) v = __builtin_fma(beta, *cp, v);
+                    *cp = v;
+                }
+            }
+    }
+}
+
+// Complex: re += Bre Are - Bim Aim, im += Bre Aim + Bim Are, four real MFMAs
+// per block; a conjugated operand has its imaginary part negated on load.
+template <int NBI, int NBJ>
+__device__ __forceinline__ void zstep(const Cplx (&av)[NBI], const Cplx (&bv)[NBJ], mfh_d4 (&re)[NBI][NBJ],
+                                      mfh_d4 (&im)[NBI][NBJ])
+{
+#pragma unroll
+    for (int i = 0; i < NBI; ++i)
+#pragma unroll
+        for (int j = 0; j < NBJ; ++j) {
+            re[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[j].re, av[i].re, re[i][j], 0, 0, 0);
+            im[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[j].re, av[i].im, im[i][j], 0, 0, 0);
+            re[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-bv[j].im, av[i].im, re[i][j], 0, 0, 0);
+            im[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[j].im, av[i].re, im[i][j], 0, 0, 0);
+        }
+}
+
+template <int NBI, int NBJ>
+__device__ __forceinline__ void ztile(const MfhGemmBatch& B, int p, int i0, int j0, int lane)
+{
+    const int m = B.m[p], n = B.n[p], k = B.k[p];
+    const int opa = B.transa[p], opb = B.transb[p];
+    const int lda = B.lda[p], ldb = B.ldb[p];
+    const double sa = is_conj(opa) ? -1.0 : 1.0, sb = is_conj(opb) ? -1.0 : 1.0;
+    Side<Cplx, NBI> SA = side_at<Cplx, NBI>((const Cplx*)B.a[p], lda, is_trans(opa), i0, m, lane);
+    Side<Cplx, NBJ> SB = side_at<Cplx, NBJ>((const Cplx*)B.b[p], ldb, !is_trans(opb), j0, n, lane);
+    mfh_d4 re[NBI][NBJ], im[NBI][NBJ];
+#pragma unroll
+    for (int i = 0; i < NBI; ++i)
+#pragma unroll
+        for (int j = 0; j < NBJ; ++j) {
+            re[i][j] = mfh_d4{0.0, 0.0, 0.0, 0.0};
+            im[i][j] = mfh_d4{0.0, 0.0, 0.0, 0.0};
+        }
+    const int full = k >> 2, tail = k & 3;
+    constexpr int U = 2;
+    int s = 0;
+    for (; s + U <= full; s += U) {
+        Cplx av[U][NBI], bv[U][NBJ];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+#pragma unroll
+            for (int f = 0; f < NBI; ++f) av[u][f] = masked(SA.p[f][u * SA.inc], SA.ok[f], sa);
+#pragma unroll
+            for (int f = 0; f < NBJ; ++f) bv[u][f] = masked(SB.p[f][u * SB.inc], SB.ok[f], sb);
+        }
+#pragma unroll
+        for (int f = 0; f < NBI; ++f) SA.p[f] += U * SA.inc;
+#pragma unroll
+        for (int f = 0; f < NBJ; ++f) SB.p[f] += U * SB.inc;
+#pragma unroll
+        for (int u = 0; u < U; ++u) zstep<NBI, NBJ>(av[u], bv[u], re, im);
+    }
+    for (; s < full; ++s) {
+        Cplx av[NBI], bv[NBJ];
+#pragma unroll
+        for (int f = 0; f < NBI; ++f) av[f] = masked(SA.p[f][0], SA.ok[f], sa);
+#pragma unroll
+        for (int f = 0; f < NBJ; ++f) bv[f] = masked(SB.p[f][0], SB.ok[f], sb);
+#pragma unroll
+        for (int f = 0; f < NBI; ++f) SA.p[f] += SA.inc;
+#pragma unroll
+        for (int f = 0; f < NBJ; ++f) SB.p[f] += SB.inc;
+        zstep<NBI, NBJ>(av, bv, re, im);
+    }
+    if (tail) {
+        const bool kin = (lane >> 4) < tail;
+        const Cplx zero{0.0, 0.0};
+        Cplx av[NBI], bv[NBJ];
+#pragma unroll
+        for (int f = 0; f < NBI; ++f) av[f] = masked(kin ? SA.p[f][0] : zero, SA.ok[f], sa);
+#pragma unroll
+        for (int f = 0; f < NBJ; ++f) bv[f] = masked(kin ? SB.p[f][0] : zero, SB.ok[f], sb);
+        zstep<NBI, NBJ>(av, bv, re, im);
+    }
+    const double ar = B.alpha[2 * p], ai = B.alpha[2 * p + 1];
+    const double br = B.beta[2 * p], bi = B.beta[2 * p + 1];
+    const bool beta_zero = br == 0.0 && bi == 0.0;
+    Cplx* C = (Cplx*)B.c[p];
+    const long ldc = B.ldc[p];
+#pragma unroll
+    for (int i = 0; i < NBI; ++i) {
+        const int gi = i0 + 16 * i + (lane & 15);
+#pragma unroll
+        for (int j = 0; j < NBJ; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int gj = j0 + 16 * j + (lane >> 4) + 4 * r;
+                if (gi < m && gj < n) {
+                    Cplx* cp = C + gi + gj * ldc;
+                    const double xr = re[i][j][r], xi = im[i][j][r];
+                    double vr = ar * xr - ai * xi, vi = ar * xi + ai * xr;
+                    if (!beta_zero) {
+                        const Cplx c0 = *cp;
+                        vr += br * c0.re - bi * c0.im;
+                        vi += br * c0.im + bi * c0.re;
+                    }
+                    *cp = Cplx{vr, vi};
+                }
+            }
+    }
+}
+
+template <bool CPLX>
+__global__ __launch_bounds__(64 * FW) void k_gemm_mfma(MfhGemmBatch B, int count, GemmWs ws)
+{
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int total = ws.tiles[count];
+    for (int tile = blockIdx.x * FW + wave; tile < total; tile += gridDim.x * FW) {
+        const int p = __builtin_amdgcn_readfirstlane(ws.map[tile]);
+        const int local = tile - ws.tiles[p];
+        const int m = B.m[p], n = B.n[p];
+        const int mt = (m + TM - 1) / TM;
+        const int i0 = (local % mt) * TM, j0 = (local / mt) * TN;
+        const int shape = (m - i0 > 16 ? 2 : 0) + (n - j0 > 16 ? 1 : 0);
+        if (CPLX) {
+            switch (shape) {
+            case 3: ztile<2, 2>(B, p, i0, j0, lane); break;
+            case 2: ztile<2, 1>(B, p, i0, j0, lane); break;
+            case 1: ztile<1, 2>(B, p, i0, j0, lane); break;
+            default: ztile<1, 1>(B, p, i0, j0, lane); break;
+            }
+        } else {
+            switch (shape) {
+            case 3: dtile<2, 2>(B, p, i0, j0, lane); break;
+            case 2: dtile<2, 1>(B, p, i0, j0, lane); break;
+            case 1: dtile<1, 2>(B, p, i0, j0, lane); break;
+            default: dtile<1, 1>(B, p, i0, j0, lane); break;
+            }
+        }
+    }
+}
+
 }  // namespace
 
 int mfh_launch_gemm(mfh_handle* h, int count, const MfhGemmBatch& B, int max_m, int max_n, int max_k, bool cplx)
 {
     (void)max_k;
-    size_t bytes = sizeof(int) * ((size_t)count + 2 + 64);
-    int st = mfh_ws_reserve(h, bytes);
+    const long maxtiles = (long)count * ((max_m + TM - 1) / TM) * ((max_n + TN - 1) / TN);
+    if (maxtiles > 0x7fffffffL - 64) return mfh_fail(MFH_ERR_UNSUPPORTED, "too many GEMM tiles in one call");
+    const size_t head = ((size_t)count + 1 + 63) / 64 * 64;
+    int st = mfh_ws_reserve(h, sizeof(int) * (head + (size_t)maxtiles));
     if (st) return st;
-    GemmWs ws{(int*)h->ws, (int*)h->ws + count + 1};
+    GemmWs ws{(int*)h->ws, (int*)h->ws + head};
     hipLaunchKernelGGL(k_gemm_validate, dim3((count + 255) / 256), dim3(256), 0, h->stream, B, count, ws, max_m,
                        max_n, cplx ? 1 : 0);
     MFH_HIP_CHECK(hipGetLastError());
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, h->stream, ws.tiles, count);
     MFH_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_gemm_map, dim3((count + 255) / 256), dim3(256), 0, h->stream, ws, count);
+    MFH_HIP_CHECK(hipGetLastError());
+    if (h->math_mode == MFH_MATH_FAST) {
+        int per_cu = 0;
+        if (cplx) MFH_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_gemm_mfma<true>, 64 * FW, 0));
+        else MFH_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_gemm_mfma<false>, 64 * FW, 0));
+        if (per_cu < 1) per_cu = 1;
+        long grid = (long)per_cu * h->num_cus;
+        const long groups = (maxtiles + FW - 1) / FW;
+        if (grid > groups) grid = groups;
+        if (grid < 1) grid = 1;
+        if (cplx) hipLaunchKernelGGL(k_gemm_mfma<true>, dim3((unsigned)grid), dim3(64 * FW), 0, h->stream, B, count, ws);
+        else hipLaunchKernelGGL(k_gemm_mfma<false>, dim3((unsigned)grid), dim3(64 * FW), 0, h->stream, B, count, ws);
+        MFH_HIP_CHECK(hipGetLastError());
+        return MFH_OK;
+    }
     int per_cu = 0;
     if (cplx) MFH_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_zgemm_exact, GT, 0));
     else MFH_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_dgemm_exact, GT, 0));
     if (per_cu < 1) per_cu = 1;
-    long maxtiles = (long)count * ((max_m + TM - 1) / TM) * ((max_n + TN - 1) / TN);
     long grid = (long)per_cu * h->num_cus;
     if (grid > maxtiles) grid = maxtiles;
     if (grid < 1) grid = 1;

@@ -7,7 +7,7 @@ import torch
 
 import oracle
 from gpu_util import Packed, assert_bits_equal, f64, i32
-from helpers import rand_buf, rand_cbuf, spd_upper_buf
+from helpers import rand_buf, rand_cbuf, spd_upper_buf, view
 from test_oracle import GEMM_CASES, LU_CASES
 
 pytestmark = pytest.mark.gpu
@@ -312,6 +312,59 @@ def test_gemm_vbatched_bit_exact(complex_):
         fn = oracle.zgemm if complex_ else oracle.dgemm
         fn(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, want, ldc)
         assert_bits_equal(gc[i], want, f"gemm {i} {(ta, tb, m, n, k)}")
+
+
+def _op(buf, ld, t, rows, cols):
+    """op(X) as a dense rows x cols array from a column-major buffer; t is the
+    CBLAS transpose code (111 N, 112 T, 113 C, 114 conj without transpose)."""
+    if t in (111, 114):
+        x = view(buf, rows, cols, ld)
+    else:
+        x = view(buf, cols, rows, ld).T
+    return np.conj(x) if t in (113, 114) else x
+
+
+@pytest.mark.parametrize("complex_", [False, True])
+def test_gemm_fast_mode_within_k_eps(complex_):
+    """MFH_MATH_FAST runs the products on FP64 matrix cores; each product
+    must satisfy ||C - C_ref||_F <= 4 k eps (|alpha| ||op(A)||_F ||op(B)||_F
+    + |beta| ||C0||_F) against the oracle's C_ref (MatFlat's order).  Rows
+    and columns outside m x n (the ld padding) must be untouched."""
+    hip = hipmod()
+    recs = gemm_batch(complex_, 29, 300, shapes=[(m, n, k) for (m, n, k, *_r) in GEMM_CASES])
+    dt = torch.complex128 if complex_ else torch.float64
+    A = Packed([r[8] for r in recs], dtype=dt)
+    Bm = Packed([r[9] for r in recs], dtype=dt)
+    C = Packed([r[10] for r in recs], dtype=dt)
+    if complex_:
+        al = f64(np.array([[complex(r[11]).real, complex(r[11]).imag] for r in recs]).reshape(-1))
+        be = f64(np.array([[complex(r[12]).real, complex(r[12]).imag] for r in recs]).reshape(-1))
+    else:
+        al = f64([r[11] for r in recs])
+        be = f64([r[12] for r in recs])
+    info = i32(np.full(len(recs), 99))
+    col = lambda j: i32([r[j] for r in recs])  # noqa: E731
+    h = hip.Handle()
+    h.set_math_mode(hip.MATH_FAST)
+    hip.gemm_vbatched(col(0), col(1), col(2), col(3), col(4), al, A.ptrs, col(5), Bm.ptrs, col(6), be, C.ptrs,
+                      col(7), info, 128, 128, 128, complex_=complex_, handle=h)
+    torch.cuda.synchronize()
+    h.close()
+    gc = C.all()
+    assert np.all(info.cpu().numpy() == 0)
+    eps = np.finfo(np.float64).eps
+    for i, (ta, tb, m, n, k, lda, ldb, ldc, a, b, c, alpha, beta) in enumerate(recs):
+        want = c.copy()
+        fn = oracle.zgemm if complex_ else oracle.dgemm
+        fn(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, want, ldc)
+        got = gc[i]
+        err = np.linalg.norm(view(got, m, n, ldc) - view(want, m, n, ldc))
+        opa, opb = _op(a, lda, ta, m, k), _op(b, ldb, tb, k, n)
+        scale = abs(alpha) * np.linalg.norm(opa) * np.linalg.norm(opb) + abs(beta) * np.linalg.norm(view(c, m, n, ldc))
+        assert err <= 4 * k * eps * scale, (i, ta, tb, m, n, k, err, scale)
+        # padding rows of C are not written
+        for j in range(n):
+            assert_bits_equal(got[j * ldc + m:(j + 1) * ldc], c[j * ldc + m:(j + 1) * ldc], f"pad {i}")
 
 
 def test_gemm_aliasing_rejected_per_product():
