@@ -31,17 +31,26 @@ constexpr int KC = 16;
 constexpr int GT = 64;  // threads per workgroup
 
 struct GemmWs {
-    int* tiles;  // per-product tile count, then exclusive scan (count + 1)
+    int* tiles;  // exclusive prefix of per-product tile counts (count + 1)
+    int* cnt;    // per-product tile count (count)
+    int* bsum;   // tile total per chunk of SCAN_CHUNK products
     int* map;    // tile -> product (tiles[count] entries)
 };
+
+// The tile list is built by two kernels over chunks of SCAN_CHUNK products:
+// validate + chunk-local scan, then chunk offsets + tile -> product map.
+constexpr int SCAN_T = 256;
+constexpr int SCAN_E = 16;
+constexpr int SCAN_CHUNK = SCAN_T * SCAN_E;
 
 __device__ inline bool is_trans(int t) { return t == MFH_TRANS || t == MFH_CONJTRANS; }
 __device__ inline bool is_conj(int t) { return t == MFH_CONJTRANS || t == MFH_CONJNOTRANS; }
 
-__global__ void k_gemm_validate(MfhGemmBatch B, int count, GemmWs ws, int max_m, int max_n, int cplx)
+// Checks product i in Blas.MulMatMat's argument order (Blas.MulMatMat.cs:
+// 230-304), stores info[i] and returns its number of ts x ts tiles of C (0
+// when it is invalid).
+__device__ inline int gemm_validate(const MfhGemmBatch& B, int i, int max_m, int max_n, int cplx, int ts)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= count) return;
     const int ta = B.transa[i], tb = B.transb[i];
     const int m = B.m[i], n = B.n[i], k = B.k[i];
     int bad = 0;
@@ -60,43 +69,75 @@ __global__ void k_gemm_validate(MfhGemmBatch B, int count, GemmWs ws, int max_m,
     else if (B.ldc[i] < m) bad = -15;
     else if ((const double*)B.c[i] == B.a[i] || (const double*)B.c[i] == B.b[i]) bad = -1;
     B.info[i] = bad;
-    ws.tiles[i] = bad ? 0 : ((m + TM - 1) / TM) * ((n + TN - 1) / TN);
+    return bad ? 0 : ((m + ts - 1) / ts) * ((n + ts - 1) / ts);
 }
 
-// Exclusive scan of count ints in place (one workgroup of 1024 threads),
-// writing the total to tiles[count].
-__global__ __launch_bounds__(1024) void k_scan(int* v, int count)
+// Exclusive scan of one int per thread over a workgroup of SCAN_T threads.
+__device__ inline int block_exclusive_scan(int v, int* part, int& total)
 {
-    __shared__ int part[1024];
     const int t = threadIdx.x;
-    const int per = (count + 1023) / 1024;
-    const int lo = t * per;
-    const int hi = lo + per < count ? lo + per : count;
-    int s = 0;
-    for (int i = lo; i < hi; ++i) s += v[i];
-    part[t] = s;
+    part[t] = v;
     __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        int add = (t >= off) ? part[t - off] : 0;
+    for (int off = 1; off < SCAN_T; off <<= 1) {
+        const int add = t >= off ? part[t - off] : 0;
         __syncthreads();
         part[t] += add;
         __syncthreads();
     }
-    int run = (t == 0) ? 0 : part[t - 1];
-    for (int i = lo; i < hi; ++i) {
-        int x = v[i];
-        v[i] = run;
-        run += x;
-    }
-    if (t == 1023) v[count] = part[1023];
+    total = part[SCAN_T - 1];
+    const int excl = part[t] - v;
+    __syncthreads();
+    return excl;
 }
 
-// map[t] = p for every tile t of product p.
-__global__ void k_gemm_map(GemmWs ws, int count)
+__global__ __launch_bounds__(SCAN_T) void k_gemm_validate_scan(MfhGemmBatch B, int count, GemmWs ws, int max_m,
+                                                               int max_n, int cplx, int ts)
 {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= count) return;
-    for (int t = ws.tiles[p]; t < ws.tiles[p + 1]; ++t) ws.map[t] = p;
+    __shared__ int part[SCAN_T];
+    const int base = blockIdx.x * SCAN_CHUNK + threadIdx.x * SCAN_E;
+    int c[SCAN_E];
+    int sum = 0;
+#pragma unroll
+    for (int e = 0; e < SCAN_E; ++e) {
+        c[e] = base + e < count ? gemm_validate(B, base + e, max_m, max_n, cplx, ts) : 0;
+        sum += c[e];
+    }
+    int total;
+    int run = block_exclusive_scan(sum, part, total);
+#pragma unroll
+    for (int e = 0; e < SCAN_E; ++e)
+        if (base + e < count) {
+            ws.tiles[base + e] = run;
+            ws.cnt[base + e] = c[e];
+            run += c[e];
+        }
+    if (threadIdx.x == 0) ws.bsum[blockIdx.x] = total;
+}
+
+// Adds the chunk offset (sum of the preceding chunks' totals) to each
+// product's prefix, writes the grand total to tiles[count] and fills
+// map[t] = p for every tile t of product p.
+__global__ __launch_bounds__(SCAN_T) void k_gemm_map(GemmWs ws, int count, int nchunks)
+{
+    __shared__ int part[SCAN_T];
+    int before = 0, all = 0;
+    for (int c = threadIdx.x; c < nchunks; c += SCAN_T) {
+        const int v = ws.bsum[c];
+        all += v;
+        if (c < (int)blockIdx.x) before += v;
+    }
+    int sum_before, sum_all;
+    block_exclusive_scan(before, part, sum_before);
+    block_exclusive_scan(all, part, sum_all);
+    const int base = blockIdx.x * SCAN_CHUNK;
+    if (blockIdx.x == 0 && threadIdx.x == 0) ws.tiles[count] = sum_all;
+    for (int i = threadIdx.x; i < SCAN_CHUNK; i += SCAN_T) {
+        const int p = base + i;
+        if (p >= count) break;
+        const int t0 = ws.tiles[p] + sum_before;
+        ws.tiles[p] = t0;
+        for (int t = 0; t < ws.cnt[p]; ++t) ws.map[t0 + t] = p;
+    }
 }
 
 // Loads a KC x 32 chunk of op(X) into LDS as S[kk][r] (k-major).  op(X) is
@@ -350,6 +391,9 @@ __global__ __launch_bounds__(GT) void k_zgemm_exact(MfhGemmBatch B, int count, G
 typedef double mfh_d4 __attribute__((ext_vector_type(4)));
 
 constexpr int FW = 4;  // wavefronts per workgroup
+#ifndef MFH_GEMM_U
+#define MFH_GEMM_U 2
+#endif
 
 // One operand side: NB fragments of 16 rows each.  Element (r, kk) of op(X)
 // is X[kk + r*ld] when kmajor, else X[r + kk*ld].
@@ -382,50 +426,63 @@ __device__ __forceinline__ Cplx masked(Cplx v, bool ok, double sgn)
     return Cplx{ok ? v.re : 0.0, ok ? sgn * v.im : 0.0};
 }
 
-// acc[bi][bj] += op(B)^T-fragment(bj) x op(A)^T-fragment(bi) over k steps
-// [s0, s0 + steps) of 4; `tail` valid k values (1..3) in the last step when
-// nonzero.
+// acc[bi][bj] += op(B)^T-fragment(bj) x op(A)^T-fragment(bi) over `steps`
+// k steps of 4, in groups of U steps.  The loads of group g+1 are issued
+// before the MFMAs of group g (the last group re-loads itself rather than
+// run past the matrix).
 template <int NBI, int NBJ, int U>
-__device__ __forceinline__ void dmain(Side<double, NBI>& SA, Side<double, NBJ>& SB, mfh_d4 (&acc)[NBI][NBJ], int steps)
+struct DGroup {
+    double a[U][NBI], b[U][NBJ];
+};
+
+template <int NBI, int NBJ, int U>
+__device__ __forceinline__ void dload(DGroup<NBI, NBJ, U>& G, const Side<double, NBI>& SA,
+                                      const Side<double, NBJ>& SB, long step)
 {
-    int s = 0;
-    for (; s + U <= steps; s += U) {
-        double av[U][NBI], bv[U][NBJ];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
+    for (int u = 0; u < U; ++u) {
 #pragma unroll
-            for (int f = 0; f < NBI; ++f) av[u][f] = masked(SA.p[f][u * SA.inc], SA.ok[f]);
+        for (int f = 0; f < NBI; ++f) G.a[u][f] = masked(SA.p[f][(step + u) * SA.inc], SA.ok[f]);
 #pragma unroll
-            for (int f = 0; f < NBJ; ++f) bv[u][f] = masked(SB.p[f][u * SB.inc], SB.ok[f]);
-        }
-#pragma unroll
-        for (int f = 0; f < NBI; ++f) SA.p[f] += U * SA.inc;
-#pragma unroll
-        for (int f = 0; f < NBJ; ++f) SB.p[f] += U * SB.inc;
-#pragma unroll
-        for (int u = 0; u < U; ++u)
-#pragma unroll
-            for (int i = 0; i < NBI; ++i)
-#pragma unroll
-                for (int j = 0; j < NBJ; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[u][j], av[u][i], acc[i][j], 0, 0, 0);
+        for (int f = 0; f < NBJ; ++f) G.b[u][f] = masked(SB.p[f][(step + u) * SB.inc], SB.ok[f]);
     }
-    for (; s < steps; ++s) {
-        double av[NBI], bv[NBJ];
+}
+
+template <int NBI, int NBJ, int U>
+__device__ __forceinline__ void dmfma(const DGroup<NBI, NBJ, U>& G, mfh_d4 (&acc)[NBI][NBJ])
+{
 #pragma unroll
-        for (int f = 0; f < NBI; ++f) av[f] = masked(SA.p[f][0], SA.ok[f]);
-#pragma unroll
-        for (int f = 0; f < NBJ; ++f) bv[f] = masked(SB.p[f][0], SB.ok[f]);
-#pragma unroll
-        for (int f = 0; f < NBI; ++f) SA.p[f] += SA.inc;
-#pragma unroll
-        for (int f = 0; f < NBJ; ++f) SB.p[f] += SB.inc;
+    for (int u = 0; u < U; ++u)
 #pragma unroll
         for (int i = 0; i < NBI; ++i)
 #pragma unroll
             for (int j = 0; j < NBJ; ++j)
-                acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[j], av[i], acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(G.b[u][j], G.a[u][i], acc[i][j], 0, 0, 0);
+}
+
+template <int NBI, int NBJ, int U>
+__device__ __forceinline__ void dmain(Side<double, NBI>& SA, Side<double, NBJ>& SB, mfh_d4 (&acc)[NBI][NBJ], int steps)
+{
+    const int groups = steps / U;
+    if (groups > 0) {
+        DGroup<NBI, NBJ, U> cur, nxt;
+        dload(cur, SA, SB, 0);
+        for (int g = 0; g < groups; ++g) {
+            const int ng = g + 1 < groups ? g + 1 : g;
+            dload(nxt, SA, SB, (long)ng * U);
+            dmfma(cur, acc);
+            cur = nxt;
+        }
     }
+    DGroup<NBI, NBJ, 1> one;
+    for (int s = groups * U; s < steps; ++s) {
+        dload(one, SA, SB, s);
+        dmfma(one, acc);
+    }
+#pragma unroll
+    for (int f = 0; f < NBI; ++f) SA.p[f] += (long)steps * SA.inc;
+#pragma unroll
+    for (int f = 0; f < NBJ; ++f) SB.p[f] += (long)steps * SB.inc;
 }
 
 template <int NBI, int NBJ>
@@ -444,7 +501,7 @@ __device__ __forceinline__ void dtile(const MfhGemmBatch& B, int p, int i0, int 
 #pragma unroll
         for (int j = 0; j < NBJ; ++j) acc[i][j] = mfh_d4{0.0, 0.0, 0.0, 0.0};
     const int full = k >> 2, tail = k & 3;
-    dmain<NBI, NBJ, 4>(SA, SB, acc, full);
+    dmain<NBI, NBJ, MFH_GEMM_U>(SA, SB, acc, full);
     if (tail) {
         const bool kin = (lane >> 4) < tail;
         double av[NBI], bv[NBJ];
@@ -465,7 +522,7 @@ __device__ __forceinline__ void dtile(const MfhGemmBatch& B, int p, int i0, int 
     for (int i = 0; i < NBI; ++i) {
         const int gi = i0 + 16 * i + (lane & 15);
 #pragma unroll
-        for (int j = 0; j < NBJ; ++j)
+        for (int j = 0; j < NBJ; ++j) {
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int gj = j0 + 16 * j + (lane >> 4) + 4 * r;
@@ -476,6 +533,7 @@ __device__ __forceinline__ void dtile(const MfhGemmBatch& B, int p, int i0, int 
                     *cp = v;
                 }
             }
+        }
     }
 }
 
@@ -582,9 +640,15 @@ __device__ __forceinline__ void ztile(const MfhGemmBatch& B, int p, int i0, int 
     }
 }
 
+// Tile edge of the FAST kernels: 32 (2x2 MFMA blocks per wavefront).  A
+// 64x64 tile per wavefront (4x4 blocks) measured no faster on the GEMM
+// workload with m, n, k ~ U{8..128}, at twice the registers.
+constexpr int FAST_TS = 32;
+
 template <bool CPLX>
 __global__ __launch_bounds__(64 * FW) void k_gemm_mfma(MfhGemmBatch B, int count, GemmWs ws)
 {
+    constexpr int TS = FAST_TS;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int total = ws.tiles[count];
@@ -592,21 +656,22 @@ __global__ __launch_bounds__(64 * FW) void k_gemm_mfma(MfhGemmBatch B, int count
         const int p = __builtin_amdgcn_readfirstlane(ws.map[tile]);
         const int local = tile - ws.tiles[p];
         const int m = B.m[p], n = B.n[p];
-        const int mt = (m + TM - 1) / TM;
-        const int i0 = (local % mt) * TM, j0 = (local / mt) * TN;
-        const int shape = (m - i0 > 16 ? 2 : 0) + (n - j0 > 16 ? 1 : 0);
+        const int mt = (m + TS - 1) / TS;
+        const int i0 = (local % mt) * TS, j0 = (local / mt) * TS;
+        // 16-row / 16-column blocks of this tile that hold any of C
+        const int nbi = min(TS / 16, (m - i0 + 15) / 16), nbj = min(TS / 16, (n - j0 + 15) / 16);
         if (CPLX) {
-            switch (shape) {
-            case 3: ztile<2, 2>(B, p, i0, j0, lane); break;
-            case 2: ztile<2, 1>(B, p, i0, j0, lane); break;
-            case 1: ztile<1, 2>(B, p, i0, j0, lane); break;
+            switch (nbi * 2 + nbj) {
+            case 6: ztile<2, 2>(B, p, i0, j0, lane); break;
+            case 5: ztile<2, 1>(B, p, i0, j0, lane); break;
+            case 4: ztile<1, 2>(B, p, i0, j0, lane); break;
             default: ztile<1, 1>(B, p, i0, j0, lane); break;
             }
         } else {
-            switch (shape) {
-            case 3: dtile<2, 2>(B, p, i0, j0, lane); break;
-            case 2: dtile<2, 1>(B, p, i0, j0, lane); break;
-            case 1: dtile<1, 2>(B, p, i0, j0, lane); break;
+            switch (nbi * 2 + nbj) {
+            case 6: dtile<2, 2>(B, p, i0, j0, lane); break;
+            case 5: dtile<2, 1>(B, p, i0, j0, lane); break;
+            case 4: dtile<1, 2>(B, p, i0, j0, lane); break;
             default: dtile<1, 1>(B, p, i0, j0, lane); break;
             }
         }
@@ -618,20 +683,23 @@ __global__ __launch_bounds__(64 * FW) void k_gemm_mfma(MfhGemmBatch B, int count
 int mfh_launch_gemm(mfh_handle* h, int count, const MfhGemmBatch& B, int max_m, int max_n, int max_k, bool cplx)
 {
     (void)max_k;
-    const long maxtiles = (long)count * ((max_m + TM - 1) / TM) * ((max_n + TN - 1) / TN);
+    const bool fast = h->math_mode == MFH_MATH_FAST;
+    const int ts = fast ? FAST_TS : TM;
+    const long maxtiles = (long)count * ((max_m + ts - 1) / ts) * ((max_n + ts - 1) / ts);
     if (maxtiles > 0x7fffffffL - 64) return mfh_fail(MFH_ERR_UNSUPPORTED, "too many GEMM tiles in one call");
-    const size_t head = ((size_t)count + 1 + 63) / 64 * 64;
-    int st = mfh_ws_reserve(h, sizeof(int) * (head + (size_t)maxtiles));
+    const int nchunks = (count + SCAN_CHUNK - 1) / SCAN_CHUNK;
+    auto up = [](size_t v) { return (v + 63) / 64 * 64; };
+    const size_t o_cnt = up((size_t)count + 1), o_bsum = o_cnt + up(count), o_map = o_bsum + up(nchunks);
+    int st = mfh_ws_reserve(h, sizeof(int) * (o_map + (size_t)maxtiles));
     if (st) return st;
-    GemmWs ws{(int*)h->ws, (int*)h->ws + head};
-    hipLaunchKernelGGL(k_gemm_validate, dim3((count + 255) / 256), dim3(256), 0, h->stream, B, count, ws, max_m,
-                       max_n, cplx ? 1 : 0);
+    int* w = (int*)h->ws;
+    GemmWs ws{w, w + o_cnt, w + o_bsum, w + o_map};
+    hipLaunchKernelGGL(k_gemm_validate_scan, dim3(nchunks), dim3(SCAN_T), 0, h->stream, B, count, ws, max_m, max_n,
+                       cplx ? 1 : 0, ts);
     MFH_HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, h->stream, ws.tiles, count);
+    hipLaunchKernelGGL(k_gemm_map, dim3(nchunks), dim3(SCAN_T), 0, h->stream, ws, count, nchunks);
     MFH_HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_gemm_map, dim3((count + 255) / 256), dim3(256), 0, h->stream, ws, count);
-    MFH_HIP_CHECK(hipGetLastError());
-    if (h->math_mode == MFH_MATH_FAST) {
+    if (fast) {
         int per_cu = 0;
         if (cplx) MFH_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_gemm_mfma<true>, 64 * FW, 0));
         else MFH_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_gemm_mfma<false>, 64 * FW, 0));
