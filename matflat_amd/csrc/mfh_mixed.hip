@@ -83,7 +83,8 @@ struct GesvArgs {
 // LDS layout (bytes), for row stride L = ldw:
 //   ut    [L][NB] T       U12 transposed (k contiguous); also the copy tile
 //   l11   [NB][NB] T
-//   cand  [2][NWAVE][NB] T, jrow [2][NB] T, ckey/cidx [2][NWAVE]: pivot search
+//   cand  [2][NWAVE][NB] T, ckey/cidx [2][NWAVE]: pivot search
+//   dst   [L] int         final row of each panel row
 //   xv, rv [L] double     solution, residual / right-hand side
 //   yv    [L] T           solve vector
 //   perm  [L] int         row permutation (position -> original row)
@@ -94,7 +95,6 @@ struct Lds {
     T* ut;
     T* l11;
     T* cand;     // [2][NWAVE][NB]: each wavefront's pivot candidate row, by column parity
-    T* jrow;     // [2][NB]: row k0+j before the exchange, by column parity
     double* ckey;  // [2][NWAVE] candidate keys
     int* cidx;     // [2][NWAVE] candidate rows
     double* xv;
@@ -105,6 +105,7 @@ struct Lds {
     int* mapk;  // rows touched by a panel's swaps (at most 64) ...
     int* mapv;  // ... and the row whose content each receives
     int* mapn;  // number of such rows
+    int* dst;   // [ldw]: final row of panel row k0 + i
     double* rkey;
     int* ridx;
     double* scal;
@@ -125,7 +126,7 @@ __host__ __device__ size_t gesv_lds_bytes(int ldw)
     size_t s = 0;
     s += ut_elems(ldw, NB) * sizeof(T);
     s += (size_t)NB * NB * sizeof(T);
-    s += 2 * (size_t)NWAVE * NB * sizeof(T) + 2 * (size_t)NB * sizeof(T);
+    s += 2 * (size_t)NWAVE * NB * sizeof(T);
     s = (s + 15) & ~(size_t)15;
     s += 2 * NWAVE * sizeof(double) + 2 * NWAVE * sizeof(int);
     s = (s + 15) & ~(size_t)15;
@@ -135,6 +136,7 @@ __host__ __device__ size_t gesv_lds_bytes(int ldw)
     s += (size_t)ldw * sizeof(int);
     s += (size_t)NB * sizeof(int);
     s += 2 * 64 * sizeof(int) + 16;
+    s += (size_t)ldw * sizeof(int);
     s = (s + 15) & ~(size_t)15;
     s += NWAVE * sizeof(double) + NWAVE * sizeof(int) + 8 * sizeof(double) + 16;
     return s;
@@ -152,8 +154,6 @@ __device__ __forceinline__ Lds<T> carve(char* base, int ldw)
     s += (size_t)NB * NB * sizeof(T);
     L.cand = (T*)(base + s);
     s += 2 * (size_t)NWAVE * NB * sizeof(T);
-    L.jrow = (T*)(base + s);
-    s += 2 * (size_t)NB * sizeof(T);
     s = (s + 15) & ~(size_t)15;
     L.ckey = (double*)(base + s);
     s += 2 * NWAVE * sizeof(double);
@@ -177,6 +177,8 @@ __device__ __forceinline__ Lds<T> carve(char* base, int ldw)
     s += 64 * sizeof(int);
     L.mapn = (int*)(base + s);
     s += 16;
+    L.dst = (int*)(base + s);
+    s += (size_t)ldw * sizeof(int);
     s = (s + 15) & ~(size_t)15;
     L.rkey = (double*)(base + s);
     s += NWAVE * sizeof(double);
@@ -308,12 +310,14 @@ __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
 
 // Column J of the panel (J a template parameter, so every register index is
 // static; a runtime column loop would index the row registers dynamically).
-// One barrier per column: before it, each wavefront's argmax winner
-// publishes its whole row and row k0+J publishes itself (double-buffered by
-// column parity); after it every thread picks the pivot row and does its part
-// of the exchange and the elimination.
+// Rows stay in their threads: a row chosen as a pivot stops taking part
+// ("live" goes false) instead of being exchanged, and the final row order is
+// applied once, when the panel is written back.  One barrier per column:
+// before it each wavefront's argmax winner publishes its entries J.. of the
+// row (double-buffered by column parity); after it every live row eliminates
+// with the pivot row.  pivs[J] records the original row chosen at column J.
 template <typename T, int J>
-__device__ __forceinline__ void panel_cols(T (&P)[Prec<T>::NB], int r, bool mine, int n, int k0, int nb,
+__device__ __forceinline__ void panel_cols(T (&P)[Prec<T>::NB], int r, bool& live, int n, int k0, int nb,
                                            const Lds<T>& L, int& zero)
 {
     constexpr int NB = Prec<T>::NB;
@@ -321,53 +325,86 @@ __device__ __forceinline__ void panel_cols(T (&P)[Prec<T>::NB], int r, bool mine
         if (J < nb) {
             const int t = threadIdx.x, w = t >> 6;
             T* cand = L.cand + (J & 1) * NWAVE * NB;
-            T* jrow = L.jrow + (J & 1) * NB;
             unsigned long long* ckey = (unsigned long long*)(L.ckey + (J & 1) * NWAVE);
-            const unsigned long long key = wave_max_u64(pivot_pack(mine && r >= k0 + J, P[J], r));
+            const unsigned long long key = wave_max_u64(pivot_pack(live, P[J], r));
             if (r == (int)~(unsigned)key) {
 #pragma unroll
-                for (int c = 0; c < NB; ++c) cand[w * NB + c] = P[c];
+                for (int c = J; c < NB; ++c) cand[w * NB + c] = P[c];
                 ckey[w] = key;
-            }
-            if (r == k0 + J) {
-#pragma unroll
-                for (int c = 0; c < NB; ++c) jrow[c] = P[c];
             }
             __syncthreads();
             unsigned long long best = ckey[0];
 #pragma unroll
             for (int ww = 1; ww < NWAVE; ++ww) best = ckey[ww] > best ? ckey[ww] : best;
-            int p = (int)~(unsigned)best;  // thread p - k0 holds row p: wavefront (p - k0) / 64
-            if ((best >> 32) == 0 || p < k0 + J || p >= n) p = k0 + J;  // keeps every row index in range
-            const T* prow = cand + ((p - k0) >> 6) * NB;
-            T pr[NB];
-#pragma unroll
-            for (int c = 0; c < NB; ++c) pr[c] = prow[c];
+            // a live row always wins (n - k0 - J >= 1 rows are live); the
+            // clamp only keeps the index in range whatever the keys hold
+            int p = (int)~(unsigned)best;
+            p = p < k0 ? k0 : (p >= n ? n - 1 : p);
+            const T* prow = cand + ((p - k0) >> 6) * NB;  // thread p - k0 is in wavefront (p - k0) / 64
+            const T d = prow[J];
             if (t == 0) L.pivs[J] = p;
-            if (r == p && p != k0 + J) {
-#pragma unroll
-                for (int c = 0; c < NB; ++c) P[c] = jrow[c];
-            }
-            if (r == k0 + J) {
-#pragma unroll
-                for (int c = 0; c < NB; ++c) P[c] = pr[c];
-            }
-            const T d = pr[J];
             if (d == (T)0 && zero == 0) zero = k0 + J + 1;
-            if (mine && r > k0 + J) {
-                const T l = (d != (T)0) ? P[J] / d : P[J];
-                P[J] = l;
+            if (r == p) live = false;
+            // Branch-free: a row that is not live eliminates with l = 0 and
+            // keeps its values (a divergent branch here makes the compiler
+            // copy the whole row in and out of the branch).  The pivot row's
+            // entries are finite unless A holds inf or NaN, when the factors
+            // carry NaN whichever way they are computed.
+            const T q = (d != (T)0) ? P[J] / d : P[J];
+            const T l = live ? q : (T)0;
+            P[J] = live ? q : P[J];
 #pragma unroll
-                for (int c = J + 1; c < NB; ++c) P[c] -= l * pr[c];
-            }
-            panel_cols<T, J + 1>(P, r, mine, n, k0, nb, L, zero);
+            for (int c = J + 1; c < NB; ++c) P[c] -= l * prow[c];
+            panel_cols<T, J + 1>(P, r, live, n, k0, nb, L, zero);
         }
     }
 }
 
+// Final row order of a panel (wavefront 0).  The pivot of column J (original
+// row pivs[J]) goes to row k0+J; the rows of k0..k0+nb-1 that were not
+// chosen move, in order, to the rows below the block that the pivots came
+// from.  This is a permutation of S = {k0..k0+nb-1} u {pivs[J]} (at most
+// 2 nb <= 64 rows), stored as a map: row mapk[i] receives the old content of
+// row mapv[i].  It also updates perm (position -> original row of A) and
+// dst (final row of each row k0 + i; dst must hold the identity on entry).
+template <typename T>
+__device__ __forceinline__ void compose_rows(int k0, int nb, const Lds<T>& L)
+{
+    const int lane = threadIdx.x;
+    const int q = lane < nb ? L.pivs[lane] : -1;
+    const bool below = lane < nb && q >= k0 + nb;  // pivot taken from below the block
+    bool chosen = false;                           // row k0 + lane was some column's pivot
+    for (int j = 0; j < nb; ++j) chosen |= __builtin_amdgcn_readlane(q, j) == k0 + lane;
+    const bool displaced = lane < nb && !chosen;
+    const unsigned long long lower = (1ull << lane) - 1;
+    const int rv = __popcll(__ballot(below) & lower), rd = __popcll(__ballot(displaced) & lower);
+    const int nd = __popcll(__ballot(displaced));
+    if (lane < nb) {
+        L.mapk[lane] = k0 + lane;
+        L.mapv[lane] = q;
+    }
+    if (below) L.mapk[nb + rv] = q;
+    if (displaced) L.mapv[nb + rd] = k0 + lane;
+    const int cnt = nb + nd;
+    if (lane == 0) *L.mapn = cnt;
+    __builtin_amdgcn_wave_barrier();
+    int mk = 0, mv = 0, old = 0;
+    if (lane < cnt) {
+        mk = L.mapk[lane];
+        mv = L.mapv[lane];
+        old = L.perm[mv];
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (lane < cnt) {
+        L.perm[mk] = old;
+        L.dst[mv - k0] = mk;
+    }
+}
+
 // Factor columns k0 .. k0+nb-1 of rows k0 .. n-1 (one row per thread, in
-// registers), record the pivots in pivs[], write the panel back, and leave
-// the unit-lower L11 in l11.  Returns the first zero pivot (1-based) or 0.
+// registers), compose the panel's row order (mapk/mapv, perm), write the
+// panel rows back to their final rows, and leave the unit-lower L11 in l11.
+// Returns the first zero pivot (1-based) or 0.
 template <typename T>
 __device__ __forceinline__ int panel(T* W, int ldw, int n, int k0, int nb, const Lds<T>& L)
 {
@@ -386,68 +423,41 @@ __device__ __forceinline__ int panel(T* W, int ldw, int n, int k0, int nb, const
         }
     }
     int zero = 0;
-    panel_cols<T, 0>(P, r, mine, n, k0, nb, L, zero);
+    bool live = mine;
+    panel_cols<T, 0>(P, r, live, n, k0, nb, L, zero);
+    if (mine) L.dst[t] = r;
+    __syncthreads();
+    if (t < 64) compose_rows<T>(k0, nb, L);
+    __syncthreads();
     if (mine) {
+        const int d = L.dst[t];
         if (nb == NB) {
 #pragma unroll
-            for (int c = 0; c < NB; ++c) W[(size_t)r * ldw + k0 + c] = P[c];
+            for (int c = 0; c < NB; ++c) W[(size_t)d * ldw + k0 + c] = P[c];
         } else {
             for (int c = 0; c < nb; ++c) {
                 T v = P[0];
 #pragma unroll
                 for (int q = 1; q < NB; ++q) v = (c == q) ? P[q] : v;
-                W[(size_t)r * ldw + k0 + c] = v;
+                W[(size_t)d * ldw + k0 + c] = v;
             }
         }
-    }
-    if (t < nb) {
+        if (d < k0 + nb) {
 #pragma unroll
-        for (int c = 0; c < NB; ++c) L.l11[t * NB + c] = P[c];
+            for (int c = 0; c < NB; ++c) L.l11[(d - k0) * NB + c] = P[c];
+        }
     }
-    __syncthreads();
     return zero;
 }
 
-// Apply the panel's row swaps to the columns outside it and to perm.  The
-// swap sequence (k0+j <-> pivs[j], in order) only moves rows of the set
-// S = {k0 .. k0+nb-1} u {pivs[j]} (at most 64 rows): wavefront 0 composes it
-// into a map (row mapk[i] receives the old content of row mapv[i]) with
-// ballots, then every thread moves its column through registers: all loads
-// first, then the stores, so the memory accesses are independent.
+// Apply the panel's row order (the map composed by compose_rows) to the
+// columns outside the panel: every thread moves its column through
+// registers, all loads first, then the stores, so the memory accesses are
+// independent.
 template <typename T>
 __device__ __forceinline__ void swap_rows(T* W, int ldw, int n, int k0, int nb, const Lds<T>& L)
 {
     const int t = threadIdx.x;
-    if (t < 64) {
-        const int lane = t;
-        int key = lane < nb ? k0 + lane : -1;  // keys never move; values do
-        int val = key;
-        int cnt = nb;
-        for (int j = 0; j < nb; ++j) {
-            const int p = L.pivs[j];
-            if (p == k0 + j) continue;
-            const unsigned long long m = __ballot(key == p);
-            int b;
-            if (m == 0) {
-                b = cnt++;
-                if (lane == b) key = val = p;
-            } else {
-                b = __builtin_ctzll(m);
-            }
-            const int va = __shfl(val, j, 64), vb = __shfl(val, b, 64);
-            if (lane == j) val = vb;
-            if (lane == b) val = va;
-        }
-        const int oldperm = lane < cnt ? L.perm[val] : 0;
-        if (lane < cnt) {
-            L.mapk[lane] = key;
-            L.mapv[lane] = val;
-        }
-        if (lane == 0) *L.mapn = cnt;
-        // perm: position key now holds the original row that was at val
-        __builtin_amdgcn_wave_barrier();
-        if (lane < cnt) L.perm[key] = oldperm;
-    }
     __syncthreads();
     const int cnt = __builtin_amdgcn_readfirstlane(*L.mapn);
     const int c = t;  // n <= BT: one column per thread
