@@ -108,6 +108,7 @@ int mfh_get_fast_waves(mfh_handle* handle, int row_blocks, int* waves);
 #define MFH_KERNEL_GETRS_EXACT 6
 #define MFH_KERNEL_GESV_MIXED 7
 #define MFH_KERNEL_GESV_F64 8
+#define MFH_KERNEL_GESV_STEP 9
 int mfh_set_timing(mfh_handle* handle, int enable);
 int mfh_get_timing(mfh_handle* handle, int capacity, int* n_records, int* kind, int* rows,
                    float* ms);

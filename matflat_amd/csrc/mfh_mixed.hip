@@ -78,6 +78,12 @@ struct GesvArgs {
     long long wstride;
     int ldw;           // row stride of W (elements)
     int* counter;      // work counter (zeroed before launch)
+    // state kept between the factorization and the refinement launches
+    int* perm;         // [count][ldw] position -> original row of A
+    double* resid;     // [count][ldw] last residual b - A x
+    double* norms;     // [count][2] ||A||_inf, ||b||_inf
+    int* zero;         // [count] first zero pivot (1-based) or 0
+    int* state;        // [count] 0 active, 1 finished, -1 invalid
 };
 
 // LDS layout (bytes), for row stride L = ldw:
@@ -190,6 +196,21 @@ __device__ __forceinline__ Lds<T> carve(char* base, int ldw)
     return L;
 }
 
+// Pointers read from the caller's pointer arrays are generic; reading
+// through them emits flat loads, which count against the LDS counter as
+// well, so every LDS wait would also wait for them.  Data of A, b, x and the
+// workspace is in global memory: these casts let the loads be global loads.
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T* gmem(const T* p)
+{
+    return (const __attribute__((address_space(1))) T*)p;
+}
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* gmem_w(T* p)
+{
+    return (__attribute__((address_space(1))) T*)p;
+}
+
 __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
 
 // A value every lane already holds, made provably uniform for the compiler:
@@ -241,7 +262,7 @@ __device__ __forceinline__ double copy_in(const double* A, int lda, T* W, int ld
 #pragma unroll
             for (int h = 0; h < 8; ++h) {
                 const int c = c0 + cq + 4 * h;
-                v[h] = (r < n && c < n) ? A[r + (size_t)c * lda] : 0.0;
+                v[h] = (r < n && c < n) ? gmem(A)[r + (size_t)c * lda] : 0.0;
             }
 #pragma unroll
             for (int h = 0; h < 8; ++h) {
@@ -275,37 +296,33 @@ __device__ __forceinline__ double copy_in(const double* A, int lda, T* W, int ld
 // pivoting's (largest |v| in single precision, lowest row on ties, NaN below
 // every number, non-candidates below NaN), packed so that one unsigned 64-bit
 // maximum finds it.
+// Pivot key of a candidate row: |v| rounded to FP32 as ordered bits + 2
+// (NaN -> 1, so it is chosen only when nothing else is left; not a
+// candidate -> 0).  Ties at FP32 resolution go to the lowest row.
 template <typename T>
-__device__ __forceinline__ unsigned long long pivot_pack(bool cand, T v, int r)
+__device__ __forceinline__ unsigned pivot_key(bool cand, T v)
 {
     const float a = __builtin_fabsf((float)v);
-    unsigned hi = 0;
-    if (cand) hi = (a != a) ? 1u : (__float_as_uint(a) + 2u);
-    return ((unsigned long long)hi << 32) | (unsigned)(~r);
+    return cand ? ((a != a) ? 1u : (__float_as_uint(a) + 2u)) : 0u;
 }
 
-#define MFH_DPP_MAX_U64(v, CTRL, RMASK)                                                                     \
-    do {                                                                                                  \
-        const unsigned lo_ = (unsigned)(v), hi_ = (unsigned)((v) >> 32);                                  \
-        const unsigned olo_ = __builtin_amdgcn_update_dpp(lo_, lo_, CTRL, RMASK, 0xF, false);             \
-        const unsigned ohi_ = __builtin_amdgcn_update_dpp(hi_, hi_, CTRL, RMASK, 0xF, false);             \
-        const unsigned long long o_ = ((unsigned long long)ohi_ << 32) | olo_;                            \
-        (v) = o_ > (v) ? o_ : (v);                                                                        \
+#define MFH_DPP_MAX_U32(v, CTRL, RMASK)                                                   \
+    do {                                                                                \
+        const unsigned o_ = __builtin_amdgcn_update_dpp((v), (v), CTRL, RMASK, 0xF, false); \
+        (v) = o_ > (v) ? o_ : (v);                                                      \
     } while (0)
 
 // Wave-wide maximum through DPP row shifts and row broadcasts (no LDS
 // round trips); every lane receives it.
-__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
+__device__ __forceinline__ unsigned wave_max_u32(unsigned v)
 {
-    MFH_DPP_MAX_U64(v, 0x111, 0xF);  // row_shr:1
-    MFH_DPP_MAX_U64(v, 0x112, 0xF);  // row_shr:2
-    MFH_DPP_MAX_U64(v, 0x114, 0xF);  // row_shr:4
-    MFH_DPP_MAX_U64(v, 0x118, 0xF);  // row_shr:8  -> lane 15 of each row: row maximum
-    MFH_DPP_MAX_U64(v, 0x142, 0xA);  // row_bcast:15 into rows 1 and 3
-    MFH_DPP_MAX_U64(v, 0x143, 0xC);  // row_bcast:31 into rows 2 and 3 -> lane 63: maximum
-    const unsigned lo = __builtin_amdgcn_readlane((unsigned)v, 63);
-    const unsigned hi = __builtin_amdgcn_readlane((unsigned)(v >> 32), 63);
-    return ((unsigned long long)hi << 32) | lo;
+    MFH_DPP_MAX_U32(v, 0x111, 0xF);  // row_shr:1
+    MFH_DPP_MAX_U32(v, 0x112, 0xF);  // row_shr:2
+    MFH_DPP_MAX_U32(v, 0x114, 0xF);  // row_shr:4
+    MFH_DPP_MAX_U32(v, 0x118, 0xF);  // row_shr:8  -> lane 15 of each row: row maximum
+    MFH_DPP_MAX_U32(v, 0x142, 0xA);  // row_bcast:15 into rows 1 and 3
+    MFH_DPP_MAX_U32(v, 0x143, 0xC);  // row_bcast:31 into rows 2 and 3 -> lane 63: maximum
+    return __builtin_amdgcn_readlane(v, 63);
 }
 
 // Column J of the panel (J a template parameter, so every register index is
@@ -326,11 +343,15 @@ __device__ __forceinline__ void panel_cols(T (&P)[Prec<T>::NB], int r, bool& liv
             const int t = threadIdx.x, w = t >> 6;
             T* cand = L.cand + (J & 1) * NWAVE * NB;
             unsigned long long* ckey = (unsigned long long*)(L.ckey + (J & 1) * NWAVE);
-            const unsigned long long key = wave_max_u64(pivot_pack(live, P[J], r));
-            if (r == (int)~(unsigned)key) {
+            // wave argmax: the largest 32-bit key, then the lowest lane that
+            // holds it (rows are in lane order, so the lowest row wins ties)
+            const unsigned k32 = pivot_key(live, P[J]);
+            const unsigned m = wave_max_u32(k32);
+            const int win = __builtin_ctzll(__ballot(k32 == m));
+            if ((t & 63) == win) {
 #pragma unroll
                 for (int c = J; c < NB; ++c) cand[w * NB + c] = P[c];
-                ckey[w] = key;
+                ckey[w] = ((unsigned long long)m << 32) | (unsigned)(~r);
             }
             __syncthreads();
             unsigned long long best = ckey[0];
@@ -733,16 +754,20 @@ __device__ __forceinline__ void trailing_f32_valu(float* W, int ldw, int n, int 
 template <typename T>
 __device__ __forceinline__ double gemv_row(const T* row, const T* y, int c0, int c1, int step)
 {
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    double s[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s[u] = 0.0;
     int c = c0;
-    for (; c + 3 * step < c1; c += 4 * step) {
-        s0 += (double)row[c] * (double)y[c];
-        s1 += (double)row[c + step] * (double)y[c + step];
-        s2 += (double)row[c + 2 * step] * (double)y[c + 2 * step];
-        s3 += (double)row[c + 3 * step] * (double)y[c + 3 * step];
+    for (; c + 7 * step < c1; c += 8 * step) {
+        T rv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) rv[u] = gmem(row)[c + u * step];
+        __builtin_amdgcn_sched_barrier(0);  // the eight loads go out together
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s[u] += (double)rv[u] * (double)y[c + u * step];
     }
-    for (; c < c1; c += step) s0 += (double)row[c] * (double)y[c];
-    return (s0 + s1) + (s2 + s3);
+    for (; c < c1; c += step) s[0] += (double)row[c] * (double)y[c];
+    return ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
 }
 
 // yv <- U^-1 L^-1 yv with the factors in W (row-major, unit-lower L), in
@@ -829,21 +854,47 @@ __device__ __forceinline__ void lu_solve(const T* W, int ldw, int n, const Lds<T
 template <typename T>
 __device__ __forceinline__ double residual(const double* A, int lda, const double* b, int n, const Lds<T>& L)
 {
-    double m = 0.0;
-    for (int r = threadIdx.x; r < n; r += BT) {
-        // eight independent partial sums keep several column loads in flight
-        double ps[8];
+    // G groups of threads split the columns (G = 4 for n <= 128, 2 for
+    // n <= 256), so every thread has a row; sixteen independent partial sums
+    // keep sixteen column loads in flight per thread.  The group sums meet
+    // in the ut region, which is free once the factorization is done.
+    const int G = n <= BT / 4 ? 4 : (n <= BT / 2 ? 2 : 1);
+    const int span = BT / G;
+    const int t = threadIdx.x;
+    const int g = t / span, r = t % span;
+    const int cper = (n + G - 1) / G;
+    const int c0 = g * cper, c1 = c0 + cper < n ? c0 + cper : n;
+    double ps[16];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) ps[u] = 0.0;
-        ps[0] = b[r];
-        int c = 0;
-        for (; c + 8 <= n; c += 8) {
+    for (int u = 0; u < 16; ++u) ps[u] = 0.0;
+    if (r < n) {
+        const auto* Ar = gmem(A + r);
+        int c = c0;
+        for (; c + 16 <= c1; c += 16) {
+            // all sixteen loads are issued before the first use: the
+            // scheduler would otherwise interleave them with the FMAs and
+            // wait for each load in turn
+            double av[16];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) ps[u] = __builtin_fma(-A[r + (size_t)(c + u) * lda], L.xv[c + u], ps[u]);
+            for (int u = 0; u < 16; ++u) av[u] = Ar[(size_t)(c + u) * lda];
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int u = 0; u < 16; ++u) ps[u] = __builtin_fma(-av[u], L.xv[c + u], ps[u]);
         }
-        for (; c < n; ++c) ps[0] = __builtin_fma(-A[r + (size_t)c * lda], L.xv[c], ps[0]);
-        const double s = ((ps[0] + ps[1]) + (ps[2] + ps[3])) + ((ps[4] + ps[5]) + (ps[6] + ps[7]));
-        L.rv[r] = s;
+        for (; c < c1; ++c) ps[0] = __builtin_fma(-Ar[(size_t)c * lda], L.xv[c], ps[0]);
+    }
+#pragma unroll
+    for (int w = 8; w >= 1; w >>= 1)
+#pragma unroll
+        for (int u = 0; u < w; ++u) ps[u] += ps[u + w];
+    double* part = (double*)L.ut;  // G * n <= BT doubles
+    if (r < n) part[g * n + r] = ps[0];
+    __syncthreads();
+    double m = 0.0;
+    for (int rr = t; rr < n; rr += BT) {
+        double s = gmem(b)[rr];
+        for (int q = 0; q < G; ++q) s += part[q * n + rr];
+        L.rv[rr] = s;
         const double a = __builtin_fabs(s);
         m = a > m ? a : m;
     }
@@ -851,21 +902,23 @@ __device__ __forceinline__ double residual(const double* A, int lda, const doubl
 }
 
 // One matrix (bi), all threads of the workgroup.
+// Factorization of matrix bi: norms, the copy into W, and the blocked LU.
+// The row order, the norms and the first zero pivot are kept for the
+// refinement launches.
 template <typename T, bool MIXED>
-__device__ __forceinline__ void gesv_matrix(const GesvArgs& G, const Lds<T>& L, int bi, int n, int lda)
+__device__ __forceinline__ void gesv_factor(const GesvArgs& G, const Lds<T>& L, int bi, int n, int lda)
 {
     constexpr int NB = Prec<T>::NB;
     const int ldw = G.ldw;
     const int t = threadIdx.x;
     const double* A = G.a[bi];
     const double* bvec = G.b[bi];
-    double* xo = G.x[bi];
     T* W = (T*)(G.ws + (long long)bi * G.wstride);
 
     // ||b||_inf; ||A||_inf comes with the copy
     double bn = 0.0;
     for (int r = t; r < n; r += BT) {
-        const double v = __builtin_fabs(bvec[r]);
+        const double v = __builtin_fabs(gmem(bvec)[r]);
         bn = v > bn ? v : bn;
     }
     const double bnorm = block_max(L, bn);
@@ -889,47 +942,65 @@ __device__ __forceinline__ void gesv_matrix(const GesvArgs& G, const Lds<T>& L, 
             }
         }
     }
-
-    // x0 = (LU)^-1 P b
-    for (int r = t; r < n; r += BT) L.yv[r] = (T)bvec[L.perm[r]];
-    __syncthreads();
-    lu_solve<T>(W, ldw, n, L);
-    for (int r = t; r < n; r += BT) L.xv[r] = (double)L.yv[r];
-    __syncthreads();
-    int it = 0;
-    double eta = 0.0;
-    for (;;) {
-        double xm = 0.0;
-        for (int r = t; r < n; r += BT) {
-            const double v = __builtin_fabs(L.xv[r]);
-            xm = v > xm ? v : xm;
-        }
-        const double xnorm = block_max(L, xm);
-        const double rnorm = residual<T>(A, lda, bvec, n, L);
-        const double den = anorm * xnorm + bnorm;
-        eta = uniform(den > 0.0 ? rnorm / den : (rnorm > 0.0 ? 1.0 : 0.0));
-        if (!MIXED || !(eta > G.tol_per_n * n) || it >= G.max_iter) break;
-        // d = (LU)^-1 P r ; x += d
-        __syncthreads();
-        for (int r = t; r < n; r += BT) L.yv[r] = (T)L.rv[L.perm[r]];
-        __syncthreads();
-        lu_solve<T>(W, ldw, n, L);
-        for (int r = t; r < n; r += BT) L.xv[r] += (double)L.yv[r];
-        __syncthreads();
-        ++it;
-    }
-    for (int r = t; r < n; r += BT) xo[r] = L.xv[r];
+    int* perm = G.perm + (long long)bi * ldw;
+    for (int r = t; r < n; r += BT) perm[r] = L.perm[r];
     if (t == 0) {
-        G.berr[bi] = eta;
-        if (MIXED) G.iters[bi] = it;
-        int inf = 0;
-        if (zero) inf = zero;
-        else if (MIXED && eta > G.tol_per_n * n) inf = n + 1;
-        G.info[bi] = inf;
+        G.norms[2 * bi] = anorm;
+        G.norms[2 * bi + 1] = bnorm;
+        G.zero[bi] = zero;
+        G.state[bi] = 0;
     }
 }
 
-// The persistent loop has a single back edge and every branch around a
+// One step of the solve for matrix bi.  Step 0: x = (LU)^-1 P b.  Step
+// it > 0: d = (LU)^-1 P r, x += d (r from the previous step).  Then
+// r = b - A x and eta = ||r|| / (||A|| ||x|| + ||b||), all in FP64; the
+// matrix is finished when eta <= tol * n, at step max_iter, or at once
+// without refinement (the plain FP64 path).
+template <typename T, bool MIXED>
+__device__ __forceinline__ void gesv_step(const GesvArgs& G, const Lds<T>& L, int bi, int n, int lda, int it)
+{
+    const int ldw = G.ldw;
+    const int t = threadIdx.x;
+    const double* A = G.a[bi];
+    const double* bvec = G.b[bi];
+    double* xo = G.x[bi];
+    const T* W = (const T*)(G.ws + (long long)bi * G.wstride);
+    const int* perm = G.perm + (long long)bi * ldw;
+    double* rs = G.resid + (long long)bi * ldw;
+    for (int r = t; r < n; r += BT) {
+        const int pr = gmem(perm)[r];
+        L.yv[r] = (T)(it == 0 ? gmem(bvec)[pr] : gmem(rs)[pr]);
+    }
+    __syncthreads();
+    lu_solve<T>(W, ldw, n, L);
+    double xm = 0.0;
+    for (int r = t; r < n; r += BT) {
+        const double v = it == 0 ? (double)L.yv[r] : gmem(xo)[r] + (double)L.yv[r];
+        L.xv[r] = v;
+        gmem_w(xo)[r] = v;
+        const double a = __builtin_fabs(v);
+        xm = a > xm ? a : xm;
+    }
+    const double xnorm = block_max(L, xm);  // its barriers also publish xv
+    const double rnorm = residual<T>(A, lda, bvec, n, L);
+    for (int r = t; r < n; r += BT) gmem_w(rs)[r] = L.rv[r];
+    const double anorm = G.norms[2 * bi], bnorm = G.norms[2 * bi + 1];
+    const double den = anorm * xnorm + bnorm;
+    const double eta = uniform(den > 0.0 ? rnorm / den : (rnorm > 0.0 ? 1.0 : 0.0));
+    const bool done = !MIXED || !(eta > G.tol_per_n * n) || it >= G.max_iter;
+    if (t == 0) {
+        G.berr[bi] = eta;
+        if (MIXED) G.iters[bi] = it;
+        if (done) {
+            const int zero = G.zero[bi];
+            G.info[bi] = zero ? zero : ((MIXED && eta > G.tol_per_n * n) ? n + 1 : 0);
+            G.state[bi] = 1;
+        }
+    }
+}
+
+// The persistent loops have a single back edge and every branch around a
 // barrier is on a wave-uniform value.
 template <typename T, bool MIXED>
 __global__ __launch_bounds__(BT) void k_gesv(GesvArgs G)
@@ -952,8 +1023,56 @@ __global__ __launch_bounds__(BT) void k_gesv(GesvArgs G)
         else if (G.b[bi] == nullptr) bad = -6;
         else if (G.x[bi] == nullptr) bad = -7;
         bad = __builtin_amdgcn_readfirstlane(bad);
-        if (bad == 0) gesv_matrix<T, MIXED>(G, L, bi, n, lda);
-        else if (t == 0) G.info[bi] = bad;
+        if (bad == 0) gesv_factor<T, MIXED>(G, L, bi, n, lda);
+        else if (t == 0) {
+            G.info[bi] = bad;
+            G.state[bi] = -1;
+        }
+        __syncthreads();
+    }
+}
+
+// LDS of the solve launches: xv, rv [ldw] double, yv [ldw] T, the group
+// partial sums of the residual [BT] double and the block-reduction slots.
+template <typename T>
+__host__ __device__ size_t step_lds_bytes(int ldw)
+{
+    return (size_t)ldw * (2 * sizeof(double) + sizeof(T)) + BT * sizeof(double) + 2 * NWAVE * sizeof(double) + 64;
+}
+
+template <typename T>
+__device__ __forceinline__ Lds<T> carve_step(char* base, int ldw)
+{
+    Lds<T> L{};
+    size_t s = 0;
+    L.xv = (double*)(base + s);
+    s += (size_t)ldw * sizeof(double);
+    L.rv = (double*)(base + s);
+    s += (size_t)ldw * sizeof(double);
+    L.ut = (T*)(base + s);  // residual group sums
+    s += BT * sizeof(double);
+    L.rkey = (double*)(base + s);
+    s += 2 * NWAVE * sizeof(double);
+    L.yv = (T*)(base + s);
+    return L;
+}
+
+// Solve step `it` for every matrix still active; workgroups walk the batch
+// with a fixed stride.  The solve and the residual are bound by memory
+// latency: this kernel keeps few registers so that several workgroups share
+// a CU and their loads overlap.
+template <typename T, bool MIXED>
+__global__ __launch_bounds__(BT) void k_gesv_step(GesvArgs G, int it)
+{
+    extern __shared__ __attribute__((aligned(16))) char lds_raw[];
+    const Lds<T> L = carve_step<T>(lds_raw, G.ldw);
+    for (int bi = blockIdx.x; bi < G.count; bi += gridDim.x) {
+        const int active = __builtin_amdgcn_readfirstlane(G.state[bi]) == 0;
+        if (active) {
+            const int n = __builtin_amdgcn_readfirstlane(G.n[bi]);
+            const int lda = __builtin_amdgcn_readfirstlane(G.lda[bi]);
+            gesv_step<T, MIXED>(G, L, bi, n, lda, it);
+        }
         __syncthreads();
     }
 }
@@ -976,25 +1095,63 @@ int launch_gesv(mfh_handle* h, GesvArgs G)
     if (st) return st;
     hipLaunchKernelGGL((k_gesv<T, MIXED>), dim3((unsigned)grid), dim3(BT), lds, h->stream, G);
     MFH_HIP_CHECK(hipGetLastError());
-    return mfh_timing_end(h);
+    st = mfh_timing_end(h);
+    if (st) return st;
+
+    const size_t slds = step_lds_bytes<T>(G.ldw);
+    MFH_HIP_CHECK(hipFuncSetAttribute((const void*)k_gesv_step<T, MIXED>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)slds));
+    per_cu = 0;
+    MFH_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_gesv_step<T, MIXED>, BT, slds));
+    if (per_cu < 1) per_cu = 1;
+    grid = (long)per_cu * h->num_cus;
+    if (grid > G.count) grid = G.count;
+    // step 0 solves; steps 1..max_iter refine what is still active
+    const int steps = MIXED ? 1 + G.max_iter : 1;
+    for (int it = 0; it < steps; ++it) {
+        st = mfh_timing_begin(h, MFH_KERNEL_GESV_STEP, G.max_n);
+        if (st) return st;
+        hipLaunchKernelGGL((k_gesv_step<T, MIXED>), dim3((unsigned)grid), dim3(BT), slds, h->stream, G, it);
+        MFH_HIP_CHECK(hipGetLastError());
+        st = mfh_timing_end(h);
+        if (st) return st;
+    }
+    return MFH_OK;
 }
 
 }  // namespace
 
-size_t mfh_gesv_ws_bytes(int count, int max_n, bool mixed)
+namespace {
+struct GesvLayout {
+    size_t per, w, perm, resid, norms, zero, state, total;
+};
+
+GesvLayout gesv_layout(int count, int max_n, bool mixed)
 {
+    auto up = [](size_t v) { return (v + 255) & ~(size_t)255; };
     const size_t ldw = (size_t)((max_n + 7) & ~7);
-    const size_t per = ((ldw * max_n * (mixed ? 4 : 8)) + 255) & ~(size_t)255;
-    return 256 + per * (size_t)count;
+    GesvLayout g;
+    g.per = up(ldw * max_n * (mixed ? 4 : 8));
+    g.w = 256;
+    g.perm = g.w + g.per * (size_t)count;
+    g.resid = g.perm + up(sizeof(int) * ldw * count);
+    g.norms = g.resid + up(sizeof(double) * ldw * count);
+    g.zero = g.norms + up(sizeof(double) * 2 * count);
+    g.state = g.zero + up(sizeof(int) * count);
+    g.total = g.state + up(sizeof(int) * count);
+    return g;
 }
+}  // namespace
+
+size_t mfh_gesv_ws_bytes(int count, int max_n, bool mixed) { return gesv_layout(count, max_n, mixed).total; }
 
 int mfh_launch_gesv_mixed(mfh_handle* h, int count, const int* n, const double* const* a, const int* lda,
                           const double* const* b, double* const* x, int max_iter, double tol_per_n, int* iters,
                           double* berr, int* info, int max_n, bool mixed)
 {
     const int ldw = (max_n + 7) & ~7;
-    const size_t per = (((size_t)ldw * max_n * (mixed ? 4 : 8)) + 255) & ~(size_t)255;
-    int st = mfh_ws_reserve(h, mfh_gesv_ws_bytes(count, max_n, mixed));
+    const GesvLayout lay = gesv_layout(count, max_n, mixed);
+    int st = mfh_ws_reserve(h, lay.total);
     if (st) return st;
     GesvArgs G;
     G.n = n;
@@ -1009,9 +1166,15 @@ int mfh_launch_gesv_mixed(mfh_handle* h, int count, const int* n, const double* 
     G.max_n = max_n;
     G.max_iter = max_iter;
     G.tol_per_n = tol_per_n;
-    G.counter = (int*)h->ws;
-    G.ws = (char*)h->ws + 256;
-    G.wstride = (long long)per;
+    char* base = (char*)h->ws;
+    G.counter = (int*)base;
+    G.ws = base + lay.w;
+    G.wstride = (long long)lay.per;
     G.ldw = ldw;
+    G.perm = (int*)(base + lay.perm);
+    G.resid = (double*)(base + lay.resid);
+    G.norms = (double*)(base + lay.norms);
+    G.zero = (int*)(base + lay.zero);
+    G.state = (int*)(base + lay.state);
     return mixed ? launch_gesv<float, true>(h, G) : launch_gesv<double, false>(h, G);
 }
