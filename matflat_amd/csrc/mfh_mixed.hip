@@ -343,39 +343,47 @@ __device__ __forceinline__ void panel_cols(T (&P)[Prec<T>::NB], int r, bool& liv
             const int t = threadIdx.x, w = t >> 6;
             T* cand = L.cand + (J & 1) * NWAVE * NB;
             unsigned long long* ckey = (unsigned long long*)(L.ckey + (J & 1) * NWAVE);
-            // wave argmax: the largest 32-bit key, then the lowest lane that
-            // holds it (rows are in lane order, so the lowest row wins ties)
-            const unsigned k32 = pivot_key(live, P[J]);
-            const unsigned m = wave_max_u32(k32);
-            const int win = __builtin_ctzll(__ballot(k32 == m));
-            if ((t & 63) == win) {
+            // wavefronts with no row of the panel only keep the barrier
+            // company; wavefront 0 always has rows (k0 < n)
+            const int nwl = (n - k0 + 63) >> 6;  // wavefronts holding rows k0 .. n-1
+            const bool wlive = wave_id() < nwl;
+            if (wlive) {
+                // wave argmax: the largest 32-bit key, then the lowest lane
+                // that holds it (rows are in lane order, so the lowest row
+                // wins ties)
+                const unsigned k32 = pivot_key(live, P[J]);
+                const unsigned m = wave_max_u32(k32);
+                const int win = __builtin_ctzll(__ballot(k32 == m));
+                if ((t & 63) == win) {
 #pragma unroll
-                for (int c = J; c < NB; ++c) cand[w * NB + c] = P[c];
-                ckey[w] = ((unsigned long long)m << 32) | (unsigned)(~r);
+                    for (int c = J; c < NB; ++c) cand[w * NB + c] = P[c];
+                    ckey[w] = ((unsigned long long)m << 32) | (unsigned)(~r);
+                }
             }
             __syncthreads();
-            unsigned long long best = ckey[0];
+            if (wlive) {
+                unsigned long long best = ckey[0];
+                for (int ww = 1; ww < nwl; ++ww) best = ckey[ww] > best ? ckey[ww] : best;
+                // a live row always wins (n - k0 - J >= 1 rows are live); the
+                // clamp only keeps the index in range whatever the keys hold
+                int p = (int)~(unsigned)best;
+                p = p < k0 ? k0 : (p >= n ? n - 1 : p);
+                const T* prow = cand + ((p - k0) >> 6) * NB;  // thread p - k0 is in wavefront (p - k0) / 64
+                const T d = prow[J];
+                if (t == 0) L.pivs[J] = p;
+                if (d == (T)0 && zero == 0) zero = k0 + J + 1;
+                if (r == p) live = false;
+                // Branch-free: a row that is not live eliminates with l = 0 and
+                // keeps its values (a divergent branch here makes the compiler
+                // copy the whole row in and out of the branch).  The pivot row's
+                // entries are finite unless A holds inf or NaN, when the factors
+                // carry NaN whichever way they are computed.
+                const T q = (d != (T)0) ? P[J] / d : P[J];
+                const T l = live ? q : (T)0;
+                P[J] = live ? q : P[J];
 #pragma unroll
-            for (int ww = 1; ww < NWAVE; ++ww) best = ckey[ww] > best ? ckey[ww] : best;
-            // a live row always wins (n - k0 - J >= 1 rows are live); the
-            // clamp only keeps the index in range whatever the keys hold
-            int p = (int)~(unsigned)best;
-            p = p < k0 ? k0 : (p >= n ? n - 1 : p);
-            const T* prow = cand + ((p - k0) >> 6) * NB;  // thread p - k0 is in wavefront (p - k0) / 64
-            const T d = prow[J];
-            if (t == 0) L.pivs[J] = p;
-            if (d == (T)0 && zero == 0) zero = k0 + J + 1;
-            if (r == p) live = false;
-            // Branch-free: a row that is not live eliminates with l = 0 and
-            // keeps its values (a divergent branch here makes the compiler
-            // copy the whole row in and out of the branch).  The pivot row's
-            // entries are finite unless A holds inf or NaN, when the factors
-            // carry NaN whichever way they are computed.
-            const T q = (d != (T)0) ? P[J] / d : P[J];
-            const T l = live ? q : (T)0;
-            P[J] = live ? q : P[J];
-#pragma unroll
-            for (int c = J + 1; c < NB; ++c) P[c] -= l * prow[c];
+                for (int c = J + 1; c < NB; ++c) P[c] -= l * prow[c];
+            }
             panel_cols<T, J + 1>(P, r, live, n, k0, nb, L, zero);
         }
     }
