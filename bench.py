@@ -194,6 +194,7 @@ def make_conditioned_batch(count: int, seed: int, chunk: int = 200):
 
     for n in MIXED_SIZES:
         idx = np.flatnonzero(ns == n)
+        log(f"  generating {idx.size} matrices of order {n}")
         for s0 in range(0, idx.size, chunk):
             sub = idx[s0:s0 + chunk]
             c = sub.size
@@ -204,6 +205,8 @@ def make_conditioned_batch(count: int, seed: int, chunk: int = 200):
             start = int(offsets[sub[0]])
             buf[start:start + c * n * n] = a.transpose(1, 2).contiguous().reshape(-1)  # column-major
             del u, v, a
+            torch.cuda.synchronize()
+            log(f"    order {n}: {s0 + c} of {idx.size}")
     boff = np.concatenate([[0], np.cumsum(ns.astype(np.int64))[:-1]])
     b = torch.randn(int(ns.sum()), dtype=torch.float64, device="cuda", generator=g)
     return buf, offsets, ns, b, boff
@@ -465,6 +468,13 @@ def gemm_leg(steps: int, warmup: int, count: int = 200_000, seed: int = 5, compl
     return res
 
 
+def other_legs(steps: int) -> dict:
+    log("other configurations")
+    return {"lu_n64": lu_leg(steps, 2), "spd_solve_adjoint_n32": adjoint_leg(steps, 2),
+            "gemm_fp64": gemm_leg(max(3, steps // 4), 1),
+            "gemm_complex128": gemm_leg(max(3, steps // 4), 1, complex_=True)}
+
+
 def cpu_baseline(buf_host, offsets, ns, ldas, xoff, b_host, max_seconds=20.0):
     import oracle
 
@@ -502,6 +512,9 @@ def main() -> int:
                     help="skip the mixed-precision LU-solve comparison (n = 128..512)")
     ap.add_argument("--no-legs", action="store_true",
                     help="skip the LU (n = 64), adjoint (n = 32) and GEMM legs")
+    ap.add_argument("--only", choices=["mixed", "legs"], default=None,
+                    help="run only the mixed-precision comparison or only the other-configuration legs and print "
+                         "their JSON (for profiling them alone); the default runs everything")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 256 MiB device copy (reference for HBM byte counters)")
     args = ap.parse_args()
@@ -513,6 +526,13 @@ def main() -> int:
     import torch
 
     from matflat_amd import hip
+
+    if args.only == "mixed":
+        print(json.dumps({"mixed_precision": mixed_leg(args.steps, args.warmup)}), flush=True)
+        return 0
+    if args.only == "legs":
+        print(json.dumps({"other_configs": other_legs(args.steps)}), flush=True)
+        return 0
 
     seed, nmin, nmax = 2, 10, 100
     log(f"generating {args.count} SPD matrices n~U{{{nmin}..{nmax}}}")
@@ -583,10 +603,7 @@ def main() -> int:
     if not args.no_legs:
         del buf, b
         torch.cuda.empty_cache()
-        log("other configurations")
-        legs = {"lu_n64": lu_leg(args.steps, 2), "spd_solve_adjoint_n32": adjoint_leg(args.steps, 2),
-                "gemm_fp64": gemm_leg(max(3, args.steps // 4), 1),
-                "gemm_complex128": gemm_leg(max(3, args.steps // 4), 1, complex_=True)}
+        legs = other_legs(args.steps)
     out = {
         "metric": METRIC,
         "value": round(value, 1),
