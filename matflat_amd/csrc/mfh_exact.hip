@@ -245,13 +245,17 @@ __global__ __launch_bounds__(T) void k_getrf_exact(MfhGetrfBatch B, MfhQueue q)
         const int lda = B.lda ? B.lda[b] : B.lda0;
         double* A = B.a ? B.a[b] : B.a0 + (size_t)b * B.stride_a;
         int* P = B.piv ? B.piv[b] : B.piv0 + (size_t)b * B.stride_piv;
-        double* L = lds;  // m x n, leading dimension m
-        int* pv = (int*)(lds + m * n);
+        // m x n in LDS with an odd leading dimension ml: the row-wise
+        // accesses (row i across columns, one thread per column) then fall
+        // in distinct banks instead of all in one
+        const int ml = m | 1;
+        double* L = lds;
+        int* pv = (int*)(lds + ml * n);
 
         for (int e = tid; e < m * n; e += T) {
             int j = e / m;
             int i = e - j * m;
-            L[e] = A[i + (size_t)j * lda];
+            L[i + j * ml] = A[i + (size_t)j * lda];
         }
         for (int i = tid; i < m; i += T) pv[i] = i;
         __syncthreads();
@@ -263,8 +267,8 @@ __global__ __launch_bounds__(T) void k_getrf_exact(MfhGetrfBatch B, MfhQueue q)
             const bool act = i < m;
             double w = 0.0;
             if (act) {
-                w = L[i + k * m] - mfh_dot_exact(k, L + i, m, L + k * m, 1);
-                L[i + k * m] = w;
+                w = L[i + k * ml] - mfh_dot_exact(k, L + i, ml, L + k * ml, 1);
+                L[i + k * ml] = w;
             }
             double key = act ? mfh_pivot_key(w) : -2.0;
             int idx = act ? i : INT_MAX;
@@ -284,9 +288,9 @@ __global__ __launch_bounds__(T) void k_getrf_exact(MfhGetrfBatch B, MfhQueue q)
             const int p = s_p;
             if (p != k) {
                 for (int j = tid; j < n; j += T) {
-                    double t = L[k + j * m];
-                    L[k + j * m] = L[p + j * m];
-                    L[p + j * m] = t;
+                    double t = L[k + j * ml];
+                    L[k + j * ml] = L[p + j * ml];
+                    L[p + j * ml] = t;
                 }
                 if (tid == 0) {
                     int t = pv[k];
@@ -296,17 +300,17 @@ __global__ __launch_bounds__(T) void k_getrf_exact(MfhGetrfBatch B, MfhQueue q)
                 sign = -sign;
                 __syncthreads();
             }
-            const double diag = L[k + k * m];
-            if (act && i > k && diag != 0.0) L[i + k * m] = L[i + k * m] / diag;
+            const double diag = L[k + k * ml];
+            if (act && i > k && diag != 0.0) L[i + k * ml] = L[i + k * ml] / diag;
             const int j = k + 1 + tid;
-            if (j < n) L[k + j * m] = L[k + j * m] - mfh_dot_exact(k, L + k, m, L + j * m, 1);
+            if (j < n) L[k + j * ml] = L[k + j * ml] - mfh_dot_exact(k, L + k, ml, L + j * ml, 1);
             __syncthreads();
         }
 
         for (int e = tid; e < m * n; e += T) {
             int j = e / m;
             int i = e - j * m;
-            A[i + (size_t)j * lda] = L[e];
+            A[i + (size_t)j * lda] = L[i + j * ml];
         }
         for (int i = tid; i < m; i += T) P[i] = pv[i];
         if (tid == 0) {
@@ -641,13 +645,14 @@ __global__ __launch_bounds__(T) void k_trsv_exact(MfhTrsvBatch B, MfhQueue q)
         const int incx = B.incx[b];
         const double* A = B.a[b];
         double* X = B.x[b];
+        const int nl = n | 1;  // odd leading dimension: row-wise reads hit distinct banks
         double* S = lds;
-        double* xs = lds + n * n;
+        double* xs = lds + nl * n;
         for (int j = 0; j < n; ++j) {
             if (upper)
-                for (int i = tid; i <= j; i += T) S[i + j * n] = A[i + (size_t)j * lda];
+                for (int i = tid; i <= j; i += T) S[i + j * nl] = A[i + (size_t)j * lda];
             else
-                for (int i = j + tid; i < n; i += T) S[i + j * n] = A[i + (size_t)j * lda];
+                for (int i = j + tid; i < n; i += T) S[i + j * nl] = A[i + (size_t)j * lda];
         }
         double xr = (tid < n) ? X[(size_t)tid * incx] : 0.0;
         __syncthreads();
@@ -664,11 +669,11 @@ __global__ __launch_bounds__(T) void k_trsv_exact(MfhTrsvBatch B, MfhQueue q)
             for (int c = 0; c < n - 1; ++c) {
                 const int i = tid;
                 if (i > c && i < n) {
-                    double aic = upper ? S[c + i * n] : S[i + c * n];
+                    double aic = upper ? S[c + i * nl] : S[i + c * nl];
                     dot_fold(i, c, aic * xs[c], s, pend);
                     if (i == c + 1) {
                         xr = xr - s;
-                        if (!unit) xr = xr / S[i + i * n];
+                        if (!unit) xr = xr / S[i + i * nl];
                         xs[i] = xr;
                     }
                 }
@@ -678,16 +683,16 @@ __global__ __launch_bounds__(T) void k_trsv_exact(MfhTrsvBatch B, MfhQueue q)
             // Axpy forms, bottom-up: Upper/NoTrans (:191-199) subtracts column
             // i of U, Lower/Trans (:224-231) row j of L (MulSub :397-421).
             if (tid == n - 1) {
-                if (!unit) xr = xr / S[(n - 1) * (n + 1)];
+                if (!unit) xr = xr / S[(n - 1) * (nl + 1)];
                 xs[n - 1] = xr;
             }
             __syncthreads();
             for (int j = n - 1; j >= 1; --j) {
                 if (tid < j) {
-                    double arj = upper ? S[tid + j * n] : S[j + tid * n];
+                    double arj = upper ? S[tid + j * nl] : S[j + tid * nl];
                     xr = xr - arj * xs[j];
                     if (tid == j - 1) {
-                        if (!unit) xr = xr / S[tid + tid * n];
+                        if (!unit) xr = xr / S[tid + tid * nl];
                         xs[tid] = xr;
                     }
                 }
@@ -737,12 +742,13 @@ __global__ __launch_bounds__(T) void k_getrs_exact(MfhGetrsBatch B, MfhQueue q)
         const int lda = B.lda[b];
         const double* LU = B.lu[b];
         const int* P = B.piv[b];
+        const int nl = n | 1;  // odd leading dimension: row-wise reads hit distinct banks
         double* S = lds;
-        double* xs = lds + n * n;
+        double* xs = lds + nl * n;
         for (int e = tid; e < n * n; e += T) {
             int j = e / n;
             int i = e - j * n;
-            S[e] = LU[i + (size_t)j * lda];
+            S[i + j * nl] = LU[i + (size_t)j * lda];
         }
         double xr = 0.0;
         if (tid < n) {
@@ -759,7 +765,7 @@ __global__ __launch_bounds__(T) void k_getrs_exact(MfhGetrsBatch B, MfhQueue q)
         for (int c = 0; c < n - 1; ++c) {
             const int i = tid;
             if (i > c && i < n) {
-                dot_fold(i, c, S[i + c * n] * xs[c], s, pend);
+                dot_fold(i, c, S[i + c * nl] * xs[c], s, pend);
                 if (i == c + 1) {
                     xr = xr - s;
                     xs[i] = xr;
@@ -768,15 +774,15 @@ __global__ __launch_bounds__(T) void k_getrs_exact(MfhGetrsBatch B, MfhQueue q)
             __syncthreads();
         }
         if (tid == n - 1) {
-            xr = xr / S[(n - 1) * (n + 1)];
+            xr = xr / S[(n - 1) * (nl + 1)];
             xs[n - 1] = xr;
         }
         __syncthreads();
         for (int j = n - 1; j >= 1; --j) {
             if (tid < j) {
-                xr = xr - S[tid + j * n] * xs[j];
+                xr = xr - S[tid + j * nl] * xs[j];
                 if (tid == j - 1) {
-                    xr = xr / S[tid + tid * n];
+                    xr = xr / S[tid + tid * nl];
                     xs[tid] = xr;
                 }
             }
@@ -788,7 +794,9 @@ __global__ __launch_bounds__(T) void k_getrs_exact(MfhGetrsBatch B, MfhQueue q)
     }
 }
 
-size_t square_lds_bytes(int nb) { return (size_t)nb * nb * 8 + (size_t)nb * 8 + (size_t)nb * 4 + 64; }
+// nb x nb doubles with one spare column (odd leading dimension), nb doubles
+// and nb ints
+size_t square_lds_bytes(int nb) { return (size_t)nb * nb * 8 + (size_t)nb * 8 + (size_t)nb * 8 + (size_t)nb * 4 + 64; }
 
 // Per-bucket shape of the packed Cholesky kernel: lanes per matrix G and
 // rows per lane S (G*S >= the bucket's largest row count).
