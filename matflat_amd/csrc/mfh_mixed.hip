@@ -112,6 +112,7 @@ struct Lds {
     int* mapv;  // ... and the row whose content each receives
     int* mapn;  // number of such rows
     int* dst;   // [ldw]: final row of panel row k0 + i
+    T* diag;    // [BS][BS + 1]: diagonal block of the triangular solves
     double* rkey;
     int* ridx;
     double* scal;
@@ -787,13 +788,28 @@ __device__ __forceinline__ void lu_solve(const T* W, int ldw, int n, const Lds<T
 {
     constexpr int BS = Prec<T>::BS;
     constexpr int TPR = BT / BS;
+    constexpr int DL = BS + 1;  // leading dimension of the staged diagonal block
     const int t = threadIdx.x, lane = t & 63;
     const int w = wave_id();
     const int nblk = (n + BS - 1) / BS;
-    // forward
+    T* D = L.diag;
+    // All threads stage the diagonal block of rows r0 .. r0+rows-1 into LDS
+    // (zero outside the triangle the sweep uses), so wavefront 0 solves it
+    // from LDS instead of holding it in registers.
+    auto stage = [&](int r0, int rows, bool lower) {
+        for (int e = t; e < BS * BS; e += BT) {
+            const int i = e / BS, j = e - i * BS;
+            const bool in = i < rows && j < rows && (lower ? j < i : j >= i);
+            const int ri = r0 + (i < rows ? i : rows - 1), cj = r0 + (j < rows ? j : rows - 1);
+            const T v = gmem(W)[(size_t)ri * ldw + cj];
+            D[i * DL + j] = in ? v : (T)0;
+        }
+    };
+    // forward: unit lower
     for (int blk = 0; blk < nblk; ++blk) {
         const int r0 = blk * BS;
         const int rows = n - r0 < BS ? n - r0 : BS;
+        stage(r0, rows, true);
         if (r0 > 0) {
             const int r = r0 + t / TPR, q = t % TPR;
             double s = 0.0;
@@ -801,32 +817,27 @@ __device__ __forceinline__ void lu_solve(const T* W, int ldw, int n, const Lds<T
 #pragma unroll
             for (int off = 1; off < TPR; off <<= 1) s += __shfl_xor(s, off, 64);
             if (q == 0 && r < r0 + rows) L.yv[r] = (T)((double)L.yv[r] - s);
-            __syncthreads();
         }
+        __syncthreads();
         if (w == 0) {
-            T row[BS];
             const int r = r0 + lane;
-            const int rr = r < n ? r : n - 1;
-#pragma unroll
-            for (int j = 0; j < BS; ++j) {
-                const T v = W[(size_t)rr * ldw + (r0 + j < n ? r0 + j : n - 1)];
-                row[j] = (lane < rows && j < lane) ? v : (T)0;
-            }
-            T y = (lane < rows) ? L.yv[rr] : (T)0;
-#pragma unroll
+            T y = (lane < rows) ? L.yv[r0 + (lane < rows ? lane : 0)] : (T)0;
+            const T* drow = D + lane * DL;
+#pragma unroll 8
             for (int j = 0; j < BS; ++j) {
                 const T yj = __shfl(y, j, 64);
-                y -= row[j] * yj;
+                y -= drow[j] * yj;
             }
             if (lane < rows) L.yv[r] = y;
         }
         __syncthreads();
     }
-    // backward
+    // backward: upper with its diagonal
     for (int blk = nblk - 1; blk >= 0; --blk) {
         const int r0 = blk * BS;
         const int rows = n - r0 < BS ? n - r0 : BS;
         const int c1 = r0 + rows;
+        stage(r0, rows, false);
         if (c1 < n) {
             const int r = r0 + t / TPR, q = t % TPR;
             double s = 0.0;
@@ -834,23 +845,18 @@ __device__ __forceinline__ void lu_solve(const T* W, int ldw, int n, const Lds<T
 #pragma unroll
             for (int off = 1; off < TPR; off <<= 1) s += __shfl_xor(s, off, 64);
             if (q == 0 && r < c1) L.yv[r] = (T)((double)L.yv[r] - s);
-            __syncthreads();
         }
+        __syncthreads();
         if (w == 0) {
-            T row[BS];
             const int r = r0 + lane;
-            const int rr = r < n ? r : n - 1;
-#pragma unroll
-            for (int j = 0; j < BS; ++j) {
-                const T v = W[(size_t)rr * ldw + (r0 + j < n ? r0 + j : n - 1)];
-                row[j] = (lane < rows && j < rows && j >= lane) ? v : (T)0;
-            }
-            T y = (lane < rows) ? L.yv[rr] : (T)0;
-#pragma unroll
+            T y = (lane < rows) ? L.yv[r0 + (lane < rows ? lane : 0)] : (T)0;
+            const T* drow = D + lane * DL;
+#pragma unroll 8
             for (int j = BS - 1; j >= 0; --j) {
-                if (lane == j) y = (row[j] != (T)0) ? y / row[j] : y;
+                const T dj = drow[j];
+                if (lane == j) y = (dj != (T)0) ? y / dj : y;
                 const T yj = __shfl(y, j, 64);
-                if (lane < j) y -= row[j] * yj;
+                if (lane < j) y -= dj * yj;
             }
             if (lane < rows) L.yv[r] = y;
         }
@@ -859,12 +865,16 @@ __device__ __forceinline__ void lu_solve(const T* W, int ldw, int n, const Lds<T
 }
 
 // rv <- b - A x (FP64, column-major A); returns ||rv||_inf (all threads).
+#ifndef MFH_RESID_U
+#define MFH_RESID_U 8
+#endif
+
 template <typename T>
 __device__ __forceinline__ double residual(const double* A, int lda, const double* b, int n, const Lds<T>& L)
 {
     // G groups of threads split the columns (G = 4 for n <= 128, 2 for
-    // n <= 256), so every thread has a row; sixteen independent partial sums
-    // keep sixteen column loads in flight per thread.  The group sums meet
+    // n <= 256), so every thread has a row; RU independent partial sums keep
+    // RU column loads in flight per thread.  The group sums meet
     // in the ut region, which is free once the factorization is done.
     const int G = n <= BT / 4 ? 4 : (n <= BT / 2 ? 2 : 1);
     const int span = BT / G;
@@ -872,27 +882,28 @@ __device__ __forceinline__ double residual(const double* A, int lda, const doubl
     const int g = t / span, r = t % span;
     const int cper = (n + G - 1) / G;
     const int c0 = g * cper, c1 = c0 + cper < n ? c0 + cper : n;
-    double ps[16];
+    constexpr int RU = MFH_RESID_U;
+    double ps[RU];
 #pragma unroll
-    for (int u = 0; u < 16; ++u) ps[u] = 0.0;
+    for (int u = 0; u < RU; ++u) ps[u] = 0.0;
     if (r < n) {
         const auto* Ar = gmem(A + r);
         int c = c0;
-        for (; c + 16 <= c1; c += 16) {
-            // all sixteen loads are issued before the first use: the
-            // scheduler would otherwise interleave them with the FMAs and
-            // wait for each load in turn
-            double av[16];
+        for (; c + RU <= c1; c += RU) {
+            // all RU loads are issued before the first use: the scheduler
+            // would otherwise interleave them with the FMAs and wait for
+            // each load in turn
+            double av[RU];
 #pragma unroll
-            for (int u = 0; u < 16; ++u) av[u] = Ar[(size_t)(c + u) * lda];
+            for (int u = 0; u < RU; ++u) av[u] = Ar[(size_t)(c + u) * lda];
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-            for (int u = 0; u < 16; ++u) ps[u] = __builtin_fma(-av[u], L.xv[c + u], ps[u]);
+            for (int u = 0; u < RU; ++u) ps[u] = __builtin_fma(-av[u], L.xv[c + u], ps[u]);
         }
         for (; c < c1; ++c) ps[0] = __builtin_fma(-Ar[(size_t)c * lda], L.xv[c], ps[0]);
     }
 #pragma unroll
-    for (int w = 8; w >= 1; w >>= 1)
+    for (int w = RU / 2; w >= 1; w >>= 1)
 #pragma unroll
         for (int u = 0; u < w; ++u) ps[u] += ps[u + w];
     double* part = (double*)L.ut;  // G * n <= BT doubles
@@ -1041,11 +1052,14 @@ __global__ __launch_bounds__(BT) void k_gesv(GesvArgs G)
 }
 
 // LDS of the solve launches: xv, rv [ldw] double, yv [ldw] T, the group
-// partial sums of the residual [BT] double and the block-reduction slots.
+// partial sums of the residual [BT] double, the block-reduction slots and
+// the staged diagonal block of the triangular solves.
 template <typename T>
 __host__ __device__ size_t step_lds_bytes(int ldw)
 {
-    return (size_t)ldw * (2 * sizeof(double) + sizeof(T)) + BT * sizeof(double) + 2 * NWAVE * sizeof(double) + 64;
+    constexpr int BS = Prec<T>::BS;
+    return (size_t)ldw * (2 * sizeof(double) + sizeof(T)) + BT * sizeof(double) + 2 * NWAVE * sizeof(double) + 64 +
+           (size_t)BS * (BS + 1) * sizeof(T) + 16;
 }
 
 template <typename T>
@@ -1062,15 +1076,22 @@ __device__ __forceinline__ Lds<T> carve_step(char* base, int ldw)
     L.rkey = (double*)(base + s);
     s += 2 * NWAVE * sizeof(double);
     L.yv = (T*)(base + s);
+    s += (size_t)ldw * sizeof(T);
+    s = (s + 15) & ~(size_t)15;
+    L.diag = (T*)(base + s);
     return L;
 }
 
 // Solve step `it` for every matrix still active; workgroups walk the batch
 // with a fixed stride.  The solve and the residual are bound by memory
-// latency: this kernel keeps few registers so that several workgroups share
-// a CU and their loads overlap.
+// latency: this kernel is held to MFH_STEP_WAVES wavefronts per SIMD (80
+// VGPRs at 6, no spills), so that three workgroups share a CU and their
+// loads overlap.
+#ifndef MFH_STEP_WAVES
+#define MFH_STEP_WAVES 6
+#endif
 template <typename T, bool MIXED>
-__global__ __launch_bounds__(BT) void k_gesv_step(GesvArgs G, int it)
+__global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(MFH_STEP_WAVES))) void k_gesv_step(GesvArgs G, int it)
 {
     extern __shared__ __attribute__((aligned(16))) char lds_raw[];
     const Lds<T> L = carve_step<T>(lds_raw, G.ldw);
