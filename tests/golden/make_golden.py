@@ -1,0 +1,167 @@
+"""Writes the golden fixtures tests/golden/*.npz from the oracle (the C
+restatement of MatFlat in oracle/matflat_oracle.c).
+
+MatFlat's own tests hold no golden vectors for this path (they compare with
+OpenBLAS at 1e-12), so these fixtures are the build's own: seeded inputs and
+the oracle's bit-exact outputs for the shapes the reference's tests use
+(LuTests.cs:103-122, CholeskyTests.cs, BlasTests.SolveTriangular*.cs,
+BlasTests.MulMatMat*.cs), stored with allow_pickle=False arrays only.  The
+GPU tests compare the HIP path against them without running the oracle.
+
+Run from the repo root:  python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+import oracle  # noqa: E402
+
+LU_SHAPES = [(1, 1, 1), (1, 1, 3), (2, 2, 2), (2, 2, 3), (3, 3, 3), (3, 3, 5), (3, 1, 3), (3, 1, 5), (1, 3, 1),
+             (1, 3, 3), (3, 4, 3), (3, 4, 5), (4, 3, 4), (4, 3, 5), (11, 23, 11), (11, 23, 17), (16, 8, 16),
+             (16, 8, 32), (23, 11, 23), (23, 11, 31), (64, 64, 64)]  # (m, n, lda): LuTests.cs:103-122, plus n = 64
+CHOL_N = [1, 3, 11, 16, 23, 33, 100]
+TRSV_N = [1, 3, 11, 16, 33]
+GEMM_SHAPES = [(1, 1, 1), (3, 4, 5), (8, 16, 4), (17, 9, 13), (19, 21, 24)]
+
+
+def pack(bufs):
+    off = np.concatenate([[0], np.cumsum([b.size for b in bufs])[:-1]]).astype(np.int64)
+    return np.concatenate(bufs) if bufs else np.zeros(0), off
+
+
+def lu_fixture(rng):
+    meta, ins, outs, pivs = [], [], [], []
+    for m, n, lda in LU_SHAPES:
+        a = np.full(lda * n, np.nan)
+        for j in range(n):
+            a[j * lda:j * lda + m] = rng.standard_normal(m)
+        lu = a.copy()
+        piv, sign = oracle.dgetrf(m, n, lu, lda)
+        meta.append((m, n, lda, sign))
+        ins.append(a)
+        outs.append(lu)
+        pivs.append(piv.astype(np.int32))
+    a_in, off = pack(ins)
+    a_out, _ = pack(outs)
+    piv, poff = pack(pivs)
+    return {"lu_meta": np.array(meta, dtype=np.int64), "lu_off": off, "lu_in": a_in, "lu_out": a_out,
+            "lu_piv": piv.astype(np.int32), "lu_piv_off": poff}
+
+
+def chol_fixture(rng):
+    meta, ins, outs, bs, xs = [], [], [], [], []
+    for n in CHOL_N:
+        lda = (n + 7) // 8 * 8
+        bm = rng.standard_normal((n, n))
+        spd = bm @ bm.T + n * np.eye(n)
+        a = np.full(lda * n, np.nan)  # strict lower triangle NaN: never read (CholeskyTests.cs:333)
+        for j in range(n):
+            a[j * lda:j * lda + j + 1] = spd[:j + 1, j]
+        l = a.copy()
+        info = oracle.dpotrf(n, l, lda)
+        b = rng.standard_normal(n)
+        x = b.copy()
+        lx = a.copy()
+        assert oracle.dposv(n, lx, lda, x) == 0
+        meta.append((n, lda, info))
+        ins.append(a)
+        outs.append(l)
+        bs.append(b)
+        xs.append(x)
+    a_in, off = pack(ins)
+    a_out, _ = pack(outs)
+    b, boff = pack(bs)
+    x, _ = pack(xs)
+    return {"chol_meta": np.array(meta, dtype=np.int64), "chol_off": off, "chol_in": a_in, "chol_out": a_out,
+            "posv_b": b, "posv_x": x, "posv_off": boff}
+
+
+def trsv_fixture(rng):
+    meta, ins, xin, xout = [], [], [], []
+    for uplo in (121, 122):
+        for trans in (111, 112):
+            for diag in (131, 132):
+                for n in TRSV_N:
+                    lda, inc = n + 1, 2
+                    a = rng.uniform(-1, 1, lda * n)  # non-diagonal triangles
+                    for j in range(n):
+                        a[j * lda + j] = 2.0 + abs(a[j * lda + j]) * np.sqrt(n)
+                    x = rng.uniform(-1, 1, n * inc)
+                    want = x.copy()
+                    if diag == 131:
+                        oracle.dtrsv(uplo, trans, n, a, lda, want, inc)
+                    else:
+                        au = a.copy()
+                        for j in range(n):
+                            au[j * lda + j] = 1.0
+                        oracle.dtrsv(uplo, trans, n, au, lda, want, inc)
+                    meta.append((uplo, trans, diag, n, lda, inc))
+                    ins.append(a)
+                    xin.append(x)
+                    xout.append(want)
+    a, off = pack(ins)
+    x0, xoff = pack(xin)
+    x1, _ = pack(xout)
+    return {"trsv_meta": np.array(meta, dtype=np.int64), "trsv_off": off, "trsv_a": a, "trsv_x_off": xoff,
+            "trsv_x_in": x0, "trsv_x_out": x1}
+
+
+def gemm_fixture(rng, cplx):
+    ops = (111, 112, 113, 114) if cplx else (111, 112)
+    meta, As, Bs, Cs, Co, ab = [], [], [], [], [], []
+    for m, n, k in GEMM_SHAPES:
+        for ta in ops:
+            for tb in ops:
+                ra, ca = (m, k) if ta in (111, 114) else (k, m)
+                rb, cb = (k, n) if tb in (111, 114) else (n, k)
+                lda, ldb, ldc = ra + 1, rb + 2, m + 3
+
+                def gen(size):
+                    v = rng.standard_normal(size)
+                    return v + 1j * rng.standard_normal(size) if cplx else v
+
+                a, b, c = gen(lda * ca), gen(ldb * cb), gen(ldc * n)
+                if cplx:
+                    alpha, beta = complex(*rng.uniform(-1, 1, 2)), complex(*rng.uniform(-1, 1, 2))
+                else:
+                    alpha, beta = float(rng.uniform(-1, 1)), float(rng.uniform(-1, 1))
+                if len(meta) % 4 == 0:
+                    alpha, beta = 1.0, 0.0  # MatFlat's own C = op(A) op(B)
+                out = c.copy()
+                (oracle.zgemm if cplx else oracle.dgemm)(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, out, ldc)
+                meta.append((ta, tb, m, n, k, lda, ldb, ldc))
+                As.append(a)
+                Bs.append(b)
+                Cs.append(c)
+                Co.append(out)
+                ab.append((complex(alpha).real, complex(alpha).imag, complex(beta).real, complex(beta).imag))
+    p = "zgemm" if cplx else "dgemm"
+    a, aoff = pack(As)
+    b, boff = pack(Bs)
+    c, coff = pack(Cs)
+    co, _ = pack(Co)
+    return {f"{p}_meta": np.array(meta, dtype=np.int64), f"{p}_a": a, f"{p}_a_off": aoff, f"{p}_b": b,
+            f"{p}_b_off": boff, f"{p}_c_in": c, f"{p}_c_off": coff, f"{p}_c_out": co,
+            f"{p}_alpha_beta": np.array(ab, dtype=np.float64)}
+
+
+def main():
+    oracle.build()
+    rng = np.random.default_rng(20240610)
+    np.savez_compressed(os.path.join(HERE, "lu.npz"), **lu_fixture(rng))
+    np.savez_compressed(os.path.join(HERE, "cholesky_posv.npz"), **chol_fixture(rng))
+    np.savez_compressed(os.path.join(HERE, "trsv.npz"), **trsv_fixture(rng))
+    np.savez_compressed(os.path.join(HERE, "gemm.npz"), **gemm_fixture(rng, False), **gemm_fixture(rng, True))
+    for f in sorted(os.listdir(HERE)):
+        if f.endswith(".npz"):
+            print(f, os.path.getsize(os.path.join(HERE, f)), "bytes")
+
+
+if __name__ == "__main__":
+    main()

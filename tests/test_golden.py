@@ -1,0 +1,69 @@
+"""The oracle against the committed golden fixtures (tests/golden/*.npz,
+written by tests/golden/make_golden.py): a change to the C restatement of
+MatFlat that alters any output bit fails here.  The fixtures' shapes follow
+the reference's tests (LuTests.cs:103-122, CholeskyTests.cs,
+BlasTests.SolveTriangular*.cs, BlasTests.MulMatMat*.cs)."""
+import os
+
+import numpy as np
+
+import oracle
+from gpu_util import assert_bits_equal
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def load(name):
+    return np.load(os.path.join(GOLDEN, name), allow_pickle=False)
+
+
+def test_lu_fixtures():
+    g = load("lu.npz")
+    for i, (m, n, lda, sign) in enumerate(g["lu_meta"]):
+        off, poff = g["lu_off"][i], g["lu_piv_off"][i]
+        a = g["lu_in"][off:off + lda * n].copy()
+        piv, s = oracle.dgetrf(int(m), int(n), a, int(lda))
+        assert_bits_equal(a, g["lu_out"][off:off + lda * n], f"lu {i}")
+        assert np.array_equal(piv, g["lu_piv"][poff:poff + m]) and s == sign
+
+
+def test_cholesky_and_posv_fixtures():
+    g = load("cholesky_posv.npz")
+    for i, (n, lda, info) in enumerate(g["chol_meta"]):
+        off, boff = g["chol_off"][i], g["posv_off"][i]
+        a = g["chol_in"][off:off + lda * n].copy()
+        assert oracle.dpotrf(int(n), a, int(lda)) == info
+        assert_bits_equal(a, g["chol_out"][off:off + lda * n], f"chol {i}")
+        a = g["chol_in"][off:off + lda * n].copy()
+        x = g["posv_b"][boff:boff + n].copy()
+        assert oracle.dposv(int(n), a, int(lda), x) == 0
+        assert_bits_equal(x, g["posv_x"][boff:boff + n], f"posv {i}")
+
+
+def test_trsv_fixtures():
+    g = load("trsv.npz")
+    for i, (uplo, trans, diag, n, lda, inc) in enumerate(g["trsv_meta"]):
+        off, xoff = g["trsv_off"][i], g["trsv_x_off"][i]
+        a = g["trsv_a"][off:off + lda * n].copy()
+        if diag == 132:
+            for j in range(n):
+                a[j * lda + j] = 1.0
+        x = g["trsv_x_in"][xoff:xoff + n * inc].copy()
+        oracle.dtrsv(int(uplo), int(trans), int(n), a, int(lda), x, int(inc))
+        assert_bits_equal(x, g["trsv_x_out"][xoff:xoff + n * inc], f"trsv {i}")
+
+
+def test_gemm_fixtures():
+    g = load("gemm.npz")
+    for p, fn in (("dgemm", oracle.dgemm), ("zgemm", oracle.zgemm)):
+        for i, (ta, tb, m, n, k, lda, ldb, ldc) in enumerate(g[f"{p}_meta"]):
+            ar, ai, br, bi = g[f"{p}_alpha_beta"][i]
+            alpha, beta = (complex(ar, ai), complex(br, bi)) if p == "zgemm" else (ar, br)
+            ca = k if ta in (111, 114) else m
+            cb = n if tb in (111, 114) else k
+            a0, b0, c0 = g[f"{p}_a_off"][i], g[f"{p}_b_off"][i], g[f"{p}_c_off"][i]
+            a = g[f"{p}_a"][a0:a0 + lda * ca]
+            b = g[f"{p}_b"][b0:b0 + ldb * cb]
+            c = g[f"{p}_c_in"][c0:c0 + ldc * n].copy()
+            fn(int(ta), int(tb), int(m), int(n), int(k), alpha, a, int(lda), b, int(ldb), beta, c, int(ldc))
+            assert_bits_equal(c, g[f"{p}_c_out"][c0:c0 + ldc * n], f"{p} {i}")
