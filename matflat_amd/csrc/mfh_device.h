@@ -25,6 +25,18 @@ __device__ __forceinline__ double mfh_dot_exact(int len, const double* x, int in
         s = 0.0;
         c = 0;
     }
+    // Four pairs per step: the eight loads and products do not depend on s,
+    // so they are issued together and only the four additions into s stay
+    // sequential, in MatFlat's order.
+    for (; c + 8 <= len; c += 8) {
+        double p[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) p[u] = x[(c + u) * incx] * y[(c + u) * incy];
+        s = s + (p[0] + p[1]);
+        s = s + (p[2] + p[3]);
+        s = s + (p[4] + p[5]);
+        s = s + (p[6] + p[7]);
+    }
     for (; c < len; c += 2) {
         double p0 = x[c * incx] * y[c * incy];
         double p1 = x[(c + 1) * incx] * y[(c + 1) * incy];
