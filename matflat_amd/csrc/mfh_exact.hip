@@ -100,28 +100,37 @@ __global__ __launch_bounds__(256) void k_sort_hist(Op op, int count, SortWs ws)
         if (h[t]) atomicAdd(&ws.hist[t], h[t]);
 }
 
-// Descending-key exclusive offsets, per-key cursors and bucket ranges.
-__global__ void k_sort_offsets(SortWs ws)
+// Descending-key exclusive offsets, per-key cursors and bucket ranges.  The
+// histogram is staged in LDS first: one thread walking it in global memory
+// waits for every load in turn.
+__global__ __launch_bounds__(64) void k_sort_offsets(SortWs ws)
 {
-    if (threadIdx.x != 0) return;
-    int run = 0;
-    for (int k = kKeys - 1; k >= 0; --k) {
-        ws.cursor[k] = run;
-        run += ws.hist[k];
+    __shared__ int h[kKeys];
+    __shared__ int cur[kKeys];
+    for (int k = threadIdx.x; k < kKeys; k += blockDim.x) h[k] = ws.hist[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int run = 0;
+        for (int k = kKeys - 1; k >= 0; --k) {
+            cur[k] = run;
+            run += h[k];
+        }
     }
-    for (int b = 0; b < MFH_NBUCKETS; ++b) {
+    __syncthreads();
+    for (int k = threadIdx.x; k < kKeys; k += blockDim.x) ws.cursor[k] = cur[k];
+    for (int b = threadIdx.x; b < MFH_NBUCKETS; b += blockDim.x) {
         const int hi = MFH_BUCKET_STEP * (b + 1), lo = MFH_BUCKET_STEP * b;
         int c = 0;
-        for (int k = lo + 1; k <= hi; ++k) c += ws.hist[k];
-        ws.bstart[b] = ws.cursor[hi];
+        for (int k = lo + 1; k <= hi; ++k) c += h[k];
+        ws.bstart[b] = cur[hi];
         ws.bcount[b] = c;
     }
-    for (int f = 0; f < kFine; ++f) {
+    for (int f = threadIdx.x; f < kFine; f += blockDim.x) {
         const int lo = MFH_FINE_STEP * f;
         const int hi = lo + MFH_FINE_STEP < kKeys - 1 ? lo + MFH_FINE_STEP : kKeys - 1;
         int c = 0;
-        for (int k = lo + 1; k <= hi; ++k) c += ws.hist[k];
-        ws.fstart[f] = ws.cursor[hi];
+        for (int k = lo + 1; k <= hi; ++k) c += h[k];
+        ws.fstart[f] = cur[hi];
         ws.fcount[f] = c;
     }
 }
