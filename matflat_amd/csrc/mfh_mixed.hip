@@ -493,12 +493,29 @@ __device__ __forceinline__ void swap_rows(T* W, int ldw, int n, int k0, int nb, 
     const int c = t;  // n <= BT: one column per thread
     if (c < n && (c < k0 || c >= k0 + nb)) {
         T v[64];
+        if constexpr (sizeof(T) == 4) {
+            // Branch-free: entries past cnt repeat the last one (the same
+            // value loaded and stored again), so the 64 loads go out
+            // together.  (For FP64 the 64 values would not fit in
+            // registers beside the rest of the kernel.)
 #pragma unroll
-        for (int i = 0; i < 64; ++i)
-            if (i < cnt) v[i] = W[(size_t)L.mapv[i] * ldw + c];
+            for (int i = 0; i < 64; ++i) {
+                const int e = i < cnt ? i : cnt - 1;
+                v[i] = gmem(W)[(size_t)L.mapv[e] * ldw + c];
+            }
 #pragma unroll
-        for (int i = 0; i < 64; ++i)
-            if (i < cnt && L.mapk[i] != L.mapv[i]) W[(size_t)L.mapk[i] * ldw + c] = v[i];
+            for (int i = 0; i < 64; ++i) {
+                const int e = i < cnt ? i : cnt - 1;
+                gmem_w(W)[(size_t)L.mapk[e] * ldw + c] = v[i];
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 64; ++i)
+                if (i < cnt) v[i] = W[(size_t)L.mapv[i] * ldw + c];
+#pragma unroll
+            for (int i = 0; i < 64; ++i)
+                if (i < cnt && L.mapk[i] != L.mapv[i]) W[(size_t)L.mapk[i] * ldw + c] = v[i];
+        }
     }
     __syncthreads();
 }
