@@ -297,14 +297,32 @@ __device__ __forceinline__ double copy_in(const double* A, int lda, T* W, int ld
 // pivoting's (largest |v| in single precision, lowest row on ties, NaN below
 // every number, non-candidates below NaN), packed so that one unsigned 64-bit
 // maximum finds it.
-// Pivot key of a candidate row: |v| rounded to FP32 as ordered bits + 2
-// (NaN -> 1, so it is chosen only when nothing else is left; not a
-// candidate -> 0).  Ties at FP32 resolution go to the lowest row.
+// Pivot key of a candidate row in wavefront w: |v| rounded to FP32, as
+// ordered bits + 16, with the low three bits replaced by 7 - w (NaN -> 8, so
+// it is chosen only when nothing else is left; not a candidate -> 0).  One
+// 32-bit maximum over the keys of all wavefronts then picks the largest
+// |v| to 2^-20 relative, the lowest wavefront on ties, and names it.
 template <typename T>
-__device__ __forceinline__ unsigned pivot_key(bool cand, T v)
+__device__ __forceinline__ unsigned pivot_key(bool cand, T v, int w)
 {
     const float a = __builtin_fabsf((float)v);
-    return cand ? ((a != a) ? 1u : (__float_as_uint(a) + 2u)) : 0u;
+    const unsigned m = (a != a) ? 8u : (__float_as_uint(a) + 16u);
+    return cand ? ((m & ~7u) | (unsigned)(7 - w)) : 0u;
+}
+
+// 1/d to working precision from the hardware reciprocal and Newton steps
+// (one for FP32, two for FP64): a few dependent instructions on the panel's
+// critical path instead of an IEEE division.
+__device__ __forceinline__ float recip(float d)
+{
+    float r = __builtin_amdgcn_rcpf(d);
+    return __builtin_fmaf(__builtin_fmaf(-d, r, 1.0f), r, r);
+}
+__device__ __forceinline__ double recip(double d)
+{
+    double r = __builtin_amdgcn_rcp(d);
+    r = __builtin_fma(__builtin_fma(-d, r, 1.0), r, r);
+    return __builtin_fma(__builtin_fma(-d, r, 1.0), r, r);
 }
 
 #define MFH_DPP_MAX_U32(v, CTRL, RMASK)                                                   \
@@ -343,7 +361,8 @@ __device__ __forceinline__ void panel_cols(T (&P)[Prec<T>::NB], int r, bool& liv
         if (J < nb) {
             const int t = threadIdx.x, w = t >> 6;
             T* cand = L.cand + (J & 1) * NWAVE * NB;
-            unsigned long long* ckey = (unsigned long long*)(L.ckey + (J & 1) * NWAVE);
+            unsigned* ckey = (unsigned*)(L.ckey + (J & 1) * NWAVE);
+            int* cidx = L.cidx + (J & 1) * NWAVE;
             // wavefronts with no row of the panel only keep the barrier
             // company; wavefront 0 always has rows (k0 < n)
             const int nwl = (n - k0 + 63) >> 6;  // wavefronts holding rows k0 .. n-1
@@ -352,24 +371,26 @@ __device__ __forceinline__ void panel_cols(T (&P)[Prec<T>::NB], int r, bool& liv
                 // wave argmax: the largest 32-bit key, then the lowest lane
                 // that holds it (rows are in lane order, so the lowest row
                 // wins ties)
-                const unsigned k32 = pivot_key(live, P[J]);
+                const unsigned k32 = pivot_key(live, P[J], w);
                 const unsigned m = wave_max_u32(k32);
                 const int win = __builtin_ctzll(__ballot(k32 == m));
                 if ((t & 63) == win) {
 #pragma unroll
                     for (int c = J; c < NB; ++c) cand[w * NB + c] = P[c];
-                    ckey[w] = ((unsigned long long)m << 32) | (unsigned)(~r);
+                    ckey[w] = m;
+                    cidx[w] = r;
                 }
             }
             __syncthreads();
             if (wlive) {
-                unsigned long long best = ckey[0];
+                unsigned best = ckey[0];
                 for (int ww = 1; ww < nwl; ++ww) best = ckey[ww] > best ? ckey[ww] : best;
+                const int wb = 7 - (int)(best & 7u);  // the winning wavefront
+                const T* prow = cand + wb * NB;
                 // a live row always wins (n - k0 - J >= 1 rows are live); the
                 // clamp only keeps the index in range whatever the keys hold
-                int p = (int)~(unsigned)best;
+                int p = cidx[wb];
                 p = p < k0 ? k0 : (p >= n ? n - 1 : p);
-                const T* prow = cand + ((p - k0) >> 6) * NB;  // thread p - k0 is in wavefront (p - k0) / 64
                 const T d = prow[J];
                 if (t == 0) L.pivs[J] = p;
                 if (d == (T)0 && zero == 0) zero = k0 + J + 1;
@@ -379,7 +400,7 @@ __device__ __forceinline__ void panel_cols(T (&P)[Prec<T>::NB], int r, bool& liv
                 // copy the whole row in and out of the branch).  The pivot row's
                 // entries are finite unless A holds inf or NaN, when the factors
                 // carry NaN whichever way they are computed.
-                const T q = (d != (T)0) ? P[J] / d : P[J];
+                const T q = (d != (T)0) ? P[J] * recip(d) : P[J];
                 const T l = live ? q : (T)0;
                 P[J] = live ? q : P[J];
 #pragma unroll
