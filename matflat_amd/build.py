@@ -13,6 +13,7 @@ Flags:
 """
 from __future__ import annotations
 
+import concurrent.futures
 import os
 import subprocess
 import sys
@@ -20,7 +21,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmatflat_hip.so")
-SOURCES = ["mfh_api.cpp", "mfh_exact.hip", "mfh_fast.hip", "mfh_gemm.hip", "mfh_adjoint.hip", "mfh_mixed.hip"]
+OBJDIR = os.path.join(HERE, "_build")
+SOURCES = ["mfh_api.cpp", "mfh_exact.hip", "mfh_fast.hip", "mfh_fast_posv.hip", "mfh_gemm.hip", "mfh_adjoint.hip", "mfh_mixed.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [
@@ -28,7 +30,6 @@ FLAGS = [
     "-O3",
     "-std=c++17",
     "-fPIC",
-    "-shared",
     "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall",
@@ -37,10 +38,26 @@ FLAGS = [
 ]
 
 
+def _compile(src: str, obj: str, extra: list[str], verbose: bool) -> None:
+    cmd = [HIPCC, *FLAGS, "-c", *extra, src, "-o", obj]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
 def build(verbose: bool = False, extra: list[str] | None = None) -> str:
+    """Compiles every source to an object in parallel (the FAST and blocked
+    kernels are template-heavy and take minutes each), then links."""
+    extra = list(extra or [])
+    os.makedirs(OBJDIR, exist_ok=True)
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
+    objs = [os.path.join(OBJDIR, os.path.splitext(s)[0] + ".o") for s in SOURCES]
+    jobs = max(1, min(len(srcs), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    with concurrent.futures.ThreadPoolExecutor(jobs) as pool:
+        for f in [pool.submit(_compile, s, o, extra, verbose) for s, o in zip(srcs, objs)]:
+            f.result()
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, *FLAGS, *(extra or []), *srcs, "-o", tmp]
+    cmd = [HIPCC, "--offload-arch=gfx950", "-fPIC", "-shared", *objs, "-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

@@ -145,6 +145,8 @@ int mfh_fast_default_waves(int rb);
 // FAST-mode Cholesky / SPD solve for one fine size bucket (mfh_fast.hip):
 // every matrix of the queue has at most 8 * rb rows (n, or n + 1 for posv).
 int mfh_launch_chol_fast_bucket(mfh_handle* h, const MfhCholBatch& B, MfhQueue q, int rb, long count, bool posv);
+// The SPD-solve half of it (mfh_fast_posv.hip).
+int mfh_launch_chol_fast_posv(mfh_handle* h, const MfhCholBatch& B, MfhQueue q, int rb, long count);
 
 // Workspace bytes of the blocked LU solvers (mfh_mixed.hip).
 size_t mfh_gesv_ws_bytes(int count, int max_n, bool mixed);
