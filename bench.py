@@ -468,11 +468,62 @@ def gemm_leg(steps: int, warmup: int, count: int = 200_000, seed: int = 5, compl
     return res
 
 
+def gemv_leg(steps: int, warmup: int, count: int = 200_000, seed: int = 6):
+    """y = op(A) x (Blas.MulMatVec) over `count` products with m, n iid
+    U{8..128}, op uniform over {N, T}, lda padded to a multiple of 4, unit
+    strides, MatFlat's summation order (bit-identical).  HBM bound: the
+    algorithmic bytes are 8 (m n + len(x) + len(y)) per product."""
+    import torch
+
+    from matflat_amd import hip
+
+    rng = np.random.default_rng(seed)
+    m, n = (rng.integers(8, 129, size=count).astype(np.int64) for _ in range(2))
+    tr = rng.choice(np.array([111, 112]), size=count)
+    lda = (m + 3) // 4 * 4
+    xl, yl = np.where(tr == 112, m, n), np.where(tr == 112, n, m)
+    dev = torch.device("cuda")
+
+    def slab(sizes):
+        off = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+        g = torch.Generator(device=dev)
+        g.manual_seed(int(rng.integers(1 << 30)))
+        t = torch.randn(int(sizes.sum()), dtype=torch.float64, device=dev, generator=g)
+        return t, (t.data_ptr() + 8 * torch.from_numpy(off).to(dev)).contiguous()
+
+    A, pa = slab(lda * n)
+    X, px = slab(xl)
+    Y, py = slab(yl)
+
+    def i32(v):
+        return torch.from_numpy(np.ascontiguousarray(v, dtype=np.int32)).to(dev)
+
+    ones = i32(np.ones(count))
+    args = [i32(tr), i32(m), i32(n), pa, i32(lda), px, ones, py, ones]
+    info = torch.zeros(count, dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream()
+    handle = hip.Handle()
+    handle.bind_stream(stream)
+    alg = float(np.sum(8.0 * (m * n + xl + yl)))
+    ms = timed_calls(lambda: hip.dgemv_vbatched(*args, info, 128, 128, handle=handle), None, steps, warmup, stream)
+    ok = bool(torch.all(info == 0).item())
+    res = {"workload": f"{count} fp64 products y = op(A) x, m, n ~ U{{8..128}}, op in {{N, T}}, lda padded to 4, "
+                       "unit strides, exact mode (MatFlat's summation order, bit-identical)",
+           "ms_per_call": round(ms, 5), "products_per_s": round(count / (ms / 1e3), 1), "all_info_zero": ok,
+           "roofline": hbm_entry(alg, ms)}
+    log(f"  gemv fp64 x{count}: {ms:.4f} ms/call, {res['roofline']['achieved']} GB/s")
+    handle.close()
+    del A, X, Y
+    torch.cuda.empty_cache()
+    return res
+
+
 def other_legs(steps: int) -> dict:
     log("other configurations")
     return {"lu_n64": lu_leg(steps, 2), "spd_solve_adjoint_n32": adjoint_leg(steps, 2),
             "gemm_fp64": gemm_leg(max(3, steps // 4), 1),
-            "gemm_complex128": gemm_leg(max(3, steps // 4), 1, complex_=True)}
+            "gemm_complex128": gemm_leg(max(3, steps // 4), 1, complex_=True),
+            "gemv_fp64": gemv_leg(steps, 2)}
 
 
 def cpu_baseline(buf_host, offsets, ns, ldas, xoff, b_host, max_seconds=20.0):

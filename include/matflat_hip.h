@@ -109,6 +109,7 @@ int mfh_get_fast_waves(mfh_handle* handle, int row_blocks, int* waves);
 #define MFH_KERNEL_GESV_MIXED 7
 #define MFH_KERNEL_GESV_F64 8
 #define MFH_KERNEL_GESV_STEP 9
+#define MFH_KERNEL_GEMV_EXACT 10
 int mfh_set_timing(mfh_handle* handle, int enable);
 int mfh_get_timing(mfh_handle* handle, int capacity, int* n_records, int* kind, int* rows,
                    float* ms);
@@ -179,6 +180,20 @@ int mfh_dgemm_vbatched(mfh_handle* h, int count, const int* transa, const int* t
                        const double* const* a, const int* lda, const double* const* b,
                        const int* ldb, const double* beta, double* const* c, const int* ldc,
                        int* info, int max_m, int max_n, int max_k);
+/* Blas.MulMatVec(Transpose trans, int m, int n, double* a, int lda, double* x,
+ * int incx, double* y, int incy) (Blas.MulMatVec.cs:138-206) for a batch
+ * with per-product transposes: y = A x (NoTrans, m outputs) or y = A^T x
+ * (Trans, n outputs), every output one Internals.Dot in MatFlat's order, so
+ * y is bit-identical to the reference.  Per-product checks follow the
+ * reference's order and set info = -(position in this argument list); a
+ * product whose y pointer equals its a or x pointer gets info = -1
+ * ("In-place operation is not supported"); the transpose is checked last,
+ * as in the reference.  lda * n, incx * len(x) and incy * len(y) must stay
+ * below 2^28 elements; m <= max_m, n <= max_n. */
+int mfh_dgemv_vbatched(mfh_handle* h, int count, const int* trans, const int* m, const int* n,
+                       const double* const* a, const int* lda, const double* const* x,
+                       const int* incx, double* const* y, const int* incy, int* info,
+                       int max_m, int max_n);
 /* Complex version: a, b and c hold interleaved (re, im) doubles (the layout
  * of System.Numerics.Complex and hipDoubleComplex); alpha and beta are device
  * arrays of count (re, im) pairs, one per product; transa/transb may also be

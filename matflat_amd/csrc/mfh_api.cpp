@@ -365,6 +365,29 @@ int mfh_zgemm_vbatched(mfh_handle* h, int count, const int* transa, const int* t
                        max_k, true);
 }
 
+int mfh_dgemv_vbatched(mfh_handle* h, int count, const int* trans, const int* m, const int* n,
+                       const double* const* a, const int* lda, const double* const* x, const int* incx,
+                       double* const* y, const int* incy, int* info, int max_m, int max_n)
+{
+    MFH_REQUIRE(h != nullptr, 1, "handle must not be null");
+    MFH_REQUIRE(count >= 0, 2, "count must be >= 0");
+    if (count == 0) return MFH_OK;
+    MFH_REQUIRE(trans != nullptr, 3, "trans must not be null");
+    MFH_REQUIRE(m != nullptr, 4, "m must not be null");
+    MFH_REQUIRE(n != nullptr, 5, "n must not be null");
+    MFH_REQUIRE(a != nullptr, 6, "a must not be null");
+    MFH_REQUIRE(lda != nullptr, 7, "lda must not be null");
+    MFH_REQUIRE(x != nullptr, 8, "x must not be null");
+    MFH_REQUIRE(incx != nullptr, 9, "incx must not be null");
+    MFH_REQUIRE(y != nullptr, 10, "y must not be null");
+    MFH_REQUIRE(incy != nullptr, 11, "incy must not be null");
+    MFH_REQUIRE(info != nullptr, 12, "info must not be null");
+    MFH_REQUIRE(max_m >= 1, 13, "max_m must be >= 1");
+    MFH_REQUIRE(max_n >= 1, 14, "max_n must be >= 1");
+    MfhGemvBatch B{trans, m, n, a, lda, x, incx, y, incy, info};
+    return mfh_launch_gemv(h, count, B, max_m, max_n);
+}
+
 int mfh_dgesv_mixed_vbatched(mfh_handle* h, int count, const int* n, const double* const* a, const int* lda,
                              const double* const* b, double* const* x, int max_iter, double tol_per_n, int* iters,
                              double* berr, int* info, int max_n)

@@ -123,6 +123,19 @@ struct MfhGemmBatch {
     int* info;
 };
 
+struct MfhGemvBatch {
+    const int* trans;
+    const int* m;
+    const int* n;
+    const double* const* a;
+    const int* lda;
+    const double* const* x;
+    const int* incx;
+    double* const* y;
+    const int* incy;
+    int* info;
+};
+
 struct MfhAdjointBatch {
     const int* n;
     const double* const* l;
@@ -158,6 +171,7 @@ int mfh_launch_trsv(mfh_handle* h, int count, const MfhTrsvBatch& B, int max_n);
 int mfh_launch_getrs(mfh_handle* h, int count, const MfhGetrsBatch& B, int max_n);
 int mfh_launch_gemm(mfh_handle* h, int count, const MfhGemmBatch& B, int max_m, int max_n,
                     int max_k, bool complex);
+int mfh_launch_gemv(mfh_handle* h, int count, const MfhGemvBatch& B, int max_m, int max_n);
 int mfh_launch_posv_backward(mfh_handle* h, int count, const MfhAdjointBatch& B, int max_n);
 int mfh_launch_gesv_mixed(mfh_handle* h, int count, const int* n, const double* const* a,
                           const int* lda, const double* const* b, double* const* x, int max_iter,

@@ -200,6 +200,13 @@ def gemm_vbatched(transa, transb, m, n, k, alpha, a_ptrs, lda, b_ptrs, ldb, beta
               _p(b_ptrs), _p(ldb), _p(beta), _p(c_ptrs), _p(ldc), _p(info), int(max_m), int(max_n), int(max_k)))
 
 
+def dgemv_vbatched(trans, m, n, a_ptrs, lda, x_ptrs, incx, y_ptrs, incy, info, max_m, max_n,
+                   handle: Handle | None = None):
+    h = handle or default_handle()
+    _check(h.lib.mfh_dgemv_vbatched(h.h, int(m.numel()), _p(trans), _p(m), _p(n), _p(a_ptrs), _p(lda), _p(x_ptrs),
+                                    _p(incx), _p(y_ptrs), _p(incy), _p(info), int(max_m), int(max_n)))
+
+
 def dgesv_mixed_vbatched(n, a_ptrs, lda, b_ptrs, x_ptrs, max_iter, tol_per_n, iters, berr, info, max_n,
                          handle: Handle | None = None):
     h = handle or default_handle()

@@ -18,6 +18,8 @@ Reference signatures:
   (MatFlat/Blas.SolveTriangular.cs:162-242)
 * Blas.MulMatMat(Transpose, Transpose, int m, int n, int k, double* a, int lda,
   double* b, int ldb, double* c, int ldc)  (MatFlat/Blas.MulMatMat.cs:226-368, :407-575)
+* Blas.MulMatVec(Transpose, int m, int n, double* a, int lda, double* x, int incx,
+  double* y, int incy)  (MatFlat/Blas.MulMatVec.cs:138-206)
 """
 from __future__ import annotations
 
@@ -181,4 +183,40 @@ class Blas:
         hip.gemm_vbatched(_i32(int(transa), dev), _i32(int(transb), dev), _i32(m, dev), _i32(n, dev), _i32(k, dev),
                           one, _ptrs(a), _i32(lda, dev), _ptrs(b), _i32(ldb, dev), zero, _ptrs(c), _i32(ldc, dev),
                           info, m, n, k, complex_=cplx)
+        torch.cuda.synchronize()
+
+    @staticmethod
+    def MulMatVec(trans: int, m: int, n: int, a: torch.Tensor | None, lda: int, x: torch.Tensor | None, incx: int,
+                  y: torch.Tensor | None, incy: int) -> None:
+        """y = op(A) x, Blas.MulMatVec.cs:138-206 (double): checks in the
+        reference's order, the transpose last."""
+        if m <= 0:
+            raise ArgumentException("The number of rows must be greater than or equal to one.", "m")
+        if n <= 0:
+            raise ArgumentException("The number of columns must be greater than or equal to one.", "n")
+        if a is None:
+            raise ArgumentNullException("a")
+        if lda < m:
+            raise ArgumentException("The leading dimension must be greater than or equal to the number of rows.",
+                                    "lda")
+        if x is None:
+            raise ArgumentNullException("x")
+        if incx <= 0:
+            raise ArgumentException("The value must be greater than or equal to one.", "incx")
+        if y is None:
+            raise ArgumentNullException("y")
+        if incy <= 0:
+            raise ArgumentException("The value must be greater than or equal to one.", "incy")
+        if a.data_ptr() == y.data_ptr() or x.data_ptr() == y.data_ptr():
+            raise ArgumentException("In-place operation is not supported.")
+        if trans not in (111, 112):
+            raise ArgumentException("Invalid enum value.", "trans")
+        xl, yl = (m, n) if trans == 112 else (n, m)
+        _need(a, lda * (n - 1) + m, "a")
+        _need(x, incx * (xl - 1) + 1, "x")
+        _need(y, incy * (yl - 1) + 1, "y")
+        dev = _dev(a)
+        info = torch.zeros(1, dtype=torch.int32, device=dev)
+        hip.dgemv_vbatched(_i32(int(trans), dev), _i32(m, dev), _i32(n, dev), _ptrs(a), _i32(lda, dev), _ptrs(x),
+                           _i32(incx, dev), _ptrs(y), _i32(incy, dev), info, m, n)
         torch.cuda.synchronize()

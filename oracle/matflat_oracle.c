@@ -211,6 +211,21 @@ int mfo_dgemm(int transa, int transb, int m, int n, int k, double alpha,
     return 0;
 }
 
+/* Blas.MulMatVec.cs:185-202: one Internals.Dot per output, in output order. */
+int mfo_dgemv(int trans, int m, int n, const double* a, int lda, const double* x, int incx,
+              double* y, int incy)
+{
+    if (trans == MFO_NOTRANS) {
+        for (int i = 0; i < m; i++) y[(size_t)i * incy] = mfo_ddot(n, a + i, lda, x, incx);
+        return 0;
+    }
+    if (trans == MFO_TRANS) {
+        for (int i = 0; i < n; i++) y[(size_t)i * incy] = mfo_ddot(m, a + (size_t)i * lda, 1, x, incx);
+        return 0;
+    }
+    return -1;
+}
+
 /* System.Numerics.Complex operator * (BCL): re = a*c - b*d, im = b*c + a*d. */
 static inline void cmul(double a, double b, double c, double d, double* re, double* im)
 {

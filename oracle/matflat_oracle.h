@@ -57,6 +57,13 @@ int mfo_dgemm(int transa, int transb, int m, int n, int k, double alpha,
               const double* a, int lda, const double* b, int ldb,
               double beta, double* c, int ldc);
 
+/* Blas.MulMatVec double, Blas.MulMatVec.cs:138-206: NoTrans gives m outputs
+ * y[i*incy] = Dot(n, a + i, lda, x, incx) (row i of A); Trans gives n
+ * outputs y[i*incy] = Dot(m, a + i*lda, 1, x, incx) (column i).  Returns -1
+ * for a transpose other than NoTrans/Trans (the reference's last check). */
+int mfo_dgemv(int trans, int m, int n, const double* a, int lda, const double* x, int incx,
+              double* y, int incy);
+
 /* Blas.MulMatMat Complex, Blas.MulMatMat.cs:407-575 with Dot_NN/CN/NC/CC
  * (:598-648), same alpha/beta extension (complex alpha, beta). */
 int mfo_zgemm(int transa, int transb, int m, int n, int k, const double* alpha,

@@ -5,7 +5,7 @@ MatFlat's own tests hold no golden vectors for this path (they compare with
 OpenBLAS at 1e-12), so these fixtures are the build's own: seeded inputs and
 the oracle's bit-exact outputs for the shapes the reference's tests use
 (LuTests.cs:103-122, CholeskyTests.cs, BlasTests.SolveTriangular*.cs,
-BlasTests.MulMatMat*.cs), stored with allow_pickle=False arrays only.  The
+BlasTests.MulMatMat*.cs, BlasTests.MulMatVecDouble.cs), stored with allow_pickle=False arrays only.  The
 GPU tests compare the HIP path against them without running the oracle.
 
 Run from the repo root:  python tests/golden/make_golden.py
@@ -27,6 +27,15 @@ LU_SHAPES = [(1, 1, 1), (1, 1, 3), (2, 2, 2), (2, 2, 3), (3, 3, 3), (3, 3, 5), (
              (16, 8, 32), (23, 11, 23), (23, 11, 31), (64, 64, 64)]  # (m, n, lda): LuTests.cs:103-122, plus n = 64
 CHOL_N = [1, 3, 11, 16, 23, 33, 100]
 TRSV_N = [1, 3, 11, 16, 33]
+# (m, n, lda, incx, incy): BlasTests.MulMatVecDouble.cs:9-22 (NoTrans) and
+# :56-63 (Trans), plus shapes with more outputs than a wavefront's 64 lanes
+GEMV_NOTRANS = [(1, 1, 1, 1, 1), (1, 1, 3, 2, 4), (2, 2, 2, 1, 1), (2, 2, 4, 3, 3), (3, 3, 3, 1, 1),
+                (3, 3, 5, 6, 7), (1, 5, 1, 1, 1), (1, 5, 3, 2, 4), (4, 1, 4, 1, 1), (4, 1, 6, 5, 7),
+                (4, 9, 4, 1, 1), (4, 9, 5, 3, 2), (8, 3, 8, 1, 1), (8, 3, 9, 2, 2)]
+GEMV_TRANS = [(1, 1, 1, 1, 1), (1, 1, 3, 2, 4), (2, 2, 2, 1, 1), (2, 2, 3, 2, 3), (3, 3, 3, 1, 1),
+              (3, 3, 4, 3, 2), (1, 5, 1, 1, 1), (1, 5, 2, 2, 2), (4, 1, 4, 1, 1), (4, 1, 5, 3, 4),
+              (4, 9, 4, 1, 1), (4, 9, 5, 3, 2), (8, 3, 8, 1, 1), (8, 3, 9, 2, 2)]
+GEMV_LARGE = [(64, 100, 67, 3, 2), (130, 40, 133, 1, 1), (70, 129, 71, 2, 1)]
 GEMM_SHAPES = [(1, 1, 1), (3, 4, 5), (8, 16, 4), (17, 9, 13), (19, 21, 24)]
 
 
@@ -151,6 +160,34 @@ def gemm_fixture(rng, cplx):
             f"{p}_alpha_beta": np.array(ab, dtype=np.float64)}
 
 
+def gemv_fixture(rng):
+    """Blas.MulMatVec (double): padding rows of A and the elements of x and y
+    between strides hold values the routine must neither read into the
+    result nor overwrite (y keeps them, as the reference's tests check)."""
+    cases = [(111, *s) for s in GEMV_NOTRANS + GEMV_LARGE] + [(112, *s) for s in GEMV_TRANS + GEMV_LARGE]
+    meta, As, Xs, Yi, Yo = [], [], [], [], []
+    for trans, m, n, lda, incx, incy in cases:
+        a = np.full(lda * n, np.nan)
+        for j in range(n):
+            a[j * lda:j * lda + m] = rng.standard_normal(m)
+        xl, yl = (m, n) if trans == 112 else (n, m)
+        x = rng.standard_normal(xl * incx)
+        y = rng.standard_normal(yl * incy)
+        out = y.copy()
+        assert oracle.dgemv(trans, m, n, a, lda, x, incx, out, incy) == 0
+        meta.append((trans, m, n, lda, incx, incy))
+        As.append(a)
+        Xs.append(x)
+        Yi.append(y)
+        Yo.append(out)
+    a, aoff = pack(As)
+    x, xoff = pack(Xs)
+    y, yoff = pack(Yi)
+    yo, _ = pack(Yo)
+    return {"gemv_meta": np.array(meta, dtype=np.int64), "gemv_a": a, "gemv_a_off": aoff, "gemv_x": x,
+            "gemv_x_off": xoff, "gemv_y_in": y, "gemv_y_off": yoff, "gemv_y_out": yo}
+
+
 def main():
     oracle.build()
     rng = np.random.default_rng(20240610)
@@ -158,6 +195,7 @@ def main():
     np.savez_compressed(os.path.join(HERE, "cholesky_posv.npz"), **chol_fixture(rng))
     np.savez_compressed(os.path.join(HERE, "trsv.npz"), **trsv_fixture(rng))
     np.savez_compressed(os.path.join(HERE, "gemm.npz"), **gemm_fixture(rng, False), **gemm_fixture(rng, True))
+    np.savez_compressed(os.path.join(HERE, "gemv.npz"), **gemv_fixture(rng))
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)), "bytes")

@@ -2,7 +2,8 @@
 written by tests/golden/make_golden.py): a change to the C restatement of
 MatFlat that alters any output bit fails here.  The fixtures' shapes follow
 the reference's tests (LuTests.cs:103-122, CholeskyTests.cs,
-BlasTests.SolveTriangular*.cs, BlasTests.MulMatMat*.cs)."""
+BlasTests.SolveTriangular*.cs, BlasTests.MulMatMat*.cs,
+BlasTests.MulMatVecDouble.cs)."""
 import os
 
 import numpy as np
@@ -67,3 +68,15 @@ def test_gemm_fixtures():
             c = g[f"{p}_c_in"][c0:c0 + ldc * n].copy()
             fn(int(ta), int(tb), int(m), int(n), int(k), alpha, a, int(lda), b, int(ldb), beta, c, int(ldc))
             assert_bits_equal(c, g[f"{p}_c_out"][c0:c0 + ldc * n], f"{p} {i}")
+
+
+def test_gemv_fixtures():
+    g = load("gemv.npz")
+    for i, (trans, m, n, lda, incx, incy) in enumerate(g["gemv_meta"]):
+        xl, yl = (m, n) if trans == 112 else (n, m)
+        a0, x0, y0 = g["gemv_a_off"][i], g["gemv_x_off"][i], g["gemv_y_off"][i]
+        a = g["gemv_a"][a0:a0 + lda * n]
+        x = g["gemv_x"][x0:x0 + xl * incx]
+        y = g["gemv_y_in"][y0:y0 + yl * incy].copy()
+        assert oracle.dgemv(int(trans), int(m), int(n), a, int(lda), x, int(incx), y, int(incy)) == 0
+        assert_bits_equal(y, g["gemv_y_out"][y0:y0 + yl * incy], f"gemv {i}")

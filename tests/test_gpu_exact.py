@@ -438,6 +438,97 @@ def test_matflat_signatures_on_device():
         Blas.MulMatMat(Transpose.NoTrans, Transpose.NoTrans, 2, 2, 2, aa, 2, aa, 2, aa, 2)
 
 
+# ------------------------------------------------------------------ GEMV
+@pytest.mark.parametrize("seed", [0, 1])
+def test_gemv_variable_sizes_bit_exact(seed):
+    """A batch of 300 products with m, n ~ U{1..300}, random transposes,
+    leading dimensions and strides: y bit-identical to the oracle."""
+    hip = hipmod()
+    rng = np.random.default_rng(4000 + seed)
+    cnt = 300
+    tr = rng.choice([111, 112], cnt)
+    m = rng.integers(1, 301, cnt)
+    n = rng.integers(1, 301, cnt)
+    lda = m + rng.integers(0, 4, cnt)
+    incx = rng.integers(1, 4, cnt)
+    incy = rng.integers(1, 4, cnt)
+    xl = np.where(tr == 112, m, n)
+    yl = np.where(tr == 112, n, m)
+    As = [rng.standard_normal(int(l * c)) for l, c in zip(lda, n)]
+    Xs = [rng.standard_normal(int(l * c)) for l, c in zip(xl, incx)]
+    Ys = [rng.standard_normal(int(l * c)) for l, c in zip(yl, incy)]
+    A, X, Y = Packed(As), Packed(Xs), Packed([y.copy() for y in Ys])
+    info = i32(np.full(cnt, 99))
+    hip.dgemv_vbatched(i32(tr), i32(m), i32(n), A.ptrs, i32(lda), X.ptrs, i32(incx), Y.ptrs, i32(incy), info,
+                       int(m.max()), int(n.max()))
+    torch.cuda.synchronize()
+    assert np.all(info.cpu().numpy() == 0)
+    gy = Y.all()
+    for i in range(cnt):
+        want = Ys[i].copy()
+        oracle.dgemv(int(tr[i]), int(m[i]), int(n[i]), As[i], int(lda[i]), Xs[i], int(incx[i]), want, int(incy[i]))
+        assert_bits_equal(gy[i], want, f"gemv {i}")
+
+
+def test_gemv_argument_checks():
+    """Per-product checks in Blas.MulMatVec.cs order, the transpose last:
+    info = -(position in mfh_dgemv_vbatched's argument list), -1 for y
+    aliasing a or x; the other products of the batch still run."""
+    hip = hipmod()
+    from matflat_amd.hip import ArgumentError
+    good = np.arange(1.0, 7.0)  # 2 x 3, lda 2
+    cases = [  # (trans, m, n, lda, incx, incy, y aliases a) -> info
+        ((111, 2, 3, 2, 1, 1, False), 0),
+        ((111, 0, 3, 2, 1, 1, False), -4),
+        ((111, 2, 0, 2, 1, 1, False), -5),
+        ((111, 2, 3, 1, 1, 1, False), -7),
+        ((111, 2, 3, 2, 0, 1, False), -9),
+        ((111, 2, 3, 2, 1, 0, False), -11),
+        ((111, 2, 3, 2, 1, 1, True), -1),
+        ((113, 2, 3, 2, 1, 1, False), -3),
+        ((113, 0, 3, 2, 1, 1, False), -4),  # the transpose is checked last
+        ((112, 2, 3, 2, 1, 1, False), 0),
+    ]
+    A = Packed([good.copy() for _ in cases])
+    X = Packed([np.ones(3) for _ in cases])
+    Y = Packed([np.full(3, 7.0) for _ in cases])
+    yp = Y.ptrs.clone()
+    for i, (c, _) in enumerate(cases):
+        if c[6]:
+            yp[i] = A.ptrs[i]
+    col = lambda j: i32([c[j] for c, _ in cases])  # noqa: E731
+    info = i32(np.full(len(cases), 99))
+    hip.dgemv_vbatched(col(0), col(1), col(2), A.ptrs, col(3), X.ptrs, col(4), yp, col(5), info, 2, 3)
+    torch.cuda.synchronize()
+    assert info.cpu().tolist() == [w for _, w in cases]
+    gy = Y.all()
+    assert gy[0][:2].tolist() == [9.0, 12.0]  # rows of [[1, 3, 5], [2, 4, 6]] summed
+    assert gy[9].tolist() == [3.0, 7.0, 11.0]  # columns summed
+    for i in range(1, 9):
+        assert np.all(gy[i] == 7.0)
+    with pytest.raises(ArgumentError):
+        hip.dgemv_vbatched(col(0), col(1), col(2), None, col(3), X.ptrs, col(4), Y.ptrs, col(5), info, 2, 3)
+
+
+def test_matflat_mulmatvec_on_device():
+    from matflat_amd.matflat import ArgumentException, Blas, Transpose
+    m, n, lda = 23, 11, 31
+    a = rand_buf(42, m, n, lda)
+    x = np.random.default_rng(5).standard_normal(2 * n)
+    y = np.full(3 * m, 0.25)
+    da, dx, dy = (torch.from_numpy(v.copy()).cuda() for v in (a, x, y))
+    Blas.MulMatVec(Transpose.NoTrans, m, n, da, lda, dx, 2, dy, 3)
+    want = y.copy()
+    oracle.dgemv(111, m, n, a, lda, x, 2, want, 3)
+    assert_bits_equal(dy.cpu().numpy(), want, "MulMatVec NoTrans")
+    with pytest.raises(ArgumentException):
+        Blas.MulMatVec(Transpose.NoTrans, m, n, da, lda, dx, 2, da, 3)
+    with pytest.raises(ArgumentException):
+        Blas.MulMatVec(Transpose.ConjTrans, m, n, da, lda, dx, 2, dy, 3)
+    with pytest.raises(ArgumentException):
+        Blas.MulMatVec(Transpose.NoTrans, m, n, da, m - 1, dx, 2, dy, 3)
+
+
 # ------------------------------------------------------------------ FAST math mode
 @pytest.mark.parametrize("waves", [0, 1, 2])
 @pytest.mark.parametrize("posv", [False, True])

@@ -1,8 +1,8 @@
 """The HIP exact path against the committed golden fixtures
 (tests/golden/*.npz, written by tests/golden/make_golden.py from the
 oracle): LU factors, pivots and signs, Cholesky factors, SPD-solve x, the
-eight triangular-solve forms and GEMM (real and complex, every transpose /
-conjugate combination) must match bit for bit.  Shapes follow the
+eight triangular-solve forms, GEMM (real and complex, every transpose /
+conjugate combination) and GEMV must match bit for bit.  Shapes follow the
 reference's tests (LuTests.cs:103-122, CholeskyTests.cs,
 BlasTests.SolveTriangular*.cs, BlasTests.MulMatMat*.cs)."""
 import os
@@ -134,3 +134,28 @@ def test_golden_gemm(p):
     for i in range(len(meta)):
         o = g[f"{p}_c_off"][i]
         assert_bits_equal(gc[i], g[f"{p}_c_out"][o:o + ldc[i] * n[i]], f"{p} {i} {tuple(meta[i])}")
+
+
+def test_golden_gemv():
+    """Blas.MulMatVec (double), NoTrans and Trans in one batch, with strides
+    and NaN padding rows: y bit-identical, elements between strides kept."""
+    hip = hipmod()
+    g = load("gemv.npz")
+    meta = g["gemv_meta"]
+    tr, m, n, lda, incx, incy = (meta[:, j] for j in range(6))
+    xl = np.where(tr == 112, m, n)
+    yl = np.where(tr == 112, n, m)
+    A = Packed([g["gemv_a"][o:o + l * c] for o, l, c in zip(g["gemv_a_off"], lda, n)])
+    X = Packed([g["gemv_x"][o:o + l * c] for o, l, c in zip(g["gemv_x_off"], xl, incx)])
+    Y = Packed([g["gemv_y_in"][o:o + l * c].copy() for o, l, c in zip(g["gemv_y_off"], yl, incy)])
+    info = i32(np.full(len(meta), 99))
+    h = exact_handle()
+    hip.dgemv_vbatched(i32(tr), i32(m), i32(n), A.ptrs, i32(lda), X.ptrs, i32(incx), Y.ptrs, i32(incy), info,
+                       int(m.max()), int(n.max()), handle=h)
+    torch.cuda.synchronize()
+    h.close()
+    assert np.all(info.cpu().numpy() == 0)
+    gy = Y.all()
+    for i in range(len(meta)):
+        o = g["gemv_y_off"][i]
+        assert_bits_equal(gy[i], g["gemv_y_out"][o:o + yl[i] * incy[i]], f"gemv {i} {tuple(meta[i])}")

@@ -274,6 +274,56 @@ def test_dgemm_matches_numpy(ta, tb, m, n, k, lda0, ldb0, ldc):
     np.testing.assert_allclose(view(c2, m, n, ldc), -0.75 * expect + 0.5 * view(c0, m, n, ldc), atol=1e-12)
 
 
+# ---------------------------------------------------------------- GEMV
+# (m, n, lda, incx, incy) of BlasTests.MulMatVecDouble.cs:9-22 and :56-63
+GEMV_CASES = [(1, 1, 1, 1, 1), (1, 1, 3, 2, 4), (2, 2, 4, 3, 3), (3, 3, 5, 6, 7), (1, 5, 3, 2, 4),
+              (4, 1, 6, 5, 7), (4, 9, 5, 3, 2), (8, 3, 9, 2, 2), (1, 5, 2, 2, 2), (4, 1, 5, 3, 4),
+              (70, 129, 71, 2, 1)]
+
+
+@pytest.mark.parametrize("trans", [111, 112])
+@pytest.mark.parametrize("m,n,lda,incx,incy", GEMV_CASES)
+def test_dgemv_matches_openblas(trans, m, n, lda, incx, incy):
+    """Blas.MulMatVec against OpenBLAS dgemv at 1e-12, as the reference's
+    tests check it; elements of y between strides stay untouched."""
+    from scipy.linalg import blas
+    a = rand_buf(42, m, n, lda)
+    xl, yl = (m, n) if trans == 112 else (n, m)
+    rng = np.random.default_rng(57)
+    x = rng.standard_normal(xl * incx)
+    y = rng.standard_normal(yl * incy)
+    y0 = y.copy()
+    assert oracle.dgemv(trans, m, n, a, lda, x, incx, y, incy) == 0
+    want = blas.dgemv(1.0, view(a, m, n, lda), x[::incx], trans=int(trans == 112))
+    np.testing.assert_allclose(y[::incy], want, rtol=0, atol=1e-12)
+    mask = np.ones(y.size, dtype=bool)
+    mask[::incy] = False
+    assert np.array_equal(y[mask], y0[mask])
+
+
+def test_dgemv_summation_order():
+    """Every output is one Internals.Dot: NoTrans over a row (stride lda),
+    Trans over a column.  Row [2^60, 1, -2^60, 1] (even length) gives 0 in
+    MatFlat's pairing and 1 left to right; column [1, 2^60, -2^60, 1, 1]
+    (odd length) gives 3, where left to right gives 2."""
+    big = 2.0**60
+    lda = 2
+    a = np.array([big, np.nan, 1.0, np.nan, -big, np.nan, 1.0, np.nan])  # 1 x 4, lda 2
+    y = np.zeros(1)
+    assert oracle.dgemv(111, 1, 4, a, lda, np.ones(4), 1, y, 1) == 0
+    assert y[0] == 0.0
+    col = np.array([1.0, big, -big, 1.0, 1.0])
+    assert oracle.dgemv(112, 5, 1, col, 5, np.ones(5), 1, y, 1) == 0
+    assert y[0] == 3.0
+    # A^T x with A = [[1, 2], [3, 4]] (column-major [1, 3, 2, 4]), x = [1, 1]
+    y2 = np.zeros(2)
+    oracle.dgemv(112, 2, 2, np.array([1.0, 3.0, 2.0, 4.0]), 2, np.ones(2), 1, y2, 1)
+    assert y2.tolist() == [4.0, 6.0]
+    oracle.dgemv(111, 2, 2, np.array([1.0, 3.0, 2.0, 4.0]), 2, np.ones(2), 1, y2, 1)
+    assert y2.tolist() == [3.0, 7.0]
+    assert oracle.dgemv(113, 2, 2, np.ones(4), 2, np.ones(2), 1, y2, 1) == -1
+
+
 @pytest.mark.parametrize("ta", [111, 112, 113, 114])
 @pytest.mark.parametrize("tb", [111, 112, 113, 114])
 @pytest.mark.parametrize("m,n,k,lda0,ldb0,ldc", GEMM_CASES[:5])
