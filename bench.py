@@ -334,14 +334,28 @@ def lu_leg(steps: int, warmup: int, count: int = 1024, n: int = 64, seed: int = 
                      lambda: a.copy_(src), steps, warmup, stream)
     torch.cuda.synchronize()
     ok = bool(torch.all(info == 0).item())
+    # Factorization.LuInverse of the factored batch; each timed call restores the factors
+    lu_src = a.clone()
+    idx = torch.arange(count, dtype=torch.int64, device=dev)
+    a_ptrs = a.data_ptr() + 8 * n * n * idx
+    piv_ptrs = piv.data_ptr() + 4 * n * idx
+    n_t = torch.full((count,), n, dtype=torch.int32, device=dev)
+    inv_info = torch.zeros(count, dtype=torch.int32, device=dev)
+    ms_inv = timed_calls(lambda: hip.dgetri_vbatched(n_t, a_ptrs, n_t, piv_ptrs, inv_info, n, handle=handle),
+                         lambda: a.copy_(lu_src), steps, warmup, stream)
+    inv_ok = bool(torch.all(inv_info == 0).item())
     handle.close()
     alg = count * (16.0 * n * n + 4 * n + 4)
     res = {"workload": f"{count} LU factorizations (partial pivoting), n = {n}, lda = {n}, N(0,1), exact mode "
                        "(bit-identical to MatFlat's Factorization.Lu)",
            "ms_per_call": round(ms, 5), "factorizations_per_s": round(count / (ms / 1e3), 1),
-           "all_info_zero": ok, "roofline": hbm_entry(alg, ms)}
+           "all_info_zero": ok, "roofline": hbm_entry(alg, ms),
+           "inverse": {"workload": f"Factorization.LuInverse of the same {count} factored matrices, exact mode "
+                                   "(bit-identical)",
+                       "ms_per_call": round(ms_inv, 5), "inverses_per_s": round(count / (ms_inv / 1e3), 1),
+                       "all_info_zero": inv_ok, "roofline": hbm_entry(alg, ms_inv)}}
     log(f"  LU n={n} x{count}: {ms:.4f} ms/call, {res['factorizations_per_s']:.0f} LU/s, "
-        f"{res['roofline']['achieved']} GB/s")
+        f"{res['roofline']['achieved']} GB/s; inverse {ms_inv:.4f} ms/call")
     return res
 
 

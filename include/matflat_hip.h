@@ -110,6 +110,7 @@ int mfh_get_fast_waves(mfh_handle* handle, int row_blocks, int* waves);
 #define MFH_KERNEL_GESV_F64 8
 #define MFH_KERNEL_GESV_STEP 9
 #define MFH_KERNEL_GEMV_EXACT 10
+#define MFH_KERNEL_GETRI_SETUP 11
 int mfh_set_timing(mfh_handle* handle, int enable);
 int mfh_get_timing(mfh_handle* handle, int capacity, int* n_records, int* kind, int* rows,
                    float* ms);
@@ -165,6 +166,19 @@ int mfh_dposv_vbatched(mfh_handle* h, int count, const int* n, double* const* a,
 int mfh_dgetrs_vbatched(mfh_handle* h, int count, const int* n, const double* const* lu,
                         const int* lda, const int* const* piv, const double* const* b,
                         double* const* x, int* info, int max_n);
+
+/* ---- LU inverse ------------------------------------------------------------
+ * Factorization.LuInverse(int n, double* a, int lda, int* piv)
+ * (Factorization.LuInverse.cs:105-192) for a batch: a and piv hold
+ * mfh_dgetrf_vbatched's output, and a is overwritten by A^-1, bit-identical
+ * to MatFlat (column j is the LU-solve composition above applied to e_j).
+ * A matrix whose U has a zero diagonal entry gets info = j + 1 for the first
+ * such j and is left unchanged (MatFlat throws "The matrix is not
+ * invertible." before writing).  Padding rows are never touched.
+ * n <= MFH_EXACT_MAX_N; count * max_n < 2^31.  Staging lives in a second
+ * handle buffer of about count * max_n^2 * 8 bytes. */
+int mfh_dgetri_vbatched(mfh_handle* h, int count, const int* n, double* const* a, const int* lda,
+                        const int* const* piv, int* info, int max_n);
 
 /* ---- GEMM ------------------------------------------------------------------
  * Replaces Blas.MulMatMat(Transpose, Transpose, m, n, k, a, lda, b, ldb, c,

@@ -39,6 +39,23 @@ int mfh_ws_reserve(mfh_handle* h, size_t bytes)
     return MFH_OK;
 }
 
+int mfh_aux_reserve(mfh_handle* h, size_t bytes)
+{
+    if (h->aux_bytes >= bytes) return MFH_OK;
+    if (h->capturing) return mfh_fail(MFH_ERR_STATE, "workspace must be reserved before graph capture");
+    size_t want = bytes < (1u << 20) ? (1u << 20) : bytes + bytes / 4;
+    if (h->aux) {
+        MFH_HIP_CHECK(hipStreamSynchronize(h->stream));
+        MFH_HIP_CHECK(hipFree(h->aux));
+        h->aux = nullptr;
+        h->aux_bytes = 0;
+    }
+    hipError_t e = hipMalloc(&h->aux, want);
+    if (e != hipSuccess) return mfh_fail(MFH_ERR_ALLOC, std::string("hipMalloc workspace: ") + hipGetErrorString(e));
+    h->aux_bytes = want;
+    return MFH_OK;
+}
+
 namespace {
 int take_event(mfh_handle* h, hipEvent_t* ev)
 {
@@ -104,10 +121,9 @@ int mfh_create(mfh_handle** handle)
 int mfh_destroy(mfh_handle* h)
 {
     MFH_REQUIRE(h != nullptr, 1, "handle must not be null");
-    if (h->ws) {
-        hipStreamSynchronize(h->stream);
-        hipFree(h->ws);
-    }
+    if (h->ws || h->aux) hipStreamSynchronize(h->stream);
+    if (h->ws) hipFree(h->ws);
+    if (h->aux) hipFree(h->aux);
     for (auto& t : h->timed) {
         hipEventSynchronize(t.stop);
         hipEventDestroy(t.start);
@@ -316,6 +332,23 @@ int mfh_dgetrs_vbatched(mfh_handle* h, int count, const int* n, const double* co
     MFH_REQUIRE(max_n >= 1 && max_n <= MFH_EXACT_MAX_N, 10, "max_n must be in [1, MFH_EXACT_MAX_N]");
     MfhGetrsBatch B{n, lu, lda, piv, b, x, info};
     return mfh_launch_getrs(h, count, B, max_n);
+}
+
+int mfh_dgetri_vbatched(mfh_handle* h, int count, const int* n, double* const* a, const int* lda,
+                        const int* const* piv, int* info, int max_n)
+{
+    MFH_REQUIRE(h != nullptr, 1, "handle must not be null");
+    MFH_REQUIRE(count >= 0, 2, "count must be >= 0");
+    if (count == 0) return MFH_OK;
+    MFH_REQUIRE(n != nullptr, 3, "n must not be null");
+    MFH_REQUIRE(a != nullptr, 4, "a must not be null");
+    MFH_REQUIRE(lda != nullptr, 5, "lda must not be null");
+    MFH_REQUIRE(piv != nullptr, 6, "piv must not be null");
+    MFH_REQUIRE(info != nullptr, 7, "info must not be null");
+    MFH_REQUIRE(max_n >= 1 && max_n <= MFH_EXACT_MAX_N, 8, "max_n must be in [1, MFH_EXACT_MAX_N]");
+    MFH_REQUIRE((long long)count * max_n <= 0x7fffffffLL, 2, "count * max_n must be < 2^31");
+    MfhGetriBatch B{n, a, lda, piv, info};
+    return mfh_launch_getri(h, count, B, max_n);
 }
 
 static int gemm_common(mfh_handle* h, int count, const int* transa, const int* transb, const int* m, const int* n,

@@ -929,5 +929,102 @@ int mfh_launch_getrs(mfh_handle* h, int count, const MfhGetrsBatch& B, int max_n
     return MFH_OK;
 }
 
+// ------------------------------------------------------------ LU inverse
+// Factorization.LuInverse.cs:105-192 sets X = P (X[j, piv[j]] = 1) and runs
+// the unit-lower and the upper solve of :156-174 on every column of X, so
+// column j of A^-1 is exactly the LU-solve composition of k_getrs_exact
+// applied to b = e_j.  The inverse is therefore max_n solves per matrix
+// through the same kernel: k_getri_setup checks each matrix (the first zero
+// diagonal entry of U gives info = j + 1 and leaves the matrix unchanged,
+// as the reference throws before writing), copies its factors aside (A is
+// overwritten column by column) and writes the solve list; columns j >= n
+// and rejected matrices get n = 0, which the solve's own check skips.
+struct GetriSolves {
+    int* n;
+    const double** lu;
+    int* lda;
+    const int** piv;
+    const double** b;
+    double** x;
+    int* info;
+    double* copy;   // count x (max_n x max_n) factors, leading dimension max_n
+    double* ident;  // max_n x max_n identity: column j is e_j
+};
+
+__global__ __launch_bounds__(256) void k_getri_setup(MfhGetriBatch B, int count, int max_n, GetriSolves S)
+{
+    __shared__ int s_zero;
+    const int i = blockIdx.x, tid = threadIdx.x;
+    if (i < max_n)
+        for (int k = tid; k < max_n; k += 256) S.ident[k + (size_t)i * max_n] = (k == i) ? 1.0 : 0.0;
+    if (i >= count) return;
+    const int n = B.n[i];
+    const int lda = B.lda[i];
+    double* a = B.a[i];
+    const int* piv = B.piv[i];
+    int bad = 0;
+    if (n < 1 || n > max_n) bad = -3;
+    else if (a == nullptr) bad = -4;
+    else if (lda < n || (long long)lda * n > MFH_MAX_MATRIX_ELEMS) bad = -5;
+    else if (piv == nullptr) bad = -6;
+    if (tid == 0) s_zero = INT_MAX;
+    __syncthreads();
+    if (!bad)
+        for (int j = tid; j < n; j += 256)
+            if (a[j + (size_t)j * lda] == 0.0) atomicMin(&s_zero, j);
+    __syncthreads();
+    if (!bad && s_zero != INT_MAX) bad = s_zero + 1;
+    double* cp = S.copy + (size_t)i * max_n * max_n;
+    if (!bad)
+        for (int e = tid; e < n * n; e += 256) {
+            const int j = e / n;
+            const int r = e - j * n;
+            cp[r + (size_t)j * max_n] = a[r + (size_t)j * lda];
+        }
+    for (int j = tid; j < max_n; j += 256) {
+        const size_t s = (size_t)i * max_n + j;
+        const bool live = !bad && j < n;
+        S.n[s] = live ? n : 0;
+        S.lu[s] = cp;
+        S.lda[s] = max_n;
+        S.piv[s] = piv;
+        S.b[s] = S.ident + (size_t)j * max_n;
+        S.x[s] = live ? a + (size_t)j * lda : nullptr;
+    }
+    if (tid == 0) B.info[i] = bad;
+}
+
 // Workspace bytes a batch of count matrices needs (plans reserve this).
 size_t mfh_exact_ws_bytes(int count) { return sort_ws_bytes(count); }
+
+int mfh_launch_getri(mfh_handle* h, int count, const MfhGetriBatch& B, int max_n)
+{
+    const size_t solves = (size_t)count * max_n;
+    size_t off = 0;
+    auto take = [&off](size_t bytes) {
+        const size_t o = off;
+        off += (bytes + 255) & ~(size_t)255;
+        return o;
+    };
+    const size_t o_copy = take(solves * max_n * sizeof(double));
+    const size_t o_ident = take((size_t)max_n * max_n * sizeof(double));
+    const size_t o_n = take(solves * sizeof(int)), o_lda = take(solves * sizeof(int));
+    const size_t o_info = take(solves * sizeof(int));
+    const size_t o_lu = take(solves * sizeof(void*)), o_piv = take(solves * sizeof(void*));
+    const size_t o_b = take(solves * sizeof(void*)), o_x = take(solves * sizeof(void*));
+    int st = mfh_aux_reserve(h, off);
+    if (st) return st;
+    char* base = (char*)h->aux;
+    GetriSolves S{(int*)(base + o_n),       (const double**)(base + o_lu), (int*)(base + o_lda),
+                  (const int**)(base + o_piv), (const double**)(base + o_b), (double**)(base + o_x),
+                  (int*)(base + o_info),    (double*)(base + o_copy),      (double*)(base + o_ident)};
+    st = mfh_timing_begin(h, MFH_KERNEL_GETRI_SETUP, max_n);
+    if (st) return st;
+    const unsigned grid = (unsigned)(count > max_n ? count : max_n);
+    hipLaunchKernelGGL(k_getri_setup, dim3(grid), dim3(256), 0, h->stream, B, count, max_n, S);
+    MFH_HIP_CHECK(hipGetLastError());
+    st = mfh_timing_end(h);
+    if (st) return st;
+    MfhGetrsBatch G{S.n, S.lu, S.lda, S.piv, S.b, S.x, S.info};
+    return mfh_launch_getrs(h, (int)solves, G, max_n);
+}

@@ -33,6 +33,8 @@ struct mfh_handle {
     int num_cus = 0;
     void* ws = nullptr;  // device workspace
     size_t ws_bytes = 0;
+    void* aux = nullptr;  // second device buffer: staging of calls that run another entry's kernels on ws
+    size_t aux_bytes = 0;
     bool capturing = false;  // plans pre-size the workspace; no allocation while set
     int math_mode = MFH_MATH_EXACT;
     int fast_waves[18] = {};              // FAST Cholesky waves per matrix by row blocks (0 = default)
@@ -47,6 +49,8 @@ int mfh_fail(int status, const std::string& msg);
 
 // Grows the handle workspace to at least bytes (device memory).
 int mfh_ws_reserve(mfh_handle* h, size_t bytes);
+// Grows the handle's second buffer (aux) to at least bytes.
+int mfh_aux_reserve(mfh_handle* h, size_t bytes);
 
 // Per-launch timing: when h->timing is set (and no plan is being captured),
 // mfh_timing_begin records a start event of a new record on the handle's
@@ -103,6 +107,14 @@ struct MfhGetrsBatch {
     const int* const* piv;
     const double* const* b;
     double* const* x;
+    int* info;
+};
+
+struct MfhGetriBatch {
+    const int* n;
+    double* const* a;
+    const int* lda;
+    const int* const* piv;
     int* info;
 };
 
@@ -171,6 +183,7 @@ int mfh_launch_trsv(mfh_handle* h, int count, const MfhTrsvBatch& B, int max_n);
 int mfh_launch_getrs(mfh_handle* h, int count, const MfhGetrsBatch& B, int max_n);
 int mfh_launch_gemm(mfh_handle* h, int count, const MfhGemmBatch& B, int max_m, int max_n,
                     int max_k, bool complex);
+int mfh_launch_getri(mfh_handle* h, int count, const MfhGetriBatch& B, int max_n);
 int mfh_launch_gemv(mfh_handle* h, int count, const MfhGemvBatch& B, int max_m, int max_n);
 int mfh_launch_posv_backward(mfh_handle* h, int count, const MfhAdjointBatch& B, int max_n);
 int mfh_launch_gesv_mixed(mfh_handle* h, int count, const int* n, const double* const* a,

@@ -200,6 +200,12 @@ def gemm_vbatched(transa, transb, m, n, k, alpha, a_ptrs, lda, b_ptrs, ldb, beta
               _p(b_ptrs), _p(ldb), _p(beta), _p(c_ptrs), _p(ldc), _p(info), int(max_m), int(max_n), int(max_k)))
 
 
+def dgetri_vbatched(n, a_ptrs, lda, piv_ptrs, info, max_n, handle: Handle | None = None):
+    h = handle or default_handle()
+    _check(h.lib.mfh_dgetri_vbatched(h.h, int(n.numel()), _p(n), _p(a_ptrs), _p(lda), _p(piv_ptrs), _p(info),
+                                     int(max_n)))
+
+
 def dgemv_vbatched(trans, m, n, a_ptrs, lda, x_ptrs, incx, y_ptrs, incy, info, max_m, max_n,
                    handle: Handle | None = None):
     h = handle or default_handle()

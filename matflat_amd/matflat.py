@@ -14,6 +14,8 @@ Reference signatures:
   (MatFlat/Factorization.Lu.cs:122-161)
 * Factorization.Cholesky(int n, double* a, int lda)
   (MatFlat/Factorization.Cholesky.cs:114-168)
+* Factorization.LuInverse(int n, double* a, int lda, int* piv)
+  (MatFlat/Factorization.LuInverse.cs:105-192)
 * Blas.SolveTriangular(Uplo, Transpose, int n, double* a, int lda, double* x, int incx)
   (MatFlat/Blas.SolveTriangular.cs:162-242)
 * Blas.MulMatMat(Transpose, Transpose, int m, int n, int k, double* a, int lda,
@@ -115,6 +117,27 @@ class Factorization:
         if int(info.item()) != 0:
             raise MatrixFactorizationException(
                 "Cholesky decomposition failed. The matrix must be positive definite.")
+
+    @staticmethod
+    def LuInverse(n: int, a: torch.Tensor | None, lda: int, piv: torch.Tensor | None) -> None:
+        """A^-1 from Lu's factors and pivots, in place
+        (Factorization.LuInverse.cs:105-192, double)."""
+        if n <= 0:
+            raise ArgumentException("The order of the matrix must be greater than or equal to one.", "n")
+        if a is None:
+            raise ArgumentNullException("a")
+        if lda < n:
+            raise ArgumentException(
+                "The leading dimension must be greater than or equal to the order of the matrix.", "lda")
+        if piv is None:
+            raise ArgumentNullException("piv")
+        _need(a, lda * (n - 1) + n, "a")
+        _need(piv, n, "piv")
+        dev = _dev(a)
+        info = torch.zeros(1, dtype=torch.int32, device=dev)
+        hip.dgetri_vbatched(_i32(n, dev), _ptrs(a), _i32(lda, dev), _ptrs(piv), info, n)
+        if int(info.item()) != 0:
+            raise MatrixFactorizationException("The matrix is not invertible.")
 
 
 class Blas:

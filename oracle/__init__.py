@@ -74,6 +74,8 @@ def lib() -> ctypes.CDLL:
         _lib.mfo_dposv.argtypes = [i, p, i, p]
         _lib.mfo_dgemm.restype = i
         _lib.mfo_dgemm.argtypes = [i, i, i, i, i, d, p, i, p, i, d, p, i]
+        _lib.mfo_dgetri.restype = i
+        _lib.mfo_dgetri.argtypes = [i, p, i, p]
         _lib.mfo_dgemv.restype = i
         _lib.mfo_dgemv.argtypes = [i, i, i, p, i, p, i, p, i]
         _lib.mfo_zgemm.restype = i
@@ -135,6 +137,11 @@ def dposv(n, a, lda, x) -> int:
 def dgemm(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc) -> int:
     return lib().mfo_dgemm(transa, transb, m, n, k, float(alpha), _ptr(a), lda, _ptr(b), ldb,
                            float(beta), _ptr(c), ldc)
+
+
+def dgetri(n, a, lda, piv) -> int:
+    piv = np.ascontiguousarray(piv, dtype=np.int32)
+    return lib().mfo_dgetri(n, _ptr(a), lda, _ptr(piv))
 
 
 def dgemv(trans, m, n, a, lda, x, incx, y, incy) -> int:

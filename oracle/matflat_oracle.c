@@ -211,6 +211,32 @@ int mfo_dgemm(int transa, int transb, int m, int n, int k, double alpha,
     return 0;
 }
 
+/* Factorization.LuInverse.cs:127-192: check the diagonal (:127-134); copy
+ * the factors to work (ld n) and clear A's columns (:141-149); X = P with
+ * X[j, piv[j]] = 1 (:151-154); a Lower/NoTrans solve of every column with
+ * the copy's diagonal set to 1 (:156-164); the diagonal restored and an
+ * Upper/NoTrans solve of every column (:166-174). */
+int mfo_dgetri(int n, double* a, int lda, const int* piv)
+{
+    for (int j = 0; j < n; j++)
+        if (a[(size_t)lda * j + j] == 0.0) return j + 1;
+    double* work = (double*)malloc(sizeof(double) * ((size_t)n * n + n));
+    double* diag = work + (size_t)n * n;
+    for (int j = 0; j < n; j++) {
+        double* colj = a + (size_t)lda * j;
+        memcpy(work + (size_t)n * j, colj, sizeof(double) * n);
+        diag[j] = colj[j];
+        memset(colj, 0, sizeof(double) * n);
+    }
+    for (int j = 0; j < n; j++) a[(size_t)lda * piv[j] + j] = 1.0;
+    for (int j = 0; j < n; j++) work[(size_t)n * j + j] = 1.0;
+    for (int j = 0; j < n; j++) mfo_dtrsv(MFO_LOWER, MFO_NOTRANS, n, work, n, a + (size_t)lda * j, 1);
+    for (int j = 0; j < n; j++) work[(size_t)n * j + j] = diag[j];
+    for (int j = 0; j < n; j++) mfo_dtrsv(MFO_UPPER, MFO_NOTRANS, n, work, n, a + (size_t)lda * j, 1);
+    free(work);
+    return 0;
+}
+
 /* Blas.MulMatVec.cs:185-202: one Internals.Dot per output, in output order. */
 int mfo_dgemv(int trans, int m, int n, const double* a, int lda, const double* x, int incx,
               double* y, int incy)

@@ -47,6 +47,12 @@ void mfo_dtrsv_unit_lower(int n, const double* a, int lda, double* x, int incx);
  * right-hand side: y[j] = b[piv[j]]; y <- L1^-1 y; x <- U^-1 y. */
 void mfo_dgetrs(int n, const double* lu, int lda, const int* piv, const double* b, double* x);
 
+/* Factorization.LuInverse double, Factorization.LuInverse.cs:105-192: a and
+ * piv hold Factorization.Lu's output; a is overwritten by A^-1.  Returns
+ * j+1 for the first zero diagonal entry of U (the reference throws
+ * MatrixFactorizationException before writing anything), else 0. */
+int mfo_dgetri(int n, double* a, int lda, const int* piv);
+
 /* Cholesky followed by Lower/NoTrans and Lower/Trans solves. Returns info. */
 int mfo_dposv(int n, double* a, int lda, double* x);
 

@@ -5,7 +5,7 @@ MatFlat's own tests hold no golden vectors for this path (they compare with
 OpenBLAS at 1e-12), so these fixtures are the build's own: seeded inputs and
 the oracle's bit-exact outputs for the shapes the reference's tests use
 (LuTests.cs:103-122, CholeskyTests.cs, BlasTests.SolveTriangular*.cs,
-BlasTests.MulMatMat*.cs, BlasTests.MulMatVecDouble.cs), stored with allow_pickle=False arrays only.  The
+BlasTests.MulMatMat*.cs, BlasTests.MulMatVecDouble.cs, LuInverseTests.cs), stored with allow_pickle=False arrays only.  The
 GPU tests compare the HIP path against them without running the oracle.
 
 Run from the repo root:  python tests/golden/make_golden.py
@@ -36,6 +36,8 @@ GEMV_TRANS = [(1, 1, 1, 1, 1), (1, 1, 3, 2, 4), (2, 2, 2, 1, 1), (2, 2, 3, 2, 3)
               (3, 3, 4, 3, 2), (1, 5, 1, 1, 1), (1, 5, 2, 2, 2), (4, 1, 4, 1, 1), (4, 1, 5, 3, 4),
               (4, 9, 4, 1, 1), (4, 9, 5, 3, 2), (8, 3, 8, 1, 1), (8, 3, 9, 2, 2)]
 GEMV_LARGE = [(64, 100, 67, 3, 2), (130, 40, 133, 1, 1), (70, 129, 71, 2, 1)]
+GETRI_SHAPES = [(1, 1), (1, 3), (2, 2), (2, 3), (3, 3), (3, 5), (5, 5), (5, 8), (10, 10), (10, 17), (64, 64),
+                (100, 103)]  # (n, lda): LuInverseTests.cs:75-84, plus n = 64 and 100
 GEMM_SHAPES = [(1, 1, 1), (3, 4, 5), (8, 16, 4), (17, 9, 13), (19, 21, 24)]
 
 
@@ -188,6 +190,28 @@ def gemv_fixture(rng):
             "gemv_x_off": xoff, "gemv_y_in": y, "gemv_y_off": yoff, "gemv_y_out": yo}
 
 
+def getri_fixture(rng):
+    """Factorization.LuInverse (double) applied to Factorization.Lu's output;
+    padding rows hold NaN, which must survive."""
+    meta, lus, pivs, outs = [], [], [], []
+    for n, lda in GETRI_SHAPES:
+        a = np.full(lda * n, np.nan)
+        for j in range(n):
+            a[j * lda:j * lda + n] = rng.standard_normal(n)
+        piv, _ = oracle.dgetrf(n, n, a, lda)
+        inv = a.copy()
+        assert oracle.dgetri(n, inv, lda, piv) == 0
+        meta.append((n, lda))
+        lus.append(a)
+        pivs.append(piv.astype(np.int32))
+        outs.append(inv)
+    lu, off = pack(lus)
+    inv, _ = pack(outs)
+    piv, poff = pack(pivs)
+    return {"getri_meta": np.array(meta, dtype=np.int64), "getri_lu": lu, "getri_off": off,
+            "getri_piv": piv.astype(np.int32), "getri_piv_off": poff, "getri_inv": inv}
+
+
 def main():
     oracle.build()
     rng = np.random.default_rng(20240610)
@@ -196,6 +220,7 @@ def main():
     np.savez_compressed(os.path.join(HERE, "trsv.npz"), **trsv_fixture(rng))
     np.savez_compressed(os.path.join(HERE, "gemm.npz"), **gemm_fixture(rng, False), **gemm_fixture(rng, True))
     np.savez_compressed(os.path.join(HERE, "gemv.npz"), **gemv_fixture(rng))
+    np.savez_compressed(os.path.join(HERE, "getri.npz"), **getri_fixture(rng))
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)), "bytes")

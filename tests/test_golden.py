@@ -3,7 +3,7 @@ written by tests/golden/make_golden.py): a change to the C restatement of
 MatFlat that alters any output bit fails here.  The fixtures' shapes follow
 the reference's tests (LuTests.cs:103-122, CholeskyTests.cs,
 BlasTests.SolveTriangular*.cs, BlasTests.MulMatMat*.cs,
-BlasTests.MulMatVecDouble.cs)."""
+BlasTests.MulMatVecDouble.cs, LuInverseTests.cs)."""
 import os
 
 import numpy as np
@@ -80,3 +80,12 @@ def test_gemv_fixtures():
         y = g["gemv_y_in"][y0:y0 + yl * incy].copy()
         assert oracle.dgemv(int(trans), int(m), int(n), a, int(lda), x, int(incx), y, int(incy)) == 0
         assert_bits_equal(y, g["gemv_y_out"][y0:y0 + yl * incy], f"gemv {i}")
+
+
+def test_getri_fixtures():
+    g = load("getri.npz")
+    for i, (n, lda) in enumerate(g["getri_meta"]):
+        o, po = g["getri_off"][i], g["getri_piv_off"][i]
+        a = g["getri_lu"][o:o + lda * n].copy()
+        assert oracle.dgetri(int(n), a, int(lda), g["getri_piv"][po:po + n]) == 0
+        assert_bits_equal(a, g["getri_inv"][o:o + lda * n], f"getri {i}")

@@ -260,6 +260,76 @@ def test_getrs_after_getrf():
         assert_bits_equal(gx[i], want, f"getrs n={n}")
 
 
+def test_getri_after_getrf_variable_sizes():
+    """Factorization.Lu then Factorization.LuInverse on the device for 120
+    matrices of order 1..128 (lda = n + 1, NaN padding): the inverse
+    bit-identical to the oracle; a matrix with an exactly zero pivot gets
+    info = j + 1 and keeps its factors; a bad order is rejected alone."""
+    hip = hipmod()
+    rng = np.random.default_rng(31)
+    ns = [int(v) for v in rng.integers(1, 129, size=120)]
+    ns[:3] = [1, 64, 128]
+    bufs = []
+    for i, n in enumerate(ns):
+        b = np.full((n + 1) * n, np.nan)
+        col = np.random.default_rng(500 + i)
+        for j in range(n):
+            b[j * (n + 1):j * (n + 1) + n] = col.standard_normal(n)
+        bufs.append(b)
+    sing = 5  # a zero last column makes U[n-1, n-1] exactly zero
+    n5 = ns[sing]
+    bufs[sing][(n5 - 1) * (n5 + 1):(n5 - 1) * (n5 + 1) + n5] = 0.0
+    ldas = [n + 1 for n in ns]
+    A = Packed(bufs)
+    P = Packed([np.zeros(n, dtype=np.int32) for n in ns], dtype=torch.int32)
+    sign = i32(np.zeros(len(ns)))
+    info = i32(np.full(len(ns), 9))
+    hip.dgetrf_vbatched(i32(ns), i32(ns), A.ptrs, i32(ldas), P.ptrs, sign, info, 128, 128)
+    torch.cuda.synchronize()
+    lus = A.all()
+    nbad = list(ns)
+    nbad[7] = 0
+    info2 = i32(np.full(len(ns), 9))
+    hip.dgetri_vbatched(i32(nbad), A.ptrs, i32(ldas), P.ptrs, info2, 128)
+    torch.cuda.synchronize()
+    got, gi = A.all(), info2.cpu().numpy()
+    gp = P.all()
+    for i, n in enumerate(ns):
+        lu = bufs[i].copy()
+        piv, _ = oracle.dgetrf(n, n, lu, n + 1)
+        assert np.array_equal(gp[i], piv)
+        assert_bits_equal(lus[i], lu, f"getrf n={n}")
+        if i == 7:
+            assert gi[i] == -3
+            assert_bits_equal(got[i], lu, "rejected matrix unchanged")
+            continue
+        want = lu.copy()
+        r = oracle.dgetri(n, want, n + 1, piv)
+        assert gi[i] == r, (i, n, gi[i], r)
+        assert_bits_equal(got[i], want, f"getri n={n}")
+    assert gi[sing] == n5
+
+
+def test_matflat_luinverse_on_device():
+    from matflat_amd.matflat import ArgumentException, Factorization, MatrixFactorizationException
+    n, lda = 10, 17
+    a = rand_buf(42, n, n, lda)
+    dev = torch.from_numpy(a.copy()).cuda()
+    piv = torch.zeros(n, dtype=torch.int32, device="cuda")
+    Factorization.Lu(n, n, dev, lda, piv)
+    Factorization.LuInverse(n, dev, lda, piv)
+    want = a.copy()
+    wpiv, _ = oracle.dgetrf(n, n, want, lda)
+    oracle.dgetri(n, want, lda, wpiv)
+    assert_bits_equal(dev.cpu().numpy(), want, "LuInverse")
+    z = torch.zeros(4, dtype=torch.float64, device="cuda")
+    zp = torch.zeros(2, dtype=torch.int32, device="cuda")
+    with pytest.raises(MatrixFactorizationException):
+        Factorization.LuInverse(2, z, 2, zp)
+    with pytest.raises(ArgumentException):
+        Factorization.LuInverse(3, dev, 2, piv)
+
+
 # ------------------------------------------------------------------ GEMM
 def gemm_batch(complex_: bool, seed: int, count: int, shapes=None):
     rng = np.random.default_rng(seed)

@@ -2,7 +2,7 @@
 (tests/golden/*.npz, written by tests/golden/make_golden.py from the
 oracle): LU factors, pivots and signs, Cholesky factors, SPD-solve x, the
 eight triangular-solve forms, GEMM (real and complex, every transpose /
-conjugate combination) and GEMV must match bit for bit.  Shapes follow the
+conjugate combination), GEMV and the LU inverse must match bit for bit.  Shapes follow the
 reference's tests (LuTests.cs:103-122, CholeskyTests.cs,
 BlasTests.SolveTriangular*.cs, BlasTests.MulMatMat*.cs)."""
 import os
@@ -159,3 +159,24 @@ def test_golden_gemv():
     for i in range(len(meta)):
         o = g["gemv_y_off"][i]
         assert_bits_equal(gy[i], g["gemv_y_out"][o:o + yl[i] * incy[i]], f"gemv {i} {tuple(meta[i])}")
+
+
+def test_golden_getri():
+    """Factorization.LuInverse (double) from the stored factors and pivots:
+    the inverse bit-identical, NaN padding rows untouched."""
+    hip = hipmod()
+    g = load("getri.npz")
+    meta = g["getri_meta"]
+    ns, ldas = meta[:, 0], meta[:, 1]
+    A = Packed([g["getri_lu"][o:o + l * n].copy() for o, n, l in zip(g["getri_off"], ns, ldas)])
+    P = Packed([g["getri_piv"][o:o + n] for o, n in zip(g["getri_piv_off"], ns)], dtype=torch.int32)
+    info = i32(np.full(len(meta), 99))
+    h = exact_handle()
+    hip.dgetri_vbatched(i32(ns), A.ptrs, i32(ldas), P.ptrs, info, int(ns.max()), handle=h)
+    torch.cuda.synchronize()
+    h.close()
+    assert np.all(info.cpu().numpy() == 0)
+    ga = A.all()
+    for i in range(len(meta)):
+        o = g["getri_off"][i]
+        assert_bits_equal(ga[i], g["getri_inv"][o:o + ldas[i] * ns[i]], f"getri {i} {tuple(meta[i])}")

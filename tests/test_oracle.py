@@ -274,6 +274,65 @@ def test_dgemm_matches_numpy(ta, tb, m, n, k, lda0, ldb0, ldc):
     np.testing.assert_allclose(view(c2, m, n, ldc), -0.75 * expect + 0.5 * view(c0, m, n, ldc), atol=1e-12)
 
 
+# ---------------------------------------------------------------- LU inverse
+# (n, lda) of LuInverseTests.cs:75-84, plus larger orders
+GETRI_CASES = [(1, 1), (1, 3), (2, 2), (2, 3), (3, 3), (3, 5), (5, 5), (5, 8), (10, 10), (10, 17), (33, 40)]
+
+
+def _lu_of(seed, n, lda):
+    a = rand_buf(seed, n, n, lda)
+    a0 = a.copy()
+    piv, _ = oracle.dgetrf(n, n, a, lda)
+    return a0, a, piv
+
+
+@pytest.mark.parametrize("n,lda", GETRI_CASES)
+def test_dgetri_times_a_is_identity(n, lda):
+    """LuInverseTests.cs:85-136: A * inv(A) = I within 1e-12, padding rows
+    untouched; also against LAPACK's inverse."""
+    a0, a, piv = _lu_of(42, n, lda)
+    assert oracle.dgetri(n, a, lda, piv) == 0
+    inv = view(a, n, n, lda)
+    np.testing.assert_allclose(view(a0, n, n, lda) @ inv, np.eye(n), rtol=0, atol=1e-12)
+    np.testing.assert_allclose(inv, np.linalg.inv(view(a0, n, n, lda)), rtol=1e-10, atol=1e-12)
+    for j in range(n):
+        assert np.array_equal(a[j * lda + n:(j + 1) * lda], a0[j * lda + n:(j + 1) * lda], equal_nan=True)
+
+
+@pytest.mark.parametrize("n,lda", [(1, 1), (5, 8), (10, 17), (33, 40), (64, 64)])
+def test_dgetri_column_is_lu_solve_of_unit_vector(n, lda):
+    """The identity the batched inverse rests on: column j of LuInverse's
+    result is, bit for bit, the LU-solve composition (LuInverse.cs:169-192
+    for one right-hand side) applied to e_j."""
+    _, a, piv = _lu_of(7, n, lda)
+    lu = a.copy()
+    assert oracle.dgetri(n, a, lda, piv) == 0
+    for j in range(n):
+        e = np.zeros(n)
+        e[j] = 1.0
+        x = oracle.dgetrs(n, lu, lda, piv, e)
+        assert np.array_equal(x.view(np.uint64), a[j * lda:j * lda + n].view(np.uint64)), j
+
+
+def test_dgetri_permutation_and_zero_diagonal():
+    """A permutation matrix factors exactly (L = I, U = I), so its inverse is
+    its transpose exactly; a zero diagonal entry of U returns j + 1 and
+    leaves the input unchanged (the reference throws before writing)."""
+    n = 6
+    perm = np.array([3, 0, 5, 1, 2, 4])
+    p = np.zeros((n, n))
+    p[np.arange(n), perm] = 1.0
+    a = p.T.reshape(-1).copy()  # column-major
+    piv, _ = oracle.dgetrf(n, n, a, n)
+    assert oracle.dgetri(n, a, n, piv) == 0
+    assert np.array_equal(view(a, n, n, n), p.T)
+    _, a, piv = _lu_of(3, 5, 5)
+    a[2 * 5 + 2] = 0.0
+    before = a.copy()
+    assert oracle.dgetri(5, a, 5, piv) == 3
+    assert np.array_equal(a, before)
+
+
 # ---------------------------------------------------------------- GEMV
 # (m, n, lda, incx, incy) of BlasTests.MulMatVecDouble.cs:9-22 and :56-63
 GEMV_CASES = [(1, 1, 1, 1, 1), (1, 1, 3, 2, 4), (2, 2, 4, 3, 3), (3, 3, 5, 6, 7), (1, 5, 3, 2, 4),
