@@ -580,6 +580,8 @@ def main() -> int:
     ap.add_argument("--only", choices=["mixed", "legs"], default=None,
                     help="run only the mixed-precision comparison or only the other-configuration legs and print "
                          "their JSON (for profiling them alone); the default runs everything")
+    ap.add_argument("--bucket-streams", type=int, default=0,
+                    help="streams the FAST size-bucket launches are forked over (0 = the library default)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 256 MiB device copy (reference for HBM byte counters)")
     args = ap.parse_args()
@@ -615,6 +617,8 @@ def main() -> int:
     stream = torch.cuda.current_stream()
     handle = hip.Handle()
     handle.bind_stream(stream)
+    if args.bucket_streams:
+        handle.set_bucket_streams(args.bucket_streams)
 
     def run_mode(mode: int, steps: int, warmup: int):
         handle.set_math_mode(mode)

@@ -91,6 +91,16 @@ int mfh_get_math_mode(mfh_handle* handle, int* mode);
 int mfh_set_fast_waves(mfh_handle* handle, int row_blocks, int waves);
 int mfh_get_fast_waves(mfh_handle* handle, int row_blocks, int* waves);
 
+/* Concurrency of the FAST Cholesky / SPD-solve size buckets (no MatFlat
+ * counterpart): the per-bucket launches are forked over `streams` (1..4,
+ * default 2) streams of default priority that the handle owns, largest
+ * bucket first, and joined back into the handle's stream, so a bucket's
+ * last partial round overlaps the next bucket.  1 keeps them in order on
+ * the handle's stream.  With per-launch timing on (mfh_set_timing) the
+ * launches always run in order, so each is timed alone. */
+int mfh_set_bucket_streams(mfh_handle* handle, int streams);
+int mfh_get_bucket_streams(mfh_handle* handle, int* streams);
+
 /* Per-launch kernel timing (a profiling aid; no MatFlat counterpart).  With
  * timing enabled, every kernel launch of the factorization / solve entry
  * points on this handle (outside plan capture) is bracketed by HIP events on

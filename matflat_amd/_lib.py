@@ -45,6 +45,8 @@ SIGNATURES = {
     "mfh_get_math_mode": [_p, _p],
     "mfh_set_fast_waves": [_p, _i, _i],
     "mfh_get_fast_waves": [_p, _i, _p],
+    "mfh_set_bucket_streams": [_p, _i],
+    "mfh_get_bucket_streams": [_p, _p],
     "mfh_set_timing": [_p, _i],
     "mfh_get_timing": [_p, _i, _p, _p, _p, _p],
     "mfh_dgetrf_vbatched": [_p, _i, _p, _p, _p, _p, _p, _p, _p, _i, _i],

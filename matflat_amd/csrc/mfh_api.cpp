@@ -56,6 +56,21 @@ int mfh_aux_reserve(mfh_handle* h, size_t bytes)
     return MFH_OK;
 }
 
+int mfh_side_streams(mfh_handle* h, int k)
+{
+    while ((int)h->side.size() < k) {
+        hipStream_t s;
+        MFH_HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        h->side.push_back(s);
+    }
+    while ((int)h->side_ev.size() < k + 1) {
+        hipEvent_t ev;
+        MFH_HIP_CHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        h->side_ev.push_back(ev);
+    }
+    return MFH_OK;
+}
+
 namespace {
 int take_event(mfh_handle* h, hipEvent_t* ev)
 {
@@ -130,6 +145,11 @@ int mfh_destroy(mfh_handle* h)
         hipEventDestroy(t.stop);
     }
     for (auto ev : h->spare_events) hipEventDestroy(ev);
+    for (auto s : h->side) {
+        hipStreamSynchronize(s);
+        hipStreamDestroy(s);
+    }
+    for (auto ev : h->side_ev) hipEventDestroy(ev);
     delete h;
     return MFH_OK;
 }
@@ -156,6 +176,22 @@ int mfh_set_fast_waves(mfh_handle* h, int row_blocks, int waves)
     MFH_REQUIRE(waves >= 0 && waves <= 2 && !(waves == 2 && row_blocks < 2), 3,
                 "waves must be 0 (default), 1 or 2 (2 needs row_blocks >= 2)");
     h->fast_waves[row_blocks] = waves;
+    return MFH_OK;
+}
+
+int mfh_set_bucket_streams(mfh_handle* h, int streams)
+{
+    MFH_REQUIRE(h != nullptr, 1, "handle must not be null");
+    MFH_REQUIRE(streams >= 1 && streams <= 4, 2, "streams must be in [1, 4]");
+    h->bucket_streams = streams;
+    return MFH_OK;
+}
+
+int mfh_get_bucket_streams(mfh_handle* h, int* streams)
+{
+    MFH_REQUIRE(h != nullptr, 1, "handle must not be null");
+    MFH_REQUIRE(streams != nullptr, 2, "streams must not be null");
+    *streams = h->bucket_streams;
     return MFH_OK;
 }
 
@@ -503,6 +539,9 @@ int mfh_plan_create_dposv(mfh_handle* h, int count, const int* n, double* const*
     // null stream, which cannot be captured) with the handle temporarily
     // pointing at the plan's workspace.
     const size_t ws_bytes = mfh_exact_ws_bytes(count);
+    // side streams of the bucket fork exist before the capture starts
+    int sst = mfh_side_streams(h, h->bucket_streams);
+    if (sst) return sst;
     void* pws = nullptr;
     MFH_HIP_CHECK(hipMalloc(&pws, ws_bytes));
     hipStream_t cap;

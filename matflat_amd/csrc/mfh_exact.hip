@@ -878,9 +878,34 @@ int mfh_launch_potrf(mfh_handle* h, int count, const MfhCholBatch& B, int max_n,
     const int key = posv ? n_cap + 1 : n_cap;
     if (h->math_mode == MFH_MATH_FAST) {
         // fine bucket f holds row counts (8f, 8f + 8]: register tile of f + 1 row blocks
-        for (int f = 0; f < (key + MFH_FINE_STEP - 1) / MFH_FINE_STEP; ++f) {
+        const int nf = (key + MFH_FINE_STEP - 1) / MFH_FINE_STEP;
+        const int k = h->timing ? 1 : (h->bucket_streams < nf ? h->bucket_streams : nf);
+        if (k <= 1) {
+            for (int f = 0; f < nf; ++f) {
+                st = mfh_launch_chol_fast_bucket(h, B, fine_queue(ws, f), f + 1, count, posv);
+                if (st) return st;
+            }
+            return MFH_OK;
+        }
+        // The buckets are independent persistent launches, each ending in a
+        // partial last round of matrices.  Forked over k streams (largest
+        // bucket first, round robin) the next bucket fills the CUs that a
+        // finishing one leaves idle; the handle's stream waits for all of
+        // them.  Under graph capture the fork and join become graph edges.
+        st = mfh_side_streams(h, k);
+        if (st) return st;
+        const hipStream_t main = h->stream;
+        MFH_HIP_CHECK(hipEventRecord(h->side_ev[k], main));
+        for (int s = 0; s < k; ++s) MFH_HIP_CHECK(hipStreamWaitEvent(h->side[s], h->side_ev[k], 0));
+        for (int f = nf - 1, i = 0; f >= 0; --f, ++i) {
+            h->stream = h->side[i % k];
             st = mfh_launch_chol_fast_bucket(h, B, fine_queue(ws, f), f + 1, count, posv);
+            h->stream = main;
             if (st) return st;
+        }
+        for (int s = 0; s < k; ++s) {
+            MFH_HIP_CHECK(hipEventRecord(h->side_ev[s], h->side[s]));
+            MFH_HIP_CHECK(hipStreamWaitEvent(main, h->side_ev[s], 0));
         }
         return MFH_OK;
     }

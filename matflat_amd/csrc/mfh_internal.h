@@ -39,6 +39,9 @@ struct mfh_handle {
     int math_mode = MFH_MATH_EXACT;
     int fast_waves[18] = {};              // FAST Cholesky waves per matrix by row blocks (0 = default)
     bool timing = false;                  // bracket launches with events
+    int bucket_streams = 2;               // FAST bucket launches forked over this many streams (1 = in order)
+    std::vector<hipStream_t> side;        // side streams of the fork (default priority), created on first use
+    std::vector<hipEvent_t> side_ev;      // fork / join events, one per side stream plus one
     std::vector<MfhTimedLaunch> timed;    // recorded launches not yet read
     std::vector<hipEvent_t> spare_events; // reusable events
 };
@@ -49,6 +52,10 @@ int mfh_fail(int status, const std::string& msg);
 
 // Grows the handle workspace to at least bytes (device memory).
 int mfh_ws_reserve(mfh_handle* h, size_t bytes);
+// Side streams for forking independent launches off the handle's stream
+// (created once, default priority); fills at least k of them.
+int mfh_side_streams(mfh_handle* h, int k);
+
 // Grows the handle's second buffer (aux) to at least bytes.
 int mfh_aux_reserve(mfh_handle* h, size_t bytes);
 

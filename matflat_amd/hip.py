@@ -88,6 +88,15 @@ class Handle:
         """FAST Cholesky: wavefronts per matrix (1, 2; 0 = built-in) for up to 8*row_blocks rows."""
         _check(self.lib.mfh_set_fast_waves(self.h, int(row_blocks), int(waves)))
 
+    def set_bucket_streams(self, streams: int) -> None:
+        """FAST bucket launches forked over this many streams (1 = in order)."""
+        _check(self.lib.mfh_set_bucket_streams(self.h, int(streams)))
+
+    def bucket_streams(self) -> int:
+        s = ctypes.c_int()
+        _check(self.lib.mfh_get_bucket_streams(self.h, ctypes.byref(s)))
+        return s.value
+
     def fast_waves(self, row_blocks: int) -> int:
         w = ctypes.c_int()
         _check(self.lib.mfh_get_fast_waves(self.h, int(row_blocks), ctypes.byref(w)))

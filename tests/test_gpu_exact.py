@@ -651,6 +651,69 @@ def test_fast_mode_cholesky_residuals(posv, waves):
     h.close()
 
 
+@pytest.mark.parametrize("use_plan", [False, True])
+def test_fast_mode_bucket_streams_same_bits(use_plan):
+    """The FAST size buckets forked over 1..4 streams (eagerly, and inside a
+    captured plan replayed several times with eager work in between) give
+    the same bits: the fork only changes when each bucket runs.  The
+    one-stream result is also checked against the oracle by residual."""
+    from helpers import backward_error, view
+    hip = hipmod()
+    ns, ldas = chol_cases(41, 600, nmax=100)
+    ns[:4] = [1, 9, 99, 100]
+    ldas[:4] = [1, 16, 104, 100]
+    bufs = [spd_upper_buf(2100 + i, n, l) for i, (n, l) in enumerate(zip(ns, ldas))]
+    xs = [np.random.default_rng(2900 + i).standard_normal(n) for i, n in enumerate(ns)]
+    A0, X0 = Packed(bufs), Packed(xs)
+    ref = None
+    for streams in (1, 2, 3, 4):
+        h = hip.Handle()
+        h.bind_stream()
+        h.set_math_mode(hip.MATH_FAST)
+        h.set_bucket_streams(streams)
+        assert h.bucket_streams() == streams
+        A, X = Packed(bufs), Packed(xs)
+        info = i32(np.full(len(ns), 99))
+        if use_plan:
+            plan = hip.DposvPlan(i32(ns), A.ptrs, i32(ldas), X.ptrs, info, max(ns), handle=h)
+            reps = 3
+        else:
+            reps = 1
+        for rep in range(reps):
+            A.dev.copy_(A0.dev)
+            X.dev.copy_(X0.dev)
+            info.fill_(99)
+            torch.cuda.synchronize()
+            if use_plan:
+                plan.execute()
+            else:
+                hip.dposv_vbatched(i32(ns), A.ptrs, i32(ldas), X.ptrs, info, max(ns), handle=h)
+            torch.cuda.synchronize()
+            _ = (A.dev.sum() + X.dev.sum()).item()  # eager work between replays
+            assert np.all(info.cpu().numpy() == 0), (streams, rep)
+            got = (A.all(), X.all())
+            if ref is None:
+                ref = got
+                eps = np.finfo(np.float64).eps
+                for i, (n, l) in enumerate(zip(ns, ldas)):
+                    full = view(bufs[i], n, n, l)
+                    a_sym = np.triu(full) + np.triu(full, 1).T
+                    assert backward_error(a_sym, got[1][i], xs[i]) <= n * eps, (i, n)
+            for i in range(len(ns)):
+                assert_bits_equal(got[0][i], ref[0][i], f"streams {streams} rep {rep} A {i}")
+                assert_bits_equal(got[1][i], ref[1][i], f"streams {streams} rep {rep} x {i}")
+        if use_plan:
+            plan.close()
+        h.close()
+    h = hip.Handle()
+    from matflat_amd.hip import ArgumentError
+    with pytest.raises(ArgumentError):
+        h.set_bucket_streams(0)
+    with pytest.raises(ArgumentError):
+        h.set_bucket_streams(5)
+    h.close()
+
+
 @pytest.mark.parametrize("waves", [1, 2])
 def test_fast_mode_not_positive_definite(waves):
     """FAST mode reports the failing pivot like MatFlat (info = j+1 for a
