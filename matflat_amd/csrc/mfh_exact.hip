@@ -869,6 +869,42 @@ int mfh_launch_getrf(mfh_handle* h, int count, const MfhGetrfBatch& B, int max_m
     return MFH_OK;
 }
 
+// Runs launch(i) for the nb size buckets i = nb-1 .. 0.  The buckets are
+// independent persistent launches, each ending in a partial last round of
+// matrices.  Forked over h->bucket_streams side streams (largest bucket
+// first, round robin) the next bucket fills the CUs a finishing one leaves
+// idle; the handle's stream waits for all of them.  Under graph capture the
+// fork and join become graph edges.  With per-launch timing on, the
+// launches run in order on the handle's stream so each is timed alone.
+template <typename F>
+int fork_buckets(mfh_handle* h, int nb, F launch)
+{
+    const int k = h->timing ? 1 : (h->bucket_streams < nb ? h->bucket_streams : nb);
+    if (k <= 1) {
+        for (int i = nb - 1; i >= 0; --i) {
+            int st = launch(i);
+            if (st) return st;
+        }
+        return MFH_OK;
+    }
+    int st = mfh_side_streams(h, k);
+    if (st) return st;
+    const hipStream_t main = h->stream;
+    MFH_HIP_CHECK(hipEventRecord(h->side_ev[k], main));
+    for (int s = 0; s < k; ++s) MFH_HIP_CHECK(hipStreamWaitEvent(h->side[s], h->side_ev[k], 0));
+    for (int i = nb - 1, r = 0; i >= 0; --i, ++r) {
+        h->stream = h->side[r % k];
+        st = launch(i);
+        h->stream = main;
+        if (st) return st;
+    }
+    for (int s = 0; s < k; ++s) {
+        MFH_HIP_CHECK(hipEventRecord(h->side_ev[s], h->side[s]));
+        MFH_HIP_CHECK(hipStreamWaitEvent(main, h->side_ev[s], 0));
+    }
+    return MFH_OK;
+}
+
 int mfh_launch_potrf(mfh_handle* h, int count, const MfhCholBatch& B, int max_n, bool posv)
 {
     SortWs ws;
@@ -879,43 +915,15 @@ int mfh_launch_potrf(mfh_handle* h, int count, const MfhCholBatch& B, int max_n,
     if (h->math_mode == MFH_MATH_FAST) {
         // fine bucket f holds row counts (8f, 8f + 8]: register tile of f + 1 row blocks
         const int nf = (key + MFH_FINE_STEP - 1) / MFH_FINE_STEP;
-        const int k = h->timing ? 1 : (h->bucket_streams < nf ? h->bucket_streams : nf);
-        if (k <= 1) {
-            for (int f = 0; f < nf; ++f) {
-                st = mfh_launch_chol_fast_bucket(h, B, fine_queue(ws, f), f + 1, count, posv);
-                if (st) return st;
-            }
-            return MFH_OK;
-        }
-        // The buckets are independent persistent launches, each ending in a
-        // partial last round of matrices.  Forked over k streams (largest
-        // bucket first, round robin) the next bucket fills the CUs that a
-        // finishing one leaves idle; the handle's stream waits for all of
-        // them.  Under graph capture the fork and join become graph edges.
-        st = mfh_side_streams(h, k);
-        if (st) return st;
-        const hipStream_t main = h->stream;
-        MFH_HIP_CHECK(hipEventRecord(h->side_ev[k], main));
-        for (int s = 0; s < k; ++s) MFH_HIP_CHECK(hipStreamWaitEvent(h->side[s], h->side_ev[k], 0));
-        for (int f = nf - 1, i = 0; f >= 0; --f, ++i) {
-            h->stream = h->side[i % k];
-            st = mfh_launch_chol_fast_bucket(h, B, fine_queue(ws, f), f + 1, count, posv);
-            h->stream = main;
-            if (st) return st;
-        }
-        for (int s = 0; s < k; ++s) {
-            MFH_HIP_CHECK(hipEventRecord(h->side_ev[s], h->side[s]));
-            MFH_HIP_CHECK(hipStreamWaitEvent(main, h->side_ev[s], 0));
-        }
-        return MFH_OK;
+        return fork_buckets(h, nf, [&](int f) {
+            return mfh_launch_chol_fast_bucket(h, B, fine_queue(ws, f), f + 1, count, posv);
+        });
     }
-    for (int b = 0; b < nbuckets_for(key); ++b) {
+    return fork_buckets(h, nbuckets_for(key), [&](int b) {
         const int nb_rows = bucket_bound(b, key) < 2 ? 2 : bucket_bound(b, key);
-        st = posv ? run_chol_bucket<true>(h, b, nb_rows, count, B, ws)
-                  : run_chol_bucket<false>(h, b, nb_rows, count, B, ws);
-        if (st) return st;
-    }
-    return MFH_OK;
+        return posv ? run_chol_bucket<true>(h, b, nb_rows, count, B, ws)
+                    : run_chol_bucket<false>(h, b, nb_rows, count, B, ws);
+    });
 }
 
 int mfh_launch_trsv(mfh_handle* h, int count, const MfhTrsvBatch& B, int max_n)
