@@ -91,7 +91,7 @@ int mfh_get_math_mode(mfh_handle* handle, int* mode);
 int mfh_set_fast_waves(mfh_handle* handle, int row_blocks, int waves);
 int mfh_get_fast_waves(mfh_handle* handle, int row_blocks, int* waves);
 
-/* Concurrency of the FAST Cholesky / SPD-solve size buckets (no MatFlat
+/* Concurrency of the FAST-mode Cholesky / SPD-solve size buckets (no MatFlat
  * counterpart): the per-bucket launches are forked over `streams` (1..4,
  * default 2) streams of default priority that the handle owns, largest
  * bucket first, and joined back into the handle's stream, so a bucket's
@@ -121,6 +121,7 @@ int mfh_get_bucket_streams(mfh_handle* handle, int* streams);
 #define MFH_KERNEL_GESV_STEP 9
 #define MFH_KERNEL_GEMV_EXACT 10
 #define MFH_KERNEL_GETRI_SETUP 11
+#define MFH_KERNEL_GETRI_EXACT 12
 int mfh_set_timing(mfh_handle* handle, int enable);
 int mfh_get_timing(mfh_handle* handle, int capacity, int* n_records, int* kind, int* rows,
                    float* ms);

@@ -27,9 +27,11 @@ BLOCKED_MAX_N = 512
 KERNEL_SORT, KERNEL_CHOL_EXACT, KERNEL_CHOL_FAST, KERNEL_GETRF_EXACT, KERNEL_TRSV_EXACT, KERNEL_GETRS_EXACT = \
     1, 2, 3, 4, 5, 6
 KERNEL_GESV_MIXED, KERNEL_GESV_F64, KERNEL_GESV_STEP, KERNEL_GEMV_EXACT, KERNEL_GETRI_SETUP = 7, 8, 9, 10, 11
+KERNEL_GETRI_EXACT = 12
 KERNEL_NAMES = {1: "size sort", 2: "k_chol_packed", 3: "k_chol_fast", 4: "k_getrf_exact", 5: "k_trsv_exact",
                 6: "k_getrs_exact", 7: "k_gesv<float, mixed>", 8: "k_gesv<double>", 9: "k_gesv_step",
-                10: "k_dgemv_exact", 11: "k_getri_setup"}
+                10: "k_dgemv_exact", 11: "k_getri_setup",
+                12: "k_getri_exact"}
 
 _i, _d, _p, _i64 = ctypes.c_int, ctypes.c_double, ctypes.c_void_p, ctypes.c_int64
 

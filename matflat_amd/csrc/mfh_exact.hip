@@ -919,11 +919,16 @@ int mfh_launch_potrf(mfh_handle* h, int count, const MfhCholBatch& B, int max_n,
             return mfh_launch_chol_fast_bucket(h, B, fine_queue(ws, f), f + 1, count, posv);
         });
     }
-    return fork_buckets(h, nbuckets_for(key), [&](int b) {
+    // The exact buckets (16 rows wide) stay in order: one bucket usually
+    // dominates, and measured on MI355X the fork only added its waits
+    // (n = 32: 0.651 -> 0.674 ms; n ~ U{10..100}: 11.49 -> 11.48 ms).
+    for (int b = 0; b < nbuckets_for(key); ++b) {
         const int nb_rows = bucket_bound(b, key) < 2 ? 2 : bucket_bound(b, key);
-        return posv ? run_chol_bucket<true>(h, b, nb_rows, count, B, ws)
-                    : run_chol_bucket<false>(h, b, nb_rows, count, B, ws);
-    });
+        st = posv ? run_chol_bucket<true>(h, b, nb_rows, count, B, ws)
+                  : run_chol_bucket<false>(h, b, nb_rows, count, B, ws);
+        if (st) return st;
+    }
+    return MFH_OK;
 }
 
 int mfh_launch_trsv(mfh_handle* h, int count, const MfhTrsvBatch& B, int max_n)
@@ -966,8 +971,9 @@ int mfh_launch_getrs(mfh_handle* h, int count, const MfhGetrsBatch& B, int max_n
 // Factorization.LuInverse.cs:105-192 sets X = P (X[j, piv[j]] = 1) and runs
 // the unit-lower and the upper solve of :156-174 on every column of X, so
 // column j of A^-1 is exactly the LU-solve composition of k_getrs_exact
-// applied to b = e_j.  The inverse is therefore max_n solves per matrix
-// through the same kernel: k_getri_setup checks each matrix (the first zero
+// applied to b = e_j.  Up to n = 64 k_getri_exact (below) runs all columns
+// of a matrix in one workgroup.  Larger orders run max_n solves per matrix
+// through k_getrs_exact itself: k_getri_setup checks each matrix (the first zero
 // diagonal entry of U gives info = j + 1 and leaves the matrix unchanged,
 // as the reference throws before writing), copies its factors aside (A is
 // overwritten column by column) and writes the solve list; columns j >= n
@@ -1027,11 +1033,180 @@ __global__ __launch_bounds__(256) void k_getri_setup(MfhGetriBatch B, int count,
     if (tid == 0) B.info[i] = bad;
 }
 
+// Factorization.LuInverse.cs:105-192 for n <= 64, one workgroup of four
+// wavefronts per matrix: the factors are staged in LDS once, and all n
+// column solves run together.  Lane i owns row i; wavefront q owns columns
+// q, q + 4, ...  For every (row, column) the arithmetic is exactly that of
+// k_getrs_exact with b = e_j: y = P e_j, the unit-lower dot-form sweep
+// folded term by term in MatFlat's pairing, then the Upper/NoTrans axpy
+// sweep.  Finished entries of X go through LDS (broadcast reads within a
+// wavefront); each step ends in one barrier.
+constexpr int GETRI_T = 256;
+constexpr int GETRI_MAXN = 64;
+constexpr int GETRI_COLS = GETRI_T / MFH_WAVE;     // column stride between a wavefront's columns
+constexpr int GETRI_JPER = GETRI_MAXN / GETRI_COLS;  // columns per wavefront
+
+size_t getri_lds_bytes(int max_n) { return 2 * (size_t)max_n * (max_n | 1) * 8; }
+
+__global__ __launch_bounds__(GETRI_T) void k_getri_exact(MfhGetriBatch B, int count, int max_n)
+{
+    extern __shared__ double lds[];
+    __shared__ int s_zero;
+    const int tid = threadIdx.x;
+    const int i = tid & (MFH_WAVE - 1);
+    const int q = tid / MFH_WAVE;
+    for (int b = blockIdx.x; b < count; b += gridDim.x) {
+        const int n = B.n[b];
+        const int lda = B.lda[b];
+        double* a = B.a[b];
+        const int* piv = B.piv[b];
+        int bad = 0;
+        if (n < 1 || n > max_n) bad = -3;
+        else if (a == nullptr) bad = -4;
+        else if (lda < n || (long long)lda * n > MFH_MAX_MATRIX_ELEMS) bad = -5;
+        else if (piv == nullptr) bad = -6;
+        if (tid == 0) s_zero = INT_MAX;
+        __syncthreads();
+        if (!bad)
+            for (int j = tid; j < n; j += GETRI_T)
+                if (a[j + (size_t)j * lda] == 0.0) atomicMin(&s_zero, j);
+        __syncthreads();
+        if (!bad && s_zero != INT_MAX) bad = s_zero + 1;
+        if (tid == 0) B.info[b] = bad;
+        if (bad) {
+            __syncthreads();
+            continue;
+        }
+        const int nl = n | 1;
+        double* S = lds;
+        double* X = lds + n * nl;
+        for (int e = tid; e < n * n; e += GETRI_T) {
+            const int j = e / n;
+            const int r = e - j * n;
+            S[r + j * nl] = a[r + (size_t)j * lda];
+        }
+        const int pr = i < n ? piv[i] : 0;
+        const bool pok = pr >= 0 && pr < n;
+        double xr[GETRI_JPER], s[GETRI_JPER], pend[GETRI_JPER];
+#pragma unroll
+        for (int jj = 0; jj < GETRI_JPER; ++jj) {
+            const int j = q + GETRI_COLS * jj;
+            xr[jj] = pok ? (pr == j ? 1.0 : 0.0) : __builtin_nan("");
+            s[jj] = 0.0;
+            pend[jj] = 0.0;
+        }
+        if (i == 0) {
+#pragma unroll
+            for (int jj = 0; jj < GETRI_JPER; ++jj) {
+                const int j = q + GETRI_COLS * jj;
+                if (j < n) {
+                    xr[jj] = xr[jj] - 0.0;
+                    X[j * nl] = xr[jj];
+                }
+            }
+        }
+        __syncthreads();
+        const bool odd = i & 1;
+        for (int c = 0; c < n - 1; ++c) {
+            // dot_fold's cases as selects (c is uniform, the row parity is
+            // not): s = p (odd row, c = 0), pend = p, or s = s + (pend + p)
+            const bool act = i > c && i < n;
+            const bool codd = c & 1;
+            const bool add = act && (odd ? (c != 0 && !codd) : codd);
+            const bool setp = act && (odd ? codd : !codd);
+            const bool sets = act && odd && c == 0;
+            const double l = S[act ? i + c * nl : 0];  // rows outside the matrix read a valid slot, unused
+            double xc[GETRI_JPER];
+#pragma unroll
+            for (int jj = 0; jj < GETRI_JPER; ++jj) {
+                const int j = q + GETRI_COLS * jj;
+                xc[jj] = X[c + (j < n ? j : 0) * nl];
+            }
+#pragma unroll
+            for (int jj = 0; jj < GETRI_JPER; ++jj) {
+                const double p = l * xc[jj];
+                const double t = s[jj] + (pend[jj] + p);
+                s[jj] = add ? t : (sets ? p : s[jj]);
+                pend[jj] = setp ? p : pend[jj];
+            }
+            if (i == c + 1) {
+#pragma unroll
+                for (int jj = 0; jj < GETRI_JPER; ++jj) {
+                    const int j = q + GETRI_COLS * jj;
+                    xr[jj] = xr[jj] - s[jj];
+                    if (j < n) X[i + j * nl] = xr[jj];
+                }
+            }
+            __syncthreads();
+        }
+        if (i == n - 1) {
+            const double d = S[(n - 1) * (nl + 1)];
+#pragma unroll
+            for (int jj = 0; jj < GETRI_JPER; ++jj) {
+                const int j = q + GETRI_COLS * jj;
+                if (j < n) {
+                    xr[jj] = xr[jj] / d;
+                    X[n - 1 + j * nl] = xr[jj];
+                }
+            }
+        }
+        __syncthreads();
+        for (int jc = n - 1; jc >= 1; --jc) {
+            const bool act = i < jc;
+            const double u = S[act ? i + jc * nl : 0];
+            double xc[GETRI_JPER];
+#pragma unroll
+            for (int jj = 0; jj < GETRI_JPER; ++jj) {
+                const int j = q + GETRI_COLS * jj;
+                xc[jj] = X[jc + (j < n ? j : 0) * nl];
+            }
+#pragma unroll
+            for (int jj = 0; jj < GETRI_JPER; ++jj) {
+                const double t = xr[jj] - u * xc[jj];
+                xr[jj] = act ? t : xr[jj];
+            }
+            if (i == jc - 1) {
+                const double d = S[i + i * nl];
+#pragma unroll
+                for (int jj = 0; jj < GETRI_JPER; ++jj) {
+                    const int j = q + GETRI_COLS * jj;
+                    xr[jj] = xr[jj] / d;
+                    if (j < n) X[i + j * nl] = xr[jj];
+                }
+            }
+            __syncthreads();
+        }
+        if (i < n) {
+#pragma unroll
+            for (int jj = 0; jj < GETRI_JPER; ++jj) {
+                const int j = q + GETRI_COLS * jj;
+                if (j < n) a[i + (size_t)j * lda] = xr[jj];
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // Workspace bytes a batch of count matrices needs (plans reserve this).
 size_t mfh_exact_ws_bytes(int count) { return sort_ws_bytes(count); }
 
 int mfh_launch_getri(mfh_handle* h, int count, const MfhGetriBatch& B, int max_n)
 {
+    if (max_n <= GETRI_MAXN) {
+        const size_t lds = getri_lds_bytes(max_n);
+        MFH_HIP_CHECK(hipFuncSetAttribute((const void*)k_getri_exact, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          (int)lds));
+        int per_cu = 0;
+        MFH_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_getri_exact, GETRI_T, lds));
+        if (per_cu < 1) per_cu = 1;
+        long grid = (long)per_cu * h->num_cus;
+        if (grid > count) grid = count;
+        int st = mfh_timing_begin(h, MFH_KERNEL_GETRI_EXACT, max_n);
+        if (st) return st;
+        hipLaunchKernelGGL(k_getri_exact, dim3((unsigned)grid), dim3(GETRI_T), lds, h->stream, B, count, max_n);
+        MFH_HIP_CHECK(hipGetLastError());
+        return mfh_timing_end(h);
+    }
     const size_t solves = (size_t)count * max_n;
     size_t off = 0;
     auto take = [&off](size_t bytes) {

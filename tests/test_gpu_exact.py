@@ -260,15 +260,17 @@ def test_getrs_after_getrf():
         assert_bits_equal(gx[i], want, f"getrs n={n}")
 
 
-def test_getri_after_getrf_variable_sizes():
+@pytest.mark.parametrize("nmax", [64, 128])
+def test_getri_after_getrf_variable_sizes(nmax):
     """Factorization.Lu then Factorization.LuInverse on the device for 120
-    matrices of order 1..128 (lda = n + 1, NaN padding): the inverse
+    matrices of order 1..nmax (lda = n + 1, NaN padding): the inverse
     bit-identical to the oracle; a matrix with an exactly zero pivot gets
-    info = j + 1 and keeps its factors; a bad order is rejected alone."""
+    info = j + 1 and keeps its factors; a bad order is rejected alone.
+    nmax = 64 runs the one-workgroup kernel, 128 the per-column solves."""
     hip = hipmod()
-    rng = np.random.default_rng(31)
-    ns = [int(v) for v in rng.integers(1, 129, size=120)]
-    ns[:3] = [1, 64, 128]
+    rng = np.random.default_rng(31 + nmax)
+    ns = [int(v) for v in rng.integers(1, nmax + 1, size=120)]
+    ns[:3] = [1, 2, nmax]
     bufs = []
     for i, n in enumerate(ns):
         b = np.full((n + 1) * n, np.nan)
@@ -284,13 +286,13 @@ def test_getri_after_getrf_variable_sizes():
     P = Packed([np.zeros(n, dtype=np.int32) for n in ns], dtype=torch.int32)
     sign = i32(np.zeros(len(ns)))
     info = i32(np.full(len(ns), 9))
-    hip.dgetrf_vbatched(i32(ns), i32(ns), A.ptrs, i32(ldas), P.ptrs, sign, info, 128, 128)
+    hip.dgetrf_vbatched(i32(ns), i32(ns), A.ptrs, i32(ldas), P.ptrs, sign, info, nmax, nmax)
     torch.cuda.synchronize()
     lus = A.all()
     nbad = list(ns)
     nbad[7] = 0
     info2 = i32(np.full(len(ns), 9))
-    hip.dgetri_vbatched(i32(nbad), A.ptrs, i32(ldas), P.ptrs, info2, 128)
+    hip.dgetri_vbatched(i32(nbad), A.ptrs, i32(ldas), P.ptrs, info2, nmax)
     torch.cuda.synchronize()
     got, gi = A.all(), info2.cpu().numpy()
     gp = P.all()
