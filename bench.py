@@ -344,6 +344,16 @@ def lu_leg(steps: int, warmup: int, count: int = 1024, n: int = 64, seed: int = 
     ms_inv = timed_calls(lambda: hip.dgetri_vbatched(n_t, a_ptrs, n_t, piv_ptrs, inv_info, n, handle=handle),
                          lambda: a.copy_(lu_src), steps, warmup, stream)
     inv_ok = bool(torch.all(inv_info == 0).item())
+    # LU solve (the LuInverse composition for one right-hand side) with the factors
+    a.copy_(lu_src)
+    bvec = torch.randn(count * n, dtype=torch.float64, device=dev, generator=g)
+    xvec = torch.zeros_like(bvec)
+    b_ptrs = bvec.data_ptr() + 8 * n * idx
+    x_ptrs = xvec.data_ptr() + 8 * n * idx
+    rs_info = torch.zeros(count, dtype=torch.int32, device=dev)
+    ms_rs = timed_calls(lambda: hip.dgetrs_vbatched(n_t, a_ptrs, n_t, piv_ptrs, b_ptrs, x_ptrs, rs_info, n,
+                                                    handle=handle), None, steps, warmup, stream)
+    rs_ok = bool(torch.all(rs_info == 0).item())
     handle.close()
     alg = count * (16.0 * n * n + 4 * n + 4)
     res = {"workload": f"{count} LU factorizations (partial pivoting), n = {n}, lda = {n}, N(0,1), exact mode "
@@ -353,9 +363,14 @@ def lu_leg(steps: int, warmup: int, count: int = 1024, n: int = 64, seed: int = 
            "inverse": {"workload": f"Factorization.LuInverse of the same {count} factored matrices, exact mode "
                                    "(bit-identical)",
                        "ms_per_call": round(ms_inv, 5), "inverses_per_s": round(count / (ms_inv / 1e3), 1),
-                       "all_info_zero": inv_ok, "roofline": hbm_entry(alg, ms_inv)}}
+                       "all_info_zero": inv_ok, "roofline": hbm_entry(alg, ms_inv)},
+           "solve": {"workload": f"LU solve (Factorization.LuInverse.cs:169-192 for one right-hand side) with the same "
+                                 f"{count} factors, exact mode (bit-identical)",
+                     "ms_per_call": round(ms_rs, 5), "solves_per_s": round(count / (ms_rs / 1e3), 1),
+                     "all_info_zero": rs_ok,
+                     "roofline": hbm_entry(count * (8.0 * n * n + 4 * n + 16 * n + 4), ms_rs)}}
     log(f"  LU n={n} x{count}: {ms:.4f} ms/call, {res['factorizations_per_s']:.0f} LU/s, "
-        f"{res['roofline']['achieved']} GB/s; inverse {ms_inv:.4f} ms/call")
+        f"{res['roofline']['achieved']} GB/s; inverse {ms_inv:.4f} ms/call; solve {ms_rs:.4f} ms/call")
     return res
 
 
@@ -482,6 +497,56 @@ def gemm_leg(steps: int, warmup: int, count: int = 200_000, seed: int = 5, compl
     return res
 
 
+def trsv_leg(steps: int, warmup: int, count: int = 100_000, seed: int = 7):
+    """Blas.SolveTriangular in its four non-unit forms over `count` triangles
+    with n ~ U{10..100}, lda = 8 ceil(n / 8), N(0,1) entries and a diagonal
+    of n + |N(0,1)|, unit stride, exact mode (bit-identical).  HBM bound:
+    8 (n (n + 1) / 2 + 2 n) + 4 bytes per solve."""
+    import torch
+
+    from matflat_amd import hip
+
+    rng = np.random.default_rng(seed)
+    ns = rng.integers(10, 101, size=count).astype(np.int64)
+    ldas = (ns + 7) // 8 * 8
+    sizes = ldas * ns
+    offs = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    xoffs = np.concatenate([[0], np.cumsum(ns)[:-1]])
+    dev = torch.device("cuda")
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    A = torch.randn(int(sizes.sum()), dtype=torch.float64, device=dev, generator=g)
+    rep = np.repeat(np.arange(count), ns)
+    jj = np.arange(int(ns.sum())) - np.repeat(xoffs, ns)
+    didx = torch.from_numpy(offs[rep] + jj * (ldas[rep] + 1)).to(dev)
+    A[didx] = A[didx].abs() + torch.from_numpy(ns[rep].astype(np.float64)).to(dev)
+    x0 = torch.randn(int(ns.sum()), dtype=torch.float64, device=dev, generator=g)
+    x = x0.clone()
+    a_ptrs = (A.data_ptr() + 8 * torch.from_numpy(offs).to(dev)).contiguous()
+    x_ptrs = (x.data_ptr() + 8 * torch.from_numpy(xoffs).to(dev)).contiguous()
+    n_t = torch.from_numpy(ns.astype(np.int32)).to(dev)
+    lda_t = torch.from_numpy(ldas.astype(np.int32)).to(dev)
+    inc = torch.ones(count, dtype=torch.int32, device=dev)
+    info = torch.zeros(count, dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream()
+    handle = hip.Handle()
+    handle.bind_stream(stream)
+    alg = float(np.sum(8.0 * (ns * (ns + 1) / 2 + 2 * ns) + 4))
+    res = {"workload": f"{count} triangular solves per form, n ~ U{{10..100}}, lda = 8 ceil(n/8), unit stride, "
+                       "exact mode (MatFlat's order, bit-identical)", "alg_bytes": alg}
+    for name, uplo, trans in (("lower_notrans", hip.LOWER, hip.NOTRANS), ("lower_trans", hip.LOWER, hip.TRANS),
+                              ("upper_notrans", hip.UPPER, hip.NOTRANS), ("upper_trans", hip.UPPER, hip.TRANS)):
+        ms = timed_calls(lambda: hip.dtrsv_vbatched(uplo, trans, hip.NONUNIT, n_t, a_ptrs, lda_t, x_ptrs, inc, info,
+                                                    100, handle=handle), lambda: x.copy_(x0), steps, warmup, stream)
+        res[name] = {"ms_per_call": round(ms, 5), "solves_per_s": round(count / (ms / 1e3), 1),
+                     "all_info_zero": bool(torch.all(info == 0).item()), "roofline": hbm_entry(alg, ms)}
+        log(f"  trsv {name} x{count}: {ms:.4f} ms/call, {res[name]['roofline']['achieved']} GB/s")
+    handle.close()
+    del A, x, x0
+    torch.cuda.empty_cache()
+    return res
+
+
 def gemv_leg(steps: int, warmup: int, count: int = 200_000, seed: int = 6):
     """y = op(A) x (Blas.MulMatVec) over `count` products with m, n iid
     U{8..128}, op uniform over {N, T}, lda padded to a multiple of 4, unit
@@ -537,7 +602,7 @@ def other_legs(steps: int) -> dict:
     return {"lu_n64": lu_leg(steps, 2), "spd_solve_adjoint_n32": adjoint_leg(steps, 2),
             "gemm_fp64": gemm_leg(max(3, steps // 4), 1),
             "gemm_complex128": gemm_leg(max(3, steps // 4), 1, complex_=True),
-            "gemv_fp64": gemv_leg(steps, 2)}
+            "gemv_fp64": gemv_leg(steps, 2), "trsv": trsv_leg(steps, 2)}
 
 
 def cpu_baseline(buf_host, offsets, ns, ldas, xoff, b_host, max_seconds=20.0):

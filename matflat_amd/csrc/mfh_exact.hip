@@ -29,6 +29,40 @@
 
 namespace {
 
+// Copies the m x n column-major block at src (leading dimension lds_, global
+// memory) into LDS at dst (leading dimension ldd), eight loads in flight per
+// thread: a loop of one load and one LDS store per iteration waits for every
+// load in turn, which made a 64 x 64 copy cost 64 HBM round trips.  TRI
+// selects what is read: 0 the whole block, 1 the upper triangle (i <= j),
+// 2 the lower triangle (i >= j); entries outside it are neither read nor
+// written.  Indices past the end are clamped to a valid element and unused.
+template <int T, int TRI = 0>
+__device__ __forceinline__ void stage_block(double* dst, int ldd, const double* src, int lds_, int m, int n, int tid)
+{
+    typedef const __attribute__((address_space(1))) double* GSrc;
+    const int tot = m * n;
+    for (int e0 = 0; e0 < tot; e0 += 8 * T) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int e = e0 + u * T + tid;
+            const int ec = e < tot ? e : tot - 1;
+            const int j = ec / m;
+            const int i = ec - j * m;
+            const bool in = TRI == 0 || (TRI == 1 ? i <= j : i >= j);
+            v[u] = in ? ((GSrc)src)[i + (size_t)j * lds_] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int e = e0 + u * T + tid;
+            const int j = e / m;
+            const int i = e - j * m;
+            const bool in = TRI == 0 || (TRI == 1 ? i <= j : i >= j);
+            if (e < tot && in) dst[i + j * ldd] = v[u];
+        }
+    }
+}
+
 constexpr int kKeys = MFH_NBUCKETS * MFH_BUCKET_STEP + 1;  // size keys 0..144
 
 __host__ __device__ inline int bucket_of(int key) { return (key + MFH_BUCKET_STEP - 1) / MFH_BUCKET_STEP - 1; }
@@ -261,11 +295,7 @@ __global__ __launch_bounds__(T) void k_getrf_exact(MfhGetrfBatch B, MfhQueue q)
         double* L = lds;
         int* pv = (int*)(lds + ml * n);
 
-        for (int e = tid; e < m * n; e += T) {
-            int j = e / m;
-            int i = e - j * m;
-            L[i + j * ml] = A[i + (size_t)j * lda];
-        }
+        stage_block<T>(L, ml, A, lda, m, n, tid);
         for (int i = tid; i < m; i += T) pv[i] = i;
         __syncthreads();
 
@@ -657,12 +687,10 @@ __global__ __launch_bounds__(T) void k_trsv_exact(MfhTrsvBatch B, MfhQueue q)
         const int nl = n | 1;  // odd leading dimension: row-wise reads hit distinct banks
         double* S = lds;
         double* xs = lds + nl * n;
-        for (int j = 0; j < n; ++j) {
-            if (upper)
-                for (int i = tid; i <= j; i += T) S[i + j * nl] = A[i + (size_t)j * lda];
-            else
-                for (int i = j + tid; i < n; i += T) S[i + j * nl] = A[i + (size_t)j * lda];
-        }
+        if (upper)
+            stage_block<T, 1>(S, nl, A, lda, n, n, tid);
+        else
+            stage_block<T, 2>(S, nl, A, lda, n, n, tid);
         double xr = (tid < n) ? X[(size_t)tid * incx] : 0.0;
         __syncthreads();
         if (upper == trans) {
@@ -754,11 +782,7 @@ __global__ __launch_bounds__(T) void k_getrs_exact(MfhGetrsBatch B, MfhQueue q)
         const int nl = n | 1;  // odd leading dimension: row-wise reads hit distinct banks
         double* S = lds;
         double* xs = lds + nl * n;
-        for (int e = tid; e < n * n; e += T) {
-            int j = e / n;
-            int i = e - j * n;
-            S[i + j * nl] = LU[i + (size_t)j * lda];
-        }
+        stage_block<T>(S, nl, LU, lda, n, n, tid);
         double xr = 0.0;
         if (tid < n) {
             int pr = P[tid];
@@ -1080,11 +1104,7 @@ __global__ __launch_bounds__(GETRI_T) void k_getri_exact(MfhGetriBatch B, int co
         const int nl = n | 1;
         double* S = lds;
         double* X = lds + n * nl;
-        for (int e = tid; e < n * n; e += GETRI_T) {
-            const int j = e / n;
-            const int r = e - j * n;
-            S[r + j * nl] = a[r + (size_t)j * lda];
-        }
+        stage_block<GETRI_T>(S, nl, a, lda, n, n, tid);
         const int pr = i < n ? piv[i] : 0;
         const bool pok = pr >= 0 && pr < n;
         double xr[GETRI_JPER], s[GETRI_JPER], pend[GETRI_JPER];
