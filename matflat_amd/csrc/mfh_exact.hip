@@ -656,16 +656,20 @@ struct TrsvKey {
 // and folds the product with x_c into its running Internals.Dot as soon as
 // x_c is final, keeping the reference's pairing (length i: odd -> x0*y0 then
 // pairs (1,2),(3,4)...; even -> pairs (0,1),(2,3)...).
-__device__ __forceinline__ void dot_fold(int i, int c, double p, double& s, double& pend)
+// c is uniform but the row parity is not, so the three cases (s = p for an
+// odd row at c = 0, pend = p, s = s + (pend + p)) are predicates and the
+// fold runs without divergent branches.  Rows that are not active (act
+// false) keep s and pend.
+__device__ __forceinline__ void dot_fold(bool act, int i, int c, double p, double& s, double& pend)
 {
-    if (i & 1) {
-        if (c == 0) s = p;
-        else if (c & 1) pend = p;
-        else s = s + (pend + p);
-    } else {
-        if ((c & 1) == 0) pend = p;
-        else s = s + (pend + p);
-    }
+    const bool odd = i & 1;
+    const bool codd = c & 1;
+    const bool add = act && (odd ? (c != 0 && !codd) : codd);
+    const bool setp = act && (odd ? codd : !codd);
+    const bool sets = act && odd && c == 0;
+    const double t = s + (pend + p);
+    s = add ? t : (sets ? p : s);
+    pend = setp ? p : pend;
 }
 
 template <int T>
@@ -705,14 +709,14 @@ __global__ __launch_bounds__(T) void k_trsv_exact(MfhTrsvBatch B, MfhQueue q)
             __syncthreads();
             for (int c = 0; c < n - 1; ++c) {
                 const int i = tid;
-                if (i > c && i < n) {
-                    double aic = upper ? S[c + i * nl] : S[i + c * nl];
-                    dot_fold(i, c, aic * xs[c], s, pend);
-                    if (i == c + 1) {
-                        xr = xr - s;
-                        if (!unit) xr = xr / S[i + i * nl];
-                        xs[i] = xr;
-                    }
+                const bool act = i > c && i < n;
+                const int ai = act ? i : c;  // inactive rows read a valid slot, unused
+                const double aic = upper ? S[c + ai * nl] : S[ai + c * nl];
+                dot_fold(act, i, c, aic * xs[c], s, pend);
+                if (i == c + 1) {
+                    xr = xr - s;
+                    if (!unit) xr = xr / S[i + i * nl];
+                    xs[i] = xr;
                 }
                 __syncthreads();
             }
@@ -797,12 +801,11 @@ __global__ __launch_bounds__(T) void k_getrs_exact(MfhGetrsBatch B, MfhQueue q)
         __syncthreads();
         for (int c = 0; c < n - 1; ++c) {
             const int i = tid;
-            if (i > c && i < n) {
-                dot_fold(i, c, S[i + c * nl] * xs[c], s, pend);
-                if (i == c + 1) {
-                    xr = xr - s;
-                    xs[i] = xr;
-                }
+            const bool act = i > c && i < n;
+            dot_fold(act, i, c, S[(act ? i : c) + c * nl] * xs[c], s, pend);
+            if (i == c + 1) {
+                xr = xr - s;
+                xs[i] = xr;
             }
             __syncthreads();
         }
