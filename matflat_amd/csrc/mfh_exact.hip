@@ -35,8 +35,11 @@ namespace {
 // load in turn, which made a 64 x 64 copy cost 64 HBM round trips.  TRI
 // selects what is read: 0 the whole block, 1 the upper triangle (i <= j),
 // 2 the lower triangle (i >= j); entries outside it are neither read nor
-// written.  Indices past the end are clamped to a valid element and unused.
-template <int T, int TRI = 0>
+// written.  PK stores the triangle packed by columns instead (upper: column
+// j holds rows 0..j at j(j+1)/2; lower: column j holds rows j..n-1 at
+// j*ldd - j(j-1)/2, with ldd = n).  Indices past the end are clamped to a
+// valid element and unused.
+template <int T, int TRI = 0, bool PK = false>
 __device__ __forceinline__ void stage_block(double* dst, int ldd, const double* src, int lds_, int m, int n, int tid)
 {
     typedef const __attribute__((address_space(1))) double* GSrc;
@@ -58,7 +61,9 @@ __device__ __forceinline__ void stage_block(double* dst, int ldd, const double* 
             const int j = e / m;
             const int i = e - j * m;
             const bool in = TRI == 0 || (TRI == 1 ? i <= j : i >= j);
-            if (e < tot && in) dst[i + j * ldd] = v[u];
+            const int d = !PK ? i + j * ldd
+                              : (TRI == 1 ? ((j * (j + 1)) >> 1) + i : j * ldd - ((j * (j - 1)) >> 1) + (i - j));
+            if (e < tot && in) dst[d] = v[u];
         }
     }
 }
@@ -688,13 +693,17 @@ __global__ __launch_bounds__(T) void k_trsv_exact(MfhTrsvBatch B, MfhQueue q)
         const int incx = B.incx[b];
         const double* A = B.a[b];
         double* X = B.x[b];
-        const int nl = n | 1;  // odd leading dimension: row-wise reads hit distinct banks
+        // the triangle packed by columns: half the LDS of a square, so twice
+        // the workgroups per CU in the large buckets
         double* S = lds;
-        double* xs = lds + nl * n;
+        double* xs = lds + ((n * (n + 1)) >> 1);
+        auto at = [upper, n](int i, int j) {
+            return upper ? ((j * (j + 1)) >> 1) + i : j * n - ((j * (j - 1)) >> 1) + (i - j);
+        };
         if (upper)
-            stage_block<T, 1>(S, nl, A, lda, n, n, tid);
+            stage_block<T, 1, true>(S, n, A, lda, n, n, tid);
         else
-            stage_block<T, 2>(S, nl, A, lda, n, n, tid);
+            stage_block<T, 2, true>(S, n, A, lda, n, n, tid);
         double xr = (tid < n) ? X[(size_t)tid * incx] : 0.0;
         __syncthreads();
         if (upper == trans) {
@@ -703,7 +712,7 @@ __global__ __launch_bounds__(T) void k_trsv_exact(MfhTrsvBatch B, MfhQueue q)
             double s = 0.0, pend = 0.0;
             if (tid == 0) {
                 xr = xr - 0.0;
-                if (!unit) xr = xr / S[0];
+                if (!unit) xr = xr / S[at(0, 0)];
                 xs[0] = xr;
             }
             __syncthreads();
@@ -711,11 +720,11 @@ __global__ __launch_bounds__(T) void k_trsv_exact(MfhTrsvBatch B, MfhQueue q)
                 const int i = tid;
                 const bool act = i > c && i < n;
                 const int ai = act ? i : c;  // inactive rows read a valid slot, unused
-                const double aic = upper ? S[c + ai * nl] : S[ai + c * nl];
+                const double aic = upper ? S[at(c, ai)] : S[at(ai, c)];
                 dot_fold(act, i, c, aic * xs[c], s, pend);
                 if (i == c + 1) {
                     xr = xr - s;
-                    if (!unit) xr = xr / S[i + i * nl];
+                    if (!unit) xr = xr / S[at(i, i)];
                     xs[i] = xr;
                 }
                 __syncthreads();
@@ -724,16 +733,16 @@ __global__ __launch_bounds__(T) void k_trsv_exact(MfhTrsvBatch B, MfhQueue q)
             // Axpy forms, bottom-up: Upper/NoTrans (:191-199) subtracts column
             // i of U, Lower/Trans (:224-231) row j of L (MulSub :397-421).
             if (tid == n - 1) {
-                if (!unit) xr = xr / S[(n - 1) * (nl + 1)];
+                if (!unit) xr = xr / S[at(n - 1, n - 1)];
                 xs[n - 1] = xr;
             }
             __syncthreads();
             for (int j = n - 1; j >= 1; --j) {
                 if (tid < j) {
-                    double arj = upper ? S[tid + j * nl] : S[j + tid * nl];
+                    double arj = upper ? S[at(tid, j)] : S[at(j, tid)];
                     xr = xr - arj * xs[j];
                     if (tid == j - 1) {
-                        if (!unit) xr = xr / S[tid + tid * nl];
+                        if (!unit) xr = xr / S[at(tid, tid)];
                         xs[tid] = xr;
                     }
                 }
@@ -967,7 +976,7 @@ int mfh_launch_trsv(mfh_handle* h, int count, const MfhTrsvBatch& B, int max_n)
     for (int b = 0; b < nbuckets_for(key); ++b) {
         const int nb = bucket_bound(b, key);
         const int t = threads_for(nb);
-        const size_t lds = square_lds_bytes(nb);
+        const size_t lds = ((size_t)nb * (nb + 1) / 2 + nb) * 8 + 64;  // packed triangle + x
         const MfhQueue q = bucket_queue(ws, b);
         if (t == 64) st = launch_bucket(h, MFH_KERNEL_TRSV_EXACT, nb, k_trsv_exact<64>, 64, lds, count, B, q);
         else st = launch_bucket(h, MFH_KERNEL_TRSV_EXACT, nb, k_trsv_exact<128>, 128, lds, count, B, q);
