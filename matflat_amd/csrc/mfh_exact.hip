@@ -315,21 +315,36 @@ __global__ __launch_bounds__(T) void k_getrf_exact(MfhGetrfBatch B, MfhQueue q)
                 L[i + k * ml] = w;
             }
             double key = act ? mfh_pivot_key(w) : -2.0;
-            int idx = act ? i : INT_MAX;
-            mfh_wave_argmax(key, idx);
-            if ((tid & (MFH_WAVE - 1)) == 0) {
-                s_rkey[tid / MFH_WAVE] = key;
-                s_ridx[tid / MFH_WAVE] = idx;
+            int p;
+            if constexpr (T == MFH_WAVE) {
+                // One wavefront: the largest key, then the lowest lane that
+                // holds it (lane = row - k, so the lowest row: the strict
+                // '>' scan's choice).  Row k is active, so the maximum is a
+                // real candidate's key.
+                double mk = key;
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) mk = __builtin_fmax(mk, __shfl_xor(mk, off, MFH_WAVE));
+                const unsigned long long hit = __ballot(key == mk);
+                const double wk = __shfl(w, 0, MFH_WAVE);  // lane 0 holds w_k
+                p = __builtin_isnan(wk) ? k : k + __ffsll((long long)hit) - 1;
+                __syncthreads();
+            } else {
+                int idx = act ? i : INT_MAX;
+                mfh_wave_argmax(key, idx);
+                if ((tid & (MFH_WAVE - 1)) == 0) {
+                    s_rkey[tid / MFH_WAVE] = key;
+                    s_ridx[tid / MFH_WAVE] = idx;
+                }
+                __syncthreads();
+                if (tid == 0) {
+                    double kk = s_rkey[0];
+                    int ii = s_ridx[0];
+                    for (int wv = 1; wv < T / MFH_WAVE; ++wv) mfh_pivot_combine(kk, ii, s_rkey[wv], s_ridx[wv]);
+                    s_p = __builtin_isnan(w) ? k : ii;  // thread 0 holds w_k
+                }
+                __syncthreads();
+                p = s_p;
             }
-            __syncthreads();
-            if (tid == 0) {
-                double kk = s_rkey[0];
-                int ii = s_ridx[0];
-                for (int wv = 1; wv < T / MFH_WAVE; ++wv) mfh_pivot_combine(kk, ii, s_rkey[wv], s_ridx[wv]);
-                s_p = __builtin_isnan(w) ? k : ii;  // thread 0 holds w_k
-            }
-            __syncthreads();
-            const int p = s_p;
             if (p != k) {
                 for (int j = tid; j < n; j += T) {
                     double t = L[k + j * ml];
