@@ -483,13 +483,20 @@ def gemm_leg(steps: int, warmup: int, count: int = 200_000, seed: int = 5, compl
         ms = timed_calls(lambda: hip.gemm_vbatched(*args, info, 128, 128, 128, complex_=complex_, handle=handle),
                          None, steps, warmup, stream)
         ok = bool(torch.all(info == 0).item())
-        res[mode_name] = {"ms_per_call": round(ms, 5), "products_per_s": round(count / (ms / 1e3), 1),
+        # the same batch as one captured graph (tile-list build + GEMM kernel), replayed
+        plan = hip.GemmPlan(*args, info, 128, 128, 128, complex_=complex_, handle=handle)
+        ms_graph = timed_calls(plan.execute, None, steps, warmup, stream)
+        plan.close()
+        ok = ok and bool(torch.all(info == 0).item())
+        res[mode_name] = {"ms_per_call": round(ms, 5), "ms_per_graph_replay": round(ms_graph, 5),
+                          "products_per_s": round(count / (ms / 1e3), 1),
                           "tflops": round(float(flops.sum()) / (ms / 1e3) / 1e12, 2), "all_info_zero": ok,
                           "roofline": hbm_entry(float(alg.sum()), ms),
                           "frac_of_model_ceiling": round(ceiling_ms / ms, 4),
                           "note": ("FP64 matrix cores, within 4 k eps of MatFlat" if mode == hip.MATH_FAST
                                    else "MatFlat's summation order, bit-identical")}
-        log(f"  gemm {name} {mode_name} x{count}: {ms:.4f} ms/call, {res[mode_name]['tflops']} TFLOP/s, "
+        log(f"  gemm {name} {mode_name} x{count}: {ms:.4f} ms/call ({ms_graph:.4f} as a graph), "
+            f"{res[mode_name]['tflops']} TFLOP/s, "
             f"ceiling {ceiling_ms:.3f} ms ({res[mode_name]['frac_of_model_ceiling']:.3f})")
     handle.close()
     del A, Bt, C

@@ -271,6 +271,21 @@ int mfh_dposv_backward_vbatched(mfh_handle* h, int count, const int* n, const do
 int mfh_plan_create_dposv(mfh_handle* h, int count, const int* n, double* const* a,
                           const int* lda, double* const* x, int* info, int max_n,
                           mfh_plan** plan);
+/* The batched GEMM of mfh_dgemm_vbatched / mfh_zgemm_vbatched (same
+ * arguments) captured into one graph: tile-list build and the GEMM kernel,
+ * in the handle's math mode at capture time.  Replays read the per-product
+ * arrays afresh, so sizes and data may change between replays within the
+ * captured max_m and max_n. */
+int mfh_plan_create_dgemm(mfh_handle* h, int count, const int* transa, const int* transb,
+                          const int* m, const int* n, const int* k, const double* alpha,
+                          const double* const* a, const int* lda, const double* const* b,
+                          const int* ldb, const double* beta, double* const* c, const int* ldc,
+                          int* info, int max_m, int max_n, int max_k, mfh_plan** plan);
+int mfh_plan_create_zgemm(mfh_handle* h, int count, const int* transa, const int* transb,
+                          const int* m, const int* n, const int* k, const double* alpha,
+                          const double* const* a, const int* lda, const double* const* b,
+                          const int* ldb, const double* beta, double* const* c, const int* ldc,
+                          int* info, int max_m, int max_n, int max_k, mfh_plan** plan);
 int mfh_plan_execute(mfh_plan* plan);
 int mfh_plan_destroy(mfh_plan* plan);
 

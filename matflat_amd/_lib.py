@@ -65,6 +65,8 @@ SIGNATURES = {
     "mfh_dgesv_vbatched": [_p, _i, _p, _p, _p, _p, _p, _p, _p, _i],
     "mfh_dposv_backward_vbatched": [_p, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i],
     "mfh_plan_create_dposv": [_p, _i, _p, _p, _p, _p, _p, _i, _p],
+    "mfh_plan_create_dgemm": [_p, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i, _i, _i, _p],
+    "mfh_plan_create_zgemm": [_p, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i, _i, _i, _p],
     "mfh_plan_execute": [_p],
     "mfh_plan_destroy": [_p],
 }

@@ -526,19 +526,13 @@ struct mfh_plan {
     void* ws = nullptr;  // the plan's own workspace: later calls on the handle may regrow the handle's
 };
 
-extern "C" {
-
-int mfh_plan_create_dposv(mfh_handle* h, int count, const int* n, double* const* a, const int* lda, double* const* x,
-                          int* info, int max_n, mfh_plan** plan)
+// Records one call into a hipGraph.  The plan owns a workspace sized
+// exactly as the launcher asks for it; the capture runs on a private stream
+// (the handle's stream may be the null stream, which cannot be captured)
+// with the handle temporarily pointing at the plan's workspace.
+template <typename F>
+static int capture_plan(mfh_handle* h, size_t ws_bytes, F call, mfh_plan** plan)
 {
-    MFH_REQUIRE(h != nullptr, 1, "handle must not be null");
-    MFH_REQUIRE(plan != nullptr, 9, "plan must not be null");
-    MFH_REQUIRE(count >= 1, 2, "count must be >= 1");
-    // The plan owns a workspace sized exactly as the launcher asks for it;
-    // the capture runs on a private stream (the handle's stream may be the
-    // null stream, which cannot be captured) with the handle temporarily
-    // pointing at the plan's workspace.
-    const size_t ws_bytes = mfh_exact_ws_bytes(count);
     // side streams of the bucket fork exist before the capture starts
     int sst = mfh_side_streams(h, h->bucket_streams);
     if (sst) return sst;
@@ -561,7 +555,7 @@ int mfh_plan_create_dposv(mfh_handle* h, int count, const int* n, double* const*
     hipGraph_t graph = nullptr;
     e = hipStreamBeginCapture(cap, hipStreamCaptureModeRelaxed);
     if (e == hipSuccess) {
-        st = mfh_dposv_vbatched(h, count, n, a, lda, x, info, max_n);
+        st = call();
         e = hipStreamEndCapture(cap, &graph);
     }
     h->stream = saved;
@@ -584,6 +578,59 @@ int mfh_plan_create_dposv(mfh_handle* h, int count, const int* n, double* const*
     p->ws = pws;
     *plan = p;
     return MFH_OK;
+}
+
+extern "C" {
+
+int mfh_plan_create_dposv(mfh_handle* h, int count, const int* n, double* const* a, const int* lda, double* const* x,
+                          int* info, int max_n, mfh_plan** plan)
+{
+    MFH_REQUIRE(h != nullptr, 1, "handle must not be null");
+    MFH_REQUIRE(plan != nullptr, 9, "plan must not be null");
+    MFH_REQUIRE(count >= 1, 2, "count must be >= 1");
+    return capture_plan(
+        h, mfh_exact_ws_bytes(count), [&]() { return mfh_dposv_vbatched(h, count, n, a, lda, x, info, max_n); },
+        plan);
+}
+
+static int plan_gemm(mfh_handle* h, int count, const int* transa, const int* transb, const int* m, const int* n,
+                     const int* k, const double* alpha, const double* const* a, const int* lda,
+                     const double* const* b, const int* ldb, const double* beta, double* const* c, const int* ldc,
+                     int* info, int max_m, int max_n, int max_k, bool cplx, mfh_plan** plan)
+{
+    MFH_REQUIRE(h != nullptr, 1, "handle must not be null");
+    MFH_REQUIRE(count >= 1, 2, "count must be >= 1");
+    MFH_REQUIRE(plan != nullptr, 20, "plan must not be null");
+    MFH_REQUIRE(max_m >= 1 && max_m <= 4096, 17, "max_m must be in [1, 4096]");
+    MFH_REQUIRE(max_n >= 1 && max_n <= 4096, 18, "max_n must be in [1, 4096]");
+    const size_t ws = mfh_gemm_ws_bytes(count, max_m, max_n, h->math_mode == MFH_MATH_FAST);
+    return capture_plan(
+        h, ws,
+        [&]() {
+            return cplx ? mfh_zgemm_vbatched(h, count, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc,
+                                             info, max_m, max_n, max_k)
+                        : mfh_dgemm_vbatched(h, count, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc,
+                                             info, max_m, max_n, max_k);
+        },
+        plan);
+}
+
+int mfh_plan_create_dgemm(mfh_handle* h, int count, const int* transa, const int* transb, const int* m, const int* n,
+                          const int* k, const double* alpha, const double* const* a, const int* lda,
+                          const double* const* b, const int* ldb, const double* beta, double* const* c,
+                          const int* ldc, int* info, int max_m, int max_n, int max_k, mfh_plan** plan)
+{
+    return plan_gemm(h, count, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, info, max_m, max_n,
+                     max_k, false, plan);
+}
+
+int mfh_plan_create_zgemm(mfh_handle* h, int count, const int* transa, const int* transb, const int* m, const int* n,
+                          const int* k, const double* alpha, const double* const* a, const int* lda,
+                          const double* const* b, const int* ldb, const double* beta, double* const* c,
+                          const int* ldc, int* info, int max_m, int max_n, int max_k, mfh_plan** plan)
+{
+    return plan_gemm(h, count, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, info, max_m, max_n,
+                     max_k, true, plan);
 }
 
 int mfh_plan_execute(mfh_plan* plan)

@@ -680,17 +680,44 @@ __global__ __launch_bounds__(64 * FW) void k_gemm_mfma(MfhGemmBatch B, int count
 
 }  // namespace
 
+// Workspace layout of one GEMM call (ints): per-product tile counts, chunk
+// sums, then the tile -> product map.
+struct GemmWsLayout {
+    long maxtiles;
+    int nchunks;
+    size_t o_cnt, o_bsum, o_map, bytes;
+};
+
+static GemmWsLayout gemm_layout(int count, int max_m, int max_n, bool fast)
+{
+    const int ts = fast ? FAST_TS : TM;
+    GemmWsLayout L;
+    L.maxtiles = (long)count * ((max_m + ts - 1) / ts) * ((max_n + ts - 1) / ts);
+    L.nchunks = (count + SCAN_CHUNK - 1) / SCAN_CHUNK;
+    auto up = [](size_t v) { return (v + 63) / 64 * 64; };
+    L.o_cnt = up((size_t)count + 1);
+    L.o_bsum = L.o_cnt + up(count);
+    L.o_map = L.o_bsum + up(L.nchunks);
+    L.bytes = sizeof(int) * (L.o_map + (size_t)(L.maxtiles > 0 ? L.maxtiles : 0));
+    return L;
+}
+
+size_t mfh_gemm_ws_bytes(int count, int max_m, int max_n, bool fast)
+{
+    return gemm_layout(count, max_m, max_n, fast).bytes;
+}
+
 int mfh_launch_gemm(mfh_handle* h, int count, const MfhGemmBatch& B, int max_m, int max_n, int max_k, bool cplx)
 {
     (void)max_k;
     const bool fast = h->math_mode == MFH_MATH_FAST;
     const int ts = fast ? FAST_TS : TM;
-    const long maxtiles = (long)count * ((max_m + ts - 1) / ts) * ((max_n + ts - 1) / ts);
+    const GemmWsLayout Lw = gemm_layout(count, max_m, max_n, fast);
+    const long maxtiles = Lw.maxtiles;
     if (maxtiles > 0x7fffffffL - 64) return mfh_fail(MFH_ERR_UNSUPPORTED, "too many GEMM tiles in one call");
-    const int nchunks = (count + SCAN_CHUNK - 1) / SCAN_CHUNK;
-    auto up = [](size_t v) { return (v + 63) / 64 * 64; };
-    const size_t o_cnt = up((size_t)count + 1), o_bsum = o_cnt + up(count), o_map = o_bsum + up(nchunks);
-    int st = mfh_ws_reserve(h, sizeof(int) * (o_map + (size_t)maxtiles));
+    const int nchunks = Lw.nchunks;
+    const size_t o_cnt = Lw.o_cnt, o_bsum = Lw.o_bsum, o_map = Lw.o_map;
+    int st = mfh_ws_reserve(h, Lw.bytes);
     if (st) return st;
     int* w = (int*)h->ws;
     GemmWs ws{w, w + o_cnt, w + o_bsum, w + o_map};

@@ -170,6 +170,9 @@ struct MfhAdjointBatch {
 // Device workspace bytes the exact kernels need for a batch of count.
 size_t mfh_exact_ws_bytes(int count);
 
+// Workspace bytes of one GEMM call (mfh_gemm.hip); plans reserve this.
+size_t mfh_gemm_ws_bytes(int count, int max_m, int max_n, bool fast);
+
 // Built-in choice of wavefronts per matrix (1 or 2) of the FAST Cholesky
 // kernel for matrices of up to 8 * rb rows.
 int mfh_fast_default_waves(int rb);

@@ -244,6 +244,37 @@ def dposv_backward_vbatched(n, l_ptrs, lda, x_ptrs, xbar_ptrs, abar_ptrs, ldabar
                                              int(max_n)))
 
 
+class GemmPlan:
+    """A batched GEMM (gemm_vbatched's arguments) captured once into a
+    hipGraph and replayed."""
+
+    def __init__(self, transa, transb, m, n, k, alpha, a_ptrs, lda, b_ptrs, ldb, beta, c_ptrs, ldc, info,
+                 max_m, max_n, max_k, complex_: bool = False, handle: Handle | None = None):
+        self.handle = handle or default_handle()
+        self._keep = (transa, transb, m, n, k, alpha, a_ptrs, lda, b_ptrs, ldb, beta, c_ptrs, ldc, info)
+        fn = self.handle.lib.mfh_plan_create_zgemm if complex_ else self.handle.lib.mfh_plan_create_dgemm
+        p = ctypes.c_void_p()
+        _check(fn(self.handle.h, int(m.numel()), _p(transa), _p(transb), _p(m), _p(n), _p(k), _p(alpha), _p(a_ptrs),
+                  _p(lda), _p(b_ptrs), _p(ldb), _p(beta), _p(c_ptrs), _p(ldc), _p(info), int(max_m), int(max_n),
+                  int(max_k), ctypes.byref(p)))
+        self.p = p
+
+    def execute(self) -> None:
+        self.handle.bind_stream()
+        _check(self.handle.lib.mfh_plan_execute(self.p))
+
+    def close(self) -> None:
+        if self.p is not None and self.p.value:
+            self.handle.lib.mfh_plan_destroy(self.p)
+            self.p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class DposvPlan:
     """A batched SPD solve captured once into a hipGraph and replayed."""
 

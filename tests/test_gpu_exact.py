@@ -396,6 +396,51 @@ def _op(buf, ld, t, rows, cols):
     return np.conj(x) if t in (113, 114) else x
 
 
+@pytest.mark.parametrize("mode", ["exact", "fast"])
+@pytest.mark.parametrize("complex_", [False, True])
+def test_gemm_plan_replays_match_eager(complex_, mode):
+    """A batched GEMM captured into a plan and replayed four times, with
+    eager work and synchronisation between replays (C restored before each),
+    gives exactly the bits of the eager call in the same math mode."""
+    hip = hipmod()
+    recs = gemm_batch(complex_, 37, 200, shapes=[(m, n, k) for (m, n, k, *_r) in GEMM_CASES])
+    dt = torch.complex128 if complex_ else torch.float64
+    A = Packed([r[8] for r in recs], dtype=dt)
+    Bm = Packed([r[9] for r in recs], dtype=dt)
+    C0 = Packed([r[10] for r in recs], dtype=dt)
+    C = Packed([r[10] for r in recs], dtype=dt)
+    if complex_:
+        al = f64(np.array([[complex(r[11]).real, complex(r[11]).imag] for r in recs]).reshape(-1))
+        be = f64(np.array([[complex(r[12]).real, complex(r[12]).imag] for r in recs]).reshape(-1))
+    else:
+        al = f64([r[11] for r in recs])
+        be = f64([r[12] for r in recs])
+    col = lambda j: i32([r[j] for r in recs])  # noqa: E731
+    args = (col(0), col(1), col(2), col(3), col(4), al, A.ptrs, col(5), Bm.ptrs, col(6), be, C.ptrs, col(7))
+    h = hip.Handle()
+    h.bind_stream()
+    h.set_math_mode(hip.MATH_FAST if mode == "fast" else hip.MATH_EXACT)
+    info = i32(np.full(len(recs), 99))
+    hip.gemm_vbatched(*args, info, 128, 128, 128, complex_=complex_, handle=h)
+    torch.cuda.synchronize()
+    want = C.all()
+    assert np.all(info.cpu().numpy() == 0)
+    plan = hip.GemmPlan(*args, info, 128, 128, 128, complex_=complex_, handle=h)
+    for rep in range(4):
+        C.dev.copy_(C0.dev)
+        info.fill_(99)
+        torch.cuda.synchronize()
+        plan.execute()
+        torch.cuda.synchronize()
+        _ = C.dev.abs().sum().item()  # eager work between replays
+        assert np.all(info.cpu().numpy() == 0), rep
+        got = C.all()
+        for i in range(len(recs)):
+            assert_bits_equal(got[i], want[i], f"replay {rep} product {i}")
+    plan.close()
+    h.close()
+
+
 @pytest.mark.parametrize("complex_", [False, True])
 def test_gemm_fast_mode_within_k_eps(complex_):
     """MFH_MATH_FAST runs the products on FP64 matrix cores; each product
