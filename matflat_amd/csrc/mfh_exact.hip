@@ -465,10 +465,37 @@ __global__ __launch_bounds__(64) void k_chol_packed(MfhCholBatch B, MfhQueue q, 
         const int lda = b >= 0 ? B.lda[b] : 0;
         double* X = (POSV && b >= 0) ? B.x[b] : nullptr;
 
-        // Column j of the strict upper triangle -> row j (both contiguous).
-        for (int j = 1; j < nmax; ++j) {
-            if (j < n)
-                for (int c = l; c < j; c += G) Mx[rowp(j) + c] = A[c + (size_t)j * lda];
+        // Column j of the strict upper triangle -> row j (both contiguous):
+        // element (c, j), c < j, to Mx[rowp(j) + c].  Lane l of the group
+        // walks packed elements l, l + G, ... (row j holds j of them) with
+        // eight loads in flight; one load, wait and store per element had
+        // serialised n^2 / (2G) HBM round trips per lane.
+        {
+            typedef const __attribute__((address_space(1))) double* GSrc;
+            const int tot = n > 1 ? rowp(n) : 0;
+            int j = 1, c = l;
+            while (c >= j) {
+                c -= j;
+                ++j;
+            }
+            for (int e0 = l; e0 < tot; e0 += 8 * G) {
+                double v[8];
+                int dst[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const bool ok = e0 + u * G < tot;
+                    dst[u] = ok ? rowp(j) + c : -1;
+                    v[u] = ok ? ((GSrc)A)[c + (size_t)j * lda] : 0.0;
+                    c += G;
+                    while (c >= j && j < n) {
+                        c -= j;
+                        ++j;
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    if (dst[u] >= 0) Mx[dst[u]] = v[u];
+            }
         }
         if (POSV)
             for (int c = l; c < n; c += G) Mx[rowp(n) + c] = X[c];
