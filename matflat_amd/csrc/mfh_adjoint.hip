@@ -53,8 +53,7 @@ __global__ __launch_bounds__(T) void k_posv_backward(MfhAdjointBatch B, int coun
         double* S = lds;
         double* xs = lds + n * n;
         double* xv = xs + n;  // x, shared for the outer product
-        for (int j = 0; j < n; ++j)
-            for (int i = j + tid; i < n; i += T) S[i + j * n] = L[i + (size_t)j * lda];
+        stage_block<T, 2>(S, n, L, lda, n, n, tid);
         double br = (tid < n) ? B.xbar[b][tid] : 0.0;
         double xr = (tid < n) ? B.x[b][tid] : 0.0;
         if (tid < n) xv[tid] = xr;

@@ -29,44 +29,6 @@
 
 namespace {
 
-// Copies the m x n column-major block at src (leading dimension lds_, global
-// memory) into LDS at dst (leading dimension ldd), eight loads in flight per
-// thread: a loop of one load and one LDS store per iteration waits for every
-// load in turn, which made a 64 x 64 copy cost 64 HBM round trips.  TRI
-// selects what is read: 0 the whole block, 1 the upper triangle (i <= j),
-// 2 the lower triangle (i >= j); entries outside it are neither read nor
-// written.  PK stores the triangle packed by columns instead (upper: column
-// j holds rows 0..j at j(j+1)/2; lower: column j holds rows j..n-1 at
-// j*ldd - j(j-1)/2, with ldd = n).  Indices past the end are clamped to a
-// valid element and unused.
-template <int T, int TRI = 0, bool PK = false>
-__device__ __forceinline__ void stage_block(double* dst, int ldd, const double* src, int lds_, int m, int n, int tid)
-{
-    typedef const __attribute__((address_space(1))) double* GSrc;
-    const int tot = m * n;
-    for (int e0 = 0; e0 < tot; e0 += 8 * T) {
-        double v[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int e = e0 + u * T + tid;
-            const int ec = e < tot ? e : tot - 1;
-            const int j = ec / m;
-            const int i = ec - j * m;
-            const bool in = TRI == 0 || (TRI == 1 ? i <= j : i >= j);
-            v[u] = in ? ((GSrc)src)[i + (size_t)j * lds_] : 0.0;
-        }
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int e = e0 + u * T + tid;
-            const int j = e / m;
-            const int i = e - j * m;
-            const bool in = TRI == 0 || (TRI == 1 ? i <= j : i >= j);
-            const int d = !PK ? i + j * ldd
-                              : (TRI == 1 ? ((j * (j + 1)) >> 1) + i : j * ldd - ((j * (j - 1)) >> 1) + (i - j));
-            if (e < tot && in) dst[d] = v[u];
-        }
-    }
-}
 
 constexpr int kKeys = MFH_NBUCKETS * MFH_BUCKET_STEP + 1;  // size keys 0..144
 
