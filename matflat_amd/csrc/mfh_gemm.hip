@@ -147,19 +147,26 @@ __global__ __launch_bounds__(SCAN_T) void k_gemm_map(GemmWs ws, int count, int n
 template <typename V>
 __device__ inline void stage(V (*S)[TM + 1], const V* X, int ld, bool trans, int r0, int rows, int k0, int kend)
 {
-    for (int e = threadIdx.x; e < KC * TM; e += GT) {
-        int r, kk;
-        if (trans) {
-            kk = e % KC;
-            r = e / KC;
-        } else {
-            r = e % TM;
-            kk = e / TM;
-        }
+    // every load of the chunk goes out before the first LDS store: a loop
+    // of load, wait and store per element serialised PER round trips
+    constexpr int PER = KC * TM / GT;
+    static_assert(KC * TM % GT == 0, "chunk must divide over the workgroup");
+    V v[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int e = threadIdx.x + u * GT;
+        const int r = trans ? e / KC : e % TM;
+        const int kk = trans ? e % KC : e / TM;
         const int gr = r0 + r, gk = k0 + kk;
-        V v{};
-        if (gr < rows && gk < kend) v = trans ? X[gk + (size_t)gr * ld] : X[gr + (size_t)gk * ld];
-        S[kk][r] = v;
+        v[u] = V{};
+        if (gr < rows && gk < kend) v[u] = trans ? X[gk + (size_t)gr * ld] : X[gr + (size_t)gk * ld];
+    }
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int e = threadIdx.x + u * GT;
+        const int r = trans ? e / KC : e % TM;
+        const int kk = trans ? e % KC : e / TM;
+        S[kk][r] = v[u];
     }
 }
 
