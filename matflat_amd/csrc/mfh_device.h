@@ -13,9 +13,14 @@
 
 // Internals.Dot (MatFlat/Internals.cs:265-292): odd length starts with
 // x0*y0, even length with 0.0; then sum = sum + (x_k*y_k + x_{k+1}*y_{k+1}).
-__device__ __forceinline__ double mfh_dot_exact(int len, const double* x, int incx, const double* y, int incy)
+// W (even) products per step: their loads and products do not depend on s,
+// so they are issued together and only the W/2 additions into s stay
+// sequential, in MatFlat's order.
+template <int W>
+__device__ __forceinline__ double mfh_dot_exact_w(int len, const double* x, int incx, const double* y, int incy)
 {
 #pragma clang fp contract(off)
+    static_assert(W % 2 == 0, "products are added in pairs");
     double s;
     int c;
     if (len & 1) {
@@ -25,17 +30,21 @@ __device__ __forceinline__ double mfh_dot_exact(int len, const double* x, int in
         s = 0.0;
         c = 0;
     }
-    // Four pairs per step: the eight loads and products do not depend on s,
-    // so they are issued together and only the four additions into s stay
-    // sequential, in MatFlat's order.
-    for (; c + 8 <= len; c += 8) {
-        double p[8];
+    for (; c + W <= len; c += W) {
+        double p[W];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) p[u] = x[(c + u) * incx] * y[(c + u) * incy];
-        s = s + (p[0] + p[1]);
-        s = s + (p[2] + p[3]);
-        s = s + (p[4] + p[5]);
-        s = s + (p[6] + p[7]);
+        for (int u = 0; u < W; ++u) p[u] = x[(c + u) * incx] * y[(c + u) * incy];
+#pragma unroll
+        for (int u = 0; u < W; u += 2) s = s + (p[u] + p[u + 1]);
+    }
+    if constexpr (W > 8) {
+        for (; c + 8 <= len; c += 8) {
+            double p[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) p[u] = x[(c + u) * incx] * y[(c + u) * incy];
+#pragma unroll
+            for (int u = 0; u < 8; u += 2) s = s + (p[u] + p[u + 1]);
+        }
     }
     for (; c < len; c += 2) {
         double p0 = x[c * incx] * y[c * incy];
@@ -43,6 +52,11 @@ __device__ __forceinline__ double mfh_dot_exact(int len, const double* x, int in
         s = s + (p0 + p1);
     }
     return s;
+}
+
+__device__ __forceinline__ double mfh_dot_exact(int len, const double* x, int incx, const double* y, int incy)
+{
+    return mfh_dot_exact_w<8>(len, x, incx, y, incy);
 }
 
 // Copies the m x n column-major block at src (leading dimension lds_, global

@@ -60,10 +60,10 @@ __global__ __launch_bounds__(64) void k_dgemv_exact(MfhGemvBatch B, int count, i
         double* y = B.y[p];
         if (t) {
             // Blas.MulMatVec.cs:194-199: y_i = Dot(m, a + i*lda, 1, x, incx)
-            for (int i = lane; i < n; i += MFH_WAVE) y[i * incy] = mfh_dot_exact(m, a + i * lda, 1, x, incx);
+            for (int i = lane; i < n; i += MFH_WAVE) y[i * incy] = mfh_dot_exact_w<16>(m, a + i * lda, 1, x, incx);
         } else {
             // Blas.MulMatVec.cs:186-191: y_i = Dot(n, a + i, lda, x, incx)
-            for (int i = lane; i < m; i += MFH_WAVE) y[i * incy] = mfh_dot_exact(n, a + i, lda, x, incx);
+            for (int i = lane; i < m; i += MFH_WAVE) y[i * incy] = mfh_dot_exact_w<16>(n, a + i, lda, x, incx);
         }
     }
 }
