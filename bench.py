@@ -658,8 +658,8 @@ def main() -> int:
                     help="also run one 256 MiB device copy (reference for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (matrices are independent: "
-                                   "a multi-GPU run would shard the batch, see DESIGN.md)"}))
+        print(json.dumps({"error": "this library runs on one GPU; --gpus must be 1 (matrices are independent: "
+                                   "a multi-GPU run would give each GPU a slice of the size-sorted batch)"}))
         return 2
 
     import torch
