@@ -40,6 +40,21 @@ def test_null_handle_rejected_as_argument_1():
     assert lib.mfh_dgetrf_vbatched(None, 1, None, None, None, None, None, None, None, 4, 4) == -1
     assert lib.mfh_plan_execute(None) == -1
     assert lib.mfh_version() == 100
+    assert lib.mfh_dgetri_vbatched(None, 1, None, None, None, None, None, 8) == -1
+    assert lib.mfh_dgemv_vbatched(None, 1, None, None, None, None, None, None, None, None, None, None, 4, 4) == -1
+    assert lib.mfh_set_bucket_streams(None, 2) == -1
+    assert lib.mfh_plan_create_dgemm(None, 1, *([None] * 14), 4, 4, 4, None) == -1
+    assert lib.mfh_plan_create_zgemm(None, 1, *([None] * 14), 4, 4, 4, None) == -1
+
+
+def test_empty_batches_launch_nothing():
+    """count = 0 returns OK before any pointer is looked at, as a batch of no
+    matrices has nothing to check (MatFlat has no batched form; its
+    per-matrix checks run per product on the device)."""
+    lib = _lib.load()
+    fake = ctypes.c_void_p(1)  # never dereferenced: the call returns first
+    assert lib.mfh_dgetri_vbatched(fake, 0, None, None, None, None, None, 8) == 0
+    assert lib.mfh_dgemv_vbatched(fake, 0, None, None, None, None, None, None, None, None, None, None, 4, 4) == 0
 
 
 def test_create_without_gpu_reports_hip_error():
