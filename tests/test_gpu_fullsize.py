@@ -1,0 +1,147 @@
+"""Parity at the full workload sizes, through properties that do not depend
+on the batch size (the oracle is far too slow for every matrix):
+
+* 100,000 SPD solves, n ~ U{10..100}, lda = 8 ceil(n/8) (the benchmark's
+  workload, built by bench.make_spd_batch): every solve's normwise backward
+  error ||b - A x||_inf / (||A||_inf ||x||_inf + ||b||_inf) <= n eps, every
+  factor's strict upper triangle exactly zero and its padding rows
+  untouched, info = 0; in exact mode a random sample of 200 matrices is also
+  compared bit for bit with the oracle (MatFlat's Factorization.Cholesky and
+  two Blas.SolveTriangular calls).
+* 200,000 FP64 products C = alpha op(A) op(B) + beta C, m, n, k ~ U{8..128}:
+  a random sample of 300 products against the oracle, bit for bit in exact
+  mode and within 4 k eps (|alpha| ||A|| ||B|| + |beta| ||C0||) in FAST mode.
+"""
+import numpy as np
+import pytest
+import torch
+
+import bench
+import oracle
+from gpu_util import assert_bits_equal
+
+pytestmark = pytest.mark.gpu
+EPS = np.finfo(np.float64).eps
+
+
+def hipmod():
+    from matflat_amd import hip
+    return hip
+
+
+@pytest.mark.parametrize("mode", ["fast", "exact"])
+def test_spd_solve_100k_backward_error(mode):
+    hip = hipmod()
+    dev = torch.device("cuda")
+    count = 100_000
+    buf, offsets, ns, ldas, xoff, b = bench.make_spd_batch(count, 10, 100, seed=2)
+    a0, b0 = buf.clone(), b.clone()
+    n_t = torch.from_numpy(ns).to(dev)
+    lda_t = torch.from_numpy(ldas).to(dev)
+    a_ptrs = (buf.data_ptr() + 8 * torch.from_numpy(offsets).to(dev)).contiguous()
+    x_ptrs = (b.data_ptr() + 8 * torch.from_numpy(xoff).to(dev)).contiguous()
+    info = torch.full((count,), 99, dtype=torch.int32, device=dev)
+    h = hip.Handle()
+    h.bind_stream()
+    h.set_math_mode(hip.MATH_FAST if mode == "fast" else hip.MATH_EXACT)
+    hip.dposv_vbatched(n_t, a_ptrs, lda_t, x_ptrs, info, 100, handle=h)
+    torch.cuda.synchronize()
+    h.close()
+    assert int((info != 0).sum().item()) == 0
+    worst = 0.0
+    for n in range(10, 101):
+        idx = np.flatnonzero(ns == n)
+        if idx.size == 0:
+            continue
+        c, lda = idx.size, int(ldas[idx[0]])
+        start = int(offsets[idx].min())
+        # matrices of one order are contiguous, in batch order: [matrix, column, row]
+        orig = a0[start:start + c * n * lda].view(c, n, lda)
+        fact = buf[start:start + c * n * lda].view(c, n, lda)
+        at = orig[:, :, :n].transpose(1, 2)  # [matrix, row, column]
+        upper = torch.triu(torch.ones(n, n, dtype=torch.bool, device=dev))
+        a_full = torch.where(upper, at, at.transpose(1, 2))
+        xi = torch.from_numpy(xoff[idx][:, None] + np.arange(n)[None, :]).to(dev)
+        x, bb = b[xi], b0[xi]
+        r = bb - torch.bmm(a_full, x.unsqueeze(2)).squeeze(2)
+        eta = r.abs().amax(1) / (a_full.abs().sum(2).amax(1) * x.abs().amax(1) + bb.abs().amax(1))
+        worst = max(worst, float((eta / (n * EPS)).max().item()))
+        lt = fact[:, :, :n].transpose(1, 2)  # factor as [matrix, row, column]
+        strict_upper = torch.triu(torch.ones(n, n, dtype=torch.bool, device=dev), 1)
+        assert bool((lt[:, strict_upper] == 0).all()), n
+        if lda > n:
+            assert bool(torch.isnan(fact[:, :, n:]).all()), n  # padding rows never written
+    assert worst <= 1.0, worst
+    if mode == "exact":
+        rng = np.random.default_rng(7)
+        host_a0, host_b0 = a0.cpu().numpy(), b0.cpu().numpy()
+        host_a, host_b = buf.cpu().numpy(), b.cpu().numpy()
+        for i in rng.choice(count, 200, replace=False):
+            n, lda, o, xo = int(ns[i]), int(ldas[i]), int(offsets[i]), int(xoff[i])
+            wa = host_a0[o:o + lda * n].copy()
+            wx = host_b0[xo:xo + n].copy()
+            assert oracle.dposv(n, wa, lda, wx) == 0
+            assert_bits_equal(host_a[o:o + lda * n], wa, f"factor {i} n={n}")
+            assert_bits_equal(host_b[xo:xo + n], wx, f"x {i} n={n}")
+
+
+@pytest.mark.parametrize("mode", ["exact", "fast"])
+def test_gemm_200k_sample_against_oracle(mode):
+    hip = hipmod()
+    dev = torch.device("cuda")
+    count = 200_000
+    rng = np.random.default_rng(5)
+    m, n, k = (rng.integers(8, 129, size=count).astype(np.int64) for _ in range(3))
+    ta, tb = rng.choice([111, 112], size=count), rng.choice([111, 112], size=count)
+    ra, ca = np.where(ta == 111, m, k), np.where(ta == 111, k, m)
+    rb, cb = np.where(tb == 111, k, n), np.where(tb == 111, n, k)
+    lda, ldb, ldc = (ra + 3) // 4 * 4, (rb + 3) // 4 * 4, (m + 3) // 4 * 4
+
+    def slab(sizes):
+        off = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+        t = torch.randn(int(sizes.sum()), dtype=torch.float64, device=dev)
+        return t, off, (t.data_ptr() + 8 * torch.from_numpy(off).to(dev)).contiguous()
+
+    A, aoff, pa = slab(lda * ca)
+    B, boff, pb = slab(ldb * cb)
+    C, coff, pc = slab(ldc * n)
+    C0 = C.clone()
+    alpha = torch.from_numpy(rng.uniform(-1, 1, count)).to(dev)
+    beta = torch.from_numpy(rng.uniform(-1, 1, count)).to(dev)
+
+    def i32(v):
+        return torch.from_numpy(np.ascontiguousarray(v, dtype=np.int32)).to(dev)
+
+    info = torch.full((count,), 99, dtype=torch.int32, device=dev)
+    h = hip.Handle()
+    h.bind_stream()
+    h.set_math_mode(hip.MATH_FAST if mode == "fast" else hip.MATH_EXACT)
+    hip.gemm_vbatched(i32(ta), i32(tb), i32(m), i32(n), i32(k), alpha, pa, i32(lda), pb, i32(ldb), beta, pc,
+                      i32(ldc), info, 128, 128, 128, handle=h)
+    torch.cuda.synchronize()
+    h.close()
+    assert int((info != 0).sum().item()) == 0
+    ha, hb, hc, hc0 = A.cpu().numpy(), B.cpu().numpy(), C.cpu().numpy(), C0.cpu().numpy()
+    al, be = alpha.cpu().numpy(), beta.cpu().numpy()
+    for i in np.random.default_rng(11).choice(count, 300, replace=False):
+        a = ha[aoff[i]:aoff[i] + lda[i] * ca[i]]
+        bm = hb[boff[i]:boff[i] + ldb[i] * cb[i]]
+        want = hc0[coff[i]:coff[i] + ldc[i] * n[i]].copy()
+        oracle.dgemm(int(ta[i]), int(tb[i]), int(m[i]), int(n[i]), int(k[i]), float(al[i]), a, int(lda[i]), bm,
+                     int(ldb[i]), float(be[i]), want, int(ldc[i]))
+        got = hc[coff[i]:coff[i] + ldc[i] * n[i]]
+        if mode == "exact":
+            assert_bits_equal(got, want, f"product {i}")
+        else:
+            def view(v, rows, cols, ld):
+                return v.reshape(cols, ld)[:, :rows].T
+
+            bound = 4 * k[i] * EPS * (abs(al[i]) * np.linalg.norm(view(a, ra[i], ca[i], lda[i]))
+                                      * np.linalg.norm(view(bm, rb[i], cb[i], ldb[i]))
+                                      + abs(be[i]) * np.linalg.norm(view(hc0[coff[i]:coff[i] + ldc[i] * n[i]],
+                                                                         m[i], n[i], ldc[i])))
+            err = np.linalg.norm(view(got, m[i], n[i], ldc[i]) - view(want, m[i], n[i], ldc[i]))
+            assert err <= bound, (i, err, bound)
+            # rows between m and ldc are never written
+            pad = view(got, ldc[i], n[i], ldc[i])[m[i]:]
+            assert np.array_equal(pad, view(hc0[coff[i]:coff[i] + ldc[i] * n[i]], ldc[i], n[i], ldc[i])[m[i]:])
