@@ -11,6 +11,8 @@ on the batch size (the oracle is far too slow for every matrix):
 * 200,000 FP64 products C = alpha op(A) op(B) + beta C, m, n, k ~ U{8..128}:
   a random sample of 300 products against the oracle, bit for bit in exact
   mode and within 4 k eps (|alpha| ||A|| ||B|| + |beta| ||C0||) in FAST mode.
+* 50,000 SPD solves of order 32 and their adjoint: a random sample of 200
+  matrices bit for bit against the oracle (factor, x, dL/dA, dL/db).
 """
 import numpy as np
 import pytest
@@ -145,3 +147,51 @@ def test_gemm_200k_sample_against_oracle(mode):
             # rows between m and ldc are never written
             pad = view(got, ldc[i], n[i], ldc[i])[m[i]:]
             assert np.array_equal(pad, view(hc0[coff[i]:coff[i] + ldc[i] * n[i]], ldc[i], n[i], ldc[i])[m[i]:])
+
+
+def test_spd_solve_adjoint_50k_n32_sample_against_oracle():
+    """The differentiated SPD-solve workload at full size: 50,000 matrices of
+    order 32 (A = B B^T + 32 I), forward exact SPD solve, then the adjoint
+    for loss = w . x.  Every info is 0, and a random sample of 200 matrices
+    gives factor, x, dL/dA and dL/db bit-identical to the oracle."""
+    hip = hipmod()
+    dev = torch.device("cuda")
+    count, n = 50_000, 32
+    g = torch.Generator(device=dev)
+    g.manual_seed(4)
+    bm = torch.randn((count, n, n), dtype=torch.float64, device=dev, generator=g)
+    a = torch.bmm(bm, bm.transpose(1, 2).contiguous()) + n * torch.eye(n, dtype=torch.float64, device=dev)
+    del bm
+    A = a.transpose(1, 2).contiguous().reshape(-1)  # column-major per matrix (symmetric anyway)
+    A0 = A.clone()
+    x = torch.randn(count * n, dtype=torch.float64, device=dev, generator=g)
+    b0 = x.clone()
+    w = torch.randn(count * n, dtype=torch.float64, device=dev, generator=g)
+    abar = torch.zeros(count * n * n, dtype=torch.float64, device=dev)
+    bbar = torch.zeros(count * n, dtype=torch.float64, device=dev)
+    idx = torch.arange(count, dtype=torch.int64, device=dev)
+    mat = lambda t: (t.data_ptr() + 8 * n * n * idx).contiguous()  # noqa: E731
+    vec = lambda t: (t.data_ptr() + 8 * n * idx).contiguous()  # noqa: E731
+    n_t = torch.full((count,), n, dtype=torch.int32, device=dev)
+    info = torch.full((count,), 99, dtype=torch.int32, device=dev)
+    h = hip.Handle()
+    h.bind_stream()
+    h.set_math_mode(hip.MATH_EXACT)
+    hip.dposv_vbatched(n_t, mat(A), n_t, vec(x), info, n, handle=h)
+    info2 = torch.full((count,), 99, dtype=torch.int32, device=dev)
+    hip.dposv_backward_vbatched(n_t, mat(A), n_t, vec(x), vec(w), mat(abar), n_t, vec(bbar), info2, n, handle=h)
+    torch.cuda.synchronize()
+    h.close()
+    assert int((info != 0).sum().item()) == 0 and int((info2 != 0).sum().item()) == 0
+    ha0, hb0, hw = A0.cpu().numpy(), b0.cpu().numpy(), w.cpu().numpy()
+    ha, hx, hab, hbb = A.cpu().numpy(), x.cpu().numpy(), abar.cpu().numpy(), bbar.cpu().numpy()
+    nn = n * n
+    for i in np.random.default_rng(13).choice(count, 200, replace=False):
+        la = ha0[i * nn:(i + 1) * nn].copy()
+        lx = hb0[i * n:(i + 1) * n].copy()
+        assert oracle.dposv(n, la, n, lx) == 0
+        assert_bits_equal(ha[i * nn:(i + 1) * nn], la, f"factor {i}")
+        assert_bits_equal(hx[i * n:(i + 1) * n], lx, f"x {i}")
+        want_abar, want_bbar = oracle.dposv_backward(n, la, n, lx, hw[i * n:(i + 1) * n])
+        assert_bits_equal(hab[i * nn:(i + 1) * nn], want_abar, f"abar {i}")
+        assert_bits_equal(hbb[i * n:(i + 1) * n], want_bbar, f"bbar {i}")
