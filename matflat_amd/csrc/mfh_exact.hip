@@ -1086,7 +1086,10 @@ constexpr int GETRI_MAXN = 64;
 constexpr int GETRI_COLS = GETRI_T / MFH_WAVE;     // column stride between a wavefront's columns
 constexpr int GETRI_JPER = GETRI_MAXN / GETRI_COLS;  // columns per wavefront
 
-size_t getri_lds_bytes(int max_n) { return 2 * (size_t)max_n * (max_n | 1) * 8; }
+// The factors (n x (n|1)) and two rows of X: a step reads only the row of X
+// finished in the previous step, and every lane keeps its own rows in
+// registers, so X never needs to be whole in LDS.
+size_t getri_lds_bytes(int max_n) { return ((size_t)max_n * (max_n | 1) + 2 * (size_t)max_n) * 8; }
 
 __global__ __launch_bounds__(GETRI_T) void k_getri_exact(MfhGetriBatch B, int count, int max_n)
 {
@@ -1119,7 +1122,7 @@ __global__ __launch_bounds__(GETRI_T) void k_getri_exact(MfhGetriBatch B, int co
         }
         const int nl = n | 1;
         double* S = lds;
-        double* X = lds + n * nl;
+        double* X = lds + n * nl;  // X[p * n + j]: row c of X in slot p = c & 1
         stage_block<GETRI_T>(S, nl, a, lda, n, n, tid);
         const int pr = i < n ? piv[i] : 0;
         const bool pok = pr >= 0 && pr < n;
@@ -1137,7 +1140,7 @@ __global__ __launch_bounds__(GETRI_T) void k_getri_exact(MfhGetriBatch B, int co
                 const int j = q + GETRI_COLS * jj;
                 if (j < n) {
                     xr[jj] = xr[jj] - 0.0;
-                    X[j * nl] = xr[jj];
+                    X[j] = xr[jj];  // row 0, slot 0
                 }
             }
         }
@@ -1156,7 +1159,7 @@ __global__ __launch_bounds__(GETRI_T) void k_getri_exact(MfhGetriBatch B, int co
 #pragma unroll
             for (int jj = 0; jj < GETRI_JPER; ++jj) {
                 const int j = q + GETRI_COLS * jj;
-                xc[jj] = X[c + (j < n ? j : 0) * nl];
+                xc[jj] = X[(c & 1) * n + (j < n ? j : 0)];
             }
 #pragma unroll
             for (int jj = 0; jj < GETRI_JPER; ++jj) {
@@ -1170,7 +1173,7 @@ __global__ __launch_bounds__(GETRI_T) void k_getri_exact(MfhGetriBatch B, int co
                 for (int jj = 0; jj < GETRI_JPER; ++jj) {
                     const int j = q + GETRI_COLS * jj;
                     xr[jj] = xr[jj] - s[jj];
-                    if (j < n) X[i + j * nl] = xr[jj];
+                    if (j < n) X[(i & 1) * n + j] = xr[jj];
                 }
             }
             __syncthreads();
@@ -1182,7 +1185,7 @@ __global__ __launch_bounds__(GETRI_T) void k_getri_exact(MfhGetriBatch B, int co
                 const int j = q + GETRI_COLS * jj;
                 if (j < n) {
                     xr[jj] = xr[jj] / d;
-                    X[n - 1 + j * nl] = xr[jj];
+                    X[((n - 1) & 1) * n + j] = xr[jj];
                 }
             }
         }
@@ -1194,7 +1197,7 @@ __global__ __launch_bounds__(GETRI_T) void k_getri_exact(MfhGetriBatch B, int co
 #pragma unroll
             for (int jj = 0; jj < GETRI_JPER; ++jj) {
                 const int j = q + GETRI_COLS * jj;
-                xc[jj] = X[jc + (j < n ? j : 0) * nl];
+                xc[jj] = X[(jc & 1) * n + (j < n ? j : 0)];
             }
 #pragma unroll
             for (int jj = 0; jj < GETRI_JPER; ++jj) {
@@ -1207,7 +1210,7 @@ __global__ __launch_bounds__(GETRI_T) void k_getri_exact(MfhGetriBatch B, int co
                 for (int jj = 0; jj < GETRI_JPER; ++jj) {
                     const int j = q + GETRI_COLS * jj;
                     xr[jj] = xr[jj] / d;
-                    if (j < n) X[i + j * nl] = xr[jj];
+                    if (j < n) X[(i & 1) * n + j] = xr[jj];
                 }
             }
             __syncthreads();
