@@ -85,9 +85,9 @@ int mfh_get_math_mode(mfh_handle* handle, int* mode);
 
 /* Tuning of the FAST Cholesky / SPD-solve kernels (no MatFlat counterpart):
  * matrices of up to 8 * row_blocks rows (n, or n + 1 for the SPD solve;
- * row_blocks 1..17) run on `waves` wavefronts per matrix: 1 or 2, or 0 for
- * the library's built-in table.  matflat_amd/autotune.py measures both and
- * prints the table. */
+ * row_blocks 1..17) run on `waves` wavefronts per matrix: 1, 2 or 4 (2 needs
+ * row_blocks >= 2, 4 needs row_blocks >= 4), or 0 for the library's built-in
+ * table.  matflat_amd/autotune.py measures every choice per bucket. */
 int mfh_set_fast_waves(mfh_handle* handle, int row_blocks, int waves);
 int mfh_get_fast_waves(mfh_handle* handle, int row_blocks, int* waves);
 

@@ -76,7 +76,7 @@ def main(argv=None) -> int:
         nmin, nmax = max(1, 8 * (rb - 1)), min(8 * rb - 1, _lib.EXACT_MAX_N)
         batch = spd_batch(args.count, nmin, nmax, seed=rb)
         res = {}
-        for waves in ((1, 2) if rb >= 2 else (1,)):
+        for waves in [w for w in (1, 2, 4) if w == 1 or (w == 2 and rb >= 2) or (w == 4 and rb >= 4)]:
             res[waves] = time_bucket(h, rb, waves, batch, nmax, args.steps)
         h.set_fast_waves(rb, 0)
         best = min(res, key=res.get)

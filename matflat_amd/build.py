@@ -38,8 +38,17 @@ FLAGS = [
 ]
 
 
+# Per-source flags.  The FAST Cholesky kernels select a register-tile block
+# by a uniform switch; SimplifyCFG's common-code sinking would merge the
+# cases into one access at a computed index and push the tile to scratch.
+SOURCE_FLAGS = {
+    "mfh_fast.hip": ["-mllvm", "-simplifycfg-sink-common=false"],
+    "mfh_fast_posv.hip": ["-mllvm", "-simplifycfg-sink-common=false"],
+}
+
+
 def _compile(src: str, obj: str, extra: list[str], verbose: bool) -> None:
-    cmd = [HIPCC, *FLAGS, "-c", *extra, src, "-o", obj]
+    cmd = [HIPCC, *FLAGS, *SOURCE_FLAGS.get(os.path.basename(src), []), "-c", *extra, src, "-o", obj]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

@@ -173,8 +173,9 @@ int mfh_set_fast_waves(mfh_handle* h, int row_blocks, int waves)
 {
     MFH_REQUIRE(h != nullptr, 1, "handle must not be null");
     MFH_REQUIRE(row_blocks >= 1 && row_blocks <= 17, 2, "row_blocks must be in [1, 17]");
-    MFH_REQUIRE(waves >= 0 && waves <= 2 && !(waves == 2 && row_blocks < 2), 3,
-                "waves must be 0 (default), 1 or 2 (2 needs row_blocks >= 2)");
+    MFH_REQUIRE((waves == 0 || waves == 1 || waves == 2 || waves == 4) && !(waves == 2 && row_blocks < 2) &&
+                    !(waves == 4 && row_blocks < 4),
+                3, "waves must be 0 (default), 1, 2 or 4 (2 needs row_blocks >= 2, 4 needs row_blocks >= 4)");
     h->fast_waves[row_blocks] = waves;
     return MFH_OK;
 }

@@ -5,36 +5,46 @@
 // mfh_fast_posv.hip (the SPD solve).
 //
 // The exact kernels (mfh_exact.hip) reproduce MatFlat's operation order bit
-// for bit, which forces one unfused multiply and one add per product and a
-// dot-product (Crout) schedule whose parallelism shrinks to the rows below
-// the current column.  FAST mode gives up bit-identity for throughput: the
-// results match MatFlat to rounding (backward error of order n*eps) and are
-// checked against the oracle by residual in the tests.
+// for bit (Factorization.Cholesky.cs:114-168), which forces one unfused
+// multiply and one add per product and a dot-product schedule.  FAST mode
+// gives up bit-identity for throughput: results match MatFlat to rounding
+// (backward error of order n*eps), checked against the oracle by residual.
 //
-// Design (one workgroup of NW = 1 or 2 wavefronts per matrix, at most 8*RB
-// rows, RB <= 17):
-//  * right-looking Cholesky on the unscaled (LDL^T-like) trailing matrix:
-//    at step k every trailing element takes a_ij -= a_ik a_jk / a_kk with
-//    FMA; column k+1 is updated and published first (look-ahead) so its
-//    publication overlaps the rest of the update.  The step's critical path
-//    holds one reciprocal, no square root: L[i,k] = a_ik / sqrt(a_kk) is
-//    applied once, in a final pass over the registers;
-//  * the lower triangle lives in VGPRs, block-cyclic over an 8 x 8 lane
-//    grid: lane (lr, lc) of wave w owns elements (lr + 8a, lc + 8b) for
-//    block rows a = w (mod NW) and b <= a.  Two wavefronts per matrix halve
-//    the registers per lane (four were measured 2x slower: every step then
-//    waits on a barrier across four wavefronts);
-//  * a published column sits in LDS in a permuted order (element i of block
-//    row a at [i % 8][(a % NW) * HALF + a / NW]), so a lane reads the row
-//    values of its wave's block rows and the column values it needs as
-//    16-byte pairs, double-buffered by step parity: one barrier per step;
-//  * the right-hand side of the SPD solve is carried as an extra row n of
-//    the triangle (its scaled "L[n,k]" is the forward substitution), and the
-//    back substitution runs as a bottom-up column sweep through LDS;
-//  * loads of the upper triangle and stores of L (and of the zeroed strict
+// Design: blocked right-looking Cholesky, one workgroup of NW (1, 2 or 4)
+// wavefronts per matrix of at most 8*RB rows (RB <= 17).
+//  * The lower triangle lives in VGPRs, block-cyclic over an 8 x 8 lane
+//    grid: lane (lr, lc) of wave w owns elements (lr + 8a, lc + 8b) of the
+//    blocks (a, b), b <= a, of its block rows a = w (mod NW).
+//  * Panel p (columns 8p .. 8p+7): the owners write its blocks to an LDS
+//    panel buffer; wavefront 0 reads it back one row per lane (two or three
+//    rows per lane past 64 and 128 rows) and factors the eight columns with v_readlane
+//    broadcasts -- the pivot and the panel rows are read from the lanes
+//    that hold them, so a column costs no LDS round trip and no barrier;
+//    it writes the scaled L panel back.  Every wave then stores its rows of
+//    the L panel to HBM straight from LDS (so the writes of L spread over
+//    the factorization) and applies the rank-8 update A22 -= L21 L21^T to
+//    the blocks it owns, as eight rank-1 rounds whose LDS operand loads are
+//    issued together.  Two barriers per panel; the panel buffer is
+//    double-buffered by panel parity.  The register tile is never written
+//    at a panel-dependent position: it keeps each panel as it stood before
+//    factoring, P_Jp, with L_Jp = P_Jp L_pp^-T.
+//  * The SPD solve carries b as an extra row n of the triangle: the panel
+//    factorization turns that row into y = L^-1 b.  The back substitution
+//    L^T x = y runs by block columns K, descending, on the kept P blocks:
+//    L_KK^T x_K = y_K - L_KK^-1 z_K with z_K = sum over J > K of P_JK^T x_J.
+//    Every wave sums its blocks' part of z_K, the sums meet in LDS (one
+//    barrier per block column) and every wave then solves the two 8 x 8
+//    triangular systems with the diagonal block kept in LDS, one lane per
+//    row and v_readlane broadcasts; x stays in each wave's own LDS copy.
+//  * A pivot that is not positive stops the factorization (info = j + 1):
+//    the columns of L of the panels before it have been written, x is left
+//    unchanged.
+//  * Loads of the upper triangle and stores of L (and of the zeroed strict
 //    upper triangle) touch each HBM byte once, eight contiguous doubles per
 //    lane group, through 32-bit offsets from a wave-uniform base.
 #pragma once
+#include <type_traits>
+
 #include "mfh_device.h"
 #include "mfh_internal.h"
 
@@ -56,7 +66,7 @@ __device__ __forceinline__ void gstore(GPtr base, int off, double v) { *(GPtr)((
 // the register index of block (m-th owned row, column block bb).
 template <int RB, int NW, int W>
 struct Own {
-    static constexpr int na = (RB - W + NW - 1) / NW;
+    static constexpr int na = (RB - W + NW - 1) / NW > 0 ? (RB - W + NW - 1) / NW : 0;
     __host__ __device__ __forceinline__ static constexpr int arow(int m) { return W + NW * m; }
     // sum of arow(t) + 1 over t < m, in closed form
     __host__ __device__ __forceinline__ static constexpr int base(int m)
@@ -65,46 +75,38 @@ struct Own {
     }
     static constexpr int size = base(na) > 0 ? base(na) : 1;
     __host__ __device__ __forceinline__ static constexpr int idx(int m, int bb) { return base(m) + bb; }
-    // first owned row index with block row >= kb
-    __host__ __device__ __forceinline__ static constexpr int first(int kb)
-    {
-        return kb <= W ? 0 : (kb - W + NW - 1) / NW;
-    }
 };
 
-// Register tile of wave W, and one value per owned block row.
 template <int RB, int NW, int W>
 using AccT = double[Own<RB, NW, W>::size];
-template <int RB, int NW, int W>
-using RowT = double[Own<RB, NW, W>::na > 0 ? Own<RB, NW, W>::na : 1];
 
-// Column buffer geometry: HALF slots per wave (even), RS per lane row.
+// LDS layout of a workgroup, in doubles.  A panel buffer holds 8*RB rows of
+// eight doubles at a row stride of PLD = 9: both access patterns -- one row
+// per lane, and eight rows each read by eight lanes -- then fall on distinct
+// banks, and every address is a per-lane base plus an immediate offset (a
+// swizzle computed per element would cost one address register each).
+constexpr int PLD = 9;
 template <int RB, int NW>
-__host__ __device__ __forceinline__ constexpr int half_slots() { return (((RB + NW - 1) / NW) + 1) & ~1; }
-template <int RB, int NW>
-__host__ __device__ __forceinline__ constexpr int row_stride() { return NW * half_slots<RB, NW>(); }
-template <int RB, int NW>
-__host__ __device__ __forceinline__ constexpr int slot_of(int a) { return (a % NW) * half_slots<RB, NW>() + a / NW; }
-
-// LDS per workgroup (doubles): col[2][8 * RS], dgl[8RB], rdg[8RB], yv[8RB],
-// xv[8RB].
-template <int RB, int NW>
-__host__ __device__ constexpr int fast_lds_doubles() { return 16 * row_stride<RB, NW>() + 32 * RB; }
-
-struct FastCtx {
-    double* col;  // two permuted column buffers
-    double* dgl;  // L[k,k]
-    double* rdg;  // 1 / L[k,k]
-    int tid, lane, lr, lc, n;
+struct FastLds {
+    static constexpr int NR = 8 * RB;
+    static constexpr int PB = PLD * NR;        // one panel buffer
+    static constexpr int NPB = NW > 1 ? 2 : 1;  // double-buffered across wavefronts
+    static constexpr int dgb = NPB * PB;       // [RB][8][8] diagonal blocks of -L
+    static constexpr int rdg = dgb + 64 * RB;  // [NR] 1 / L[j][j]
+    static constexpr int yv = rdg + NR;        // [NR] y = L^-1 b (SPD solve)
+    static constexpr int xw = yv + NR;         // [NW][NR] each wave's copy of x
+    static constexpr int part = xw + NW * NR;  // [2][NW][8] block-column sums
+    static constexpr int flag = part + 16 * NW;  // failing row + 1 (as a double)
+    static constexpr int total = flag + 2;
 };
 
-// 1/d for a positive normal d: hardware estimate and two Newton steps.
-__device__ __forceinline__ double fast_rcp(double d)
+__device__ __forceinline__ int pidx(int r, int t) { return r * PLD + t; }
+
+__device__ __forceinline__ double readlane_d(double v, int lane)
 {
-    double r = __builtin_amdgcn_rcp(d);
-#pragma unroll
-    for (int it = 0; it < 2; ++it) r = __builtin_fma(r, __builtin_fma(-d, r, 1.0), r);
-    return r;
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+    return __hiloint2double(hi, lo);
 }
 
 // 1/sqrt(d) for a positive normal d: hardware estimate, then two Newton
@@ -117,210 +119,271 @@ __device__ __forceinline__ double fast_rsqrt(double d)
     return y;
 }
 
-// Publish column col (block column KB) of wave W's rows, unscaled: the lanes
-// with lc == col % 8 write every slot of their wave's half (0 for rows above
-// block row KB), so no stale value is ever read.
-template <int RB, int NW, int W, int KB>
-__device__ __forceinline__ void fast_publish(const AccT<RB, NW, W>& acc, const FastCtx& c, int col,
-                                             double* cn)
+// Calls f(std::integral_constant<int, P>) for the uniform value P = p < N:
+// one scalar branch tree, and the body sees the panel index as a constant,
+// so every register of the tile it names is fixed.  The bodies only read
+// the tile (a body that wrote it would leave the compiler merging whole
+// copies of the tile after the switch).
+template <int N, typename F>
+__device__ __forceinline__ void for_panel(int p, F&& f)
 {
-    using O = Own<RB, NW, W>;
-    constexpr int H = half_slots<RB, NW>();
-    if (c.lc == (col & 7)) {
-        double2* p = reinterpret_cast<double2*>(cn + c.lr * row_stride<RB, NW>() + W * H);
-#pragma unroll
-        for (int m = 0; m < H; m += 2) {
-            const bool v0 = m < O::na && O::arow(m) >= KB;
-            const bool v1 = m + 1 < O::na && O::arow(m + 1) >= KB;
-            const double x0 = v0 ? acc[O::idx(v0 ? m : 0, v0 ? KB : 0)] : 0.0;
-            const double x1 = v1 ? acc[O::idx(v1 ? m + 1 : 0, v1 ? KB : 0)] : 0.0;
-            p[m / 2] = make_double2(x0, x1);
-        }
+    switch (p) {
+#define MFH_FP_CASE(Q)                                                            \
+    case Q:                                                                       \
+        if constexpr ((Q) < N) f(std::integral_constant<int, ((Q) < N ? (Q) : 0)>{}); \
+        break;
+        MFH_FP_CASE(0) MFH_FP_CASE(1) MFH_FP_CASE(2) MFH_FP_CASE(3) MFH_FP_CASE(4) MFH_FP_CASE(5)
+        MFH_FP_CASE(6) MFH_FP_CASE(7) MFH_FP_CASE(8) MFH_FP_CASE(9) MFH_FP_CASE(10) MFH_FP_CASE(11)
+        MFH_FP_CASE(12) MFH_FP_CASE(13) MFH_FP_CASE(14) MFH_FP_CASE(15) MFH_FP_CASE(16)
+#undef MFH_FP_CASE
+    default:
+        break;
     }
 }
 
-// Rank-1 update of wave W's blocks (m, bb), bb >= KB0: acc -= rs[m] * cv[bb],
-// with rs the row values already divided by a_kk.
-template <int RB, int NW, int W, int KB0>
-__device__ __forceinline__ void fast_update(AccT<RB, NW, W>& acc, const RowT<RB, NW, W>& rs, const double (&cv)[RB])
+// Writes block column P of the owned rows at or below block row P into the
+// panel buffer (entry (8a + lr, lc)).
+template <int RB, int NW, int W, int P>
+__device__ __forceinline__ void publish_p(const AccT<RB, NW, W>& acc, double* pr)
 {
     using O = Own<RB, NW, W>;
-    // constant trip counts (the bounds are resolved after unrolling)
 #pragma unroll
-    for (int m = O::first(KB0); m < O::na; ++m)
-#pragma unroll
-        for (int bb = KB0; bb < RB; ++bb) {
-            if (bb > O::arow(m)) continue;
-            acc[O::idx(m, bb)] -= rs[m] * cv[bb];
-        }
+    for (int m = 0; m < O::na; ++m)
+        if (O::arow(m) >= P) pr[PLD * 8 * O::arow(m)] = acc[O::idx(m, O::arow(m) >= P ? P : 0)];
 }
 
-template <int RB, int NW, int W, int COL>
-__device__ __forceinline__ void fast_update_col(AccT<RB, NW, W>& acc,
-                                                const RowT<RB, NW, W>& rs, const double (&cv)[RB])
+// A22 -= L21 L21^T on the owned blocks (m, bb), p < bb <= arow(m): eight
+// rank-1 rounds; a round's operands (the panel value of each owned row and
+// of each block column's row) are loaded from LDS together before its FMAs,
+// which run block column by block column under a uniform test.  The tile
+// holds -A and the panel buffer -L, so every update is one FMA with no
+// negation.
+template <int RB, int NW, int W>
+__device__ __forceinline__ void trailing(AccT<RB, NW, W>& acc, const double* pr, const double* pc, int p)
 {
     using O = Own<RB, NW, W>;
+    if constexpr (O::na > 0 && RB > 1) {
+        // a loop, not unrolled: the compiler would otherwise hoist the
+        // operand loads of every round together
+#pragma clang loop unroll(disable)
+        for (int t = 0; t < 8; ++t) {
+            double rv[O::na];
+            double cv[RB];
 #pragma unroll
-    for (int m = O::first(COL); m < O::na; ++m) acc[O::idx(m, COL)] -= rs[m] * cv[COL];
-}
-
-// Steps k = 8*KB .. min(8*KB + 8, n) - 1, then the next block column.  Step
-// k reads column k (unscaled, with a_kk at row k) from buffer k & 1, updates
-// the column k+1 belongs to (block column KB, or KB+1 at the block's last
-// step) and publishes it into the other buffer, then updates the rest of
-// the trailing matrix.  Both candidate look-ahead block columns are updated
-// before the (uniform) branch, so the bulk update is one straight path.  At
-// the very last step the look-ahead publishes a column nobody reads.  The
-// first pivot that is not positive is recorded in fail (the factorization
-// goes on with garbage, which the caller discards).
-template <int RB, int NW, int W, int KB>
-__device__ __forceinline__ void fast_block_steps(AccT<RB, NW, W>& acc, const FastCtx& c, int& fail)
-{
-    using O = Own<RB, NW, W>;
-    constexpr int H = half_slots<RB, NW>();
-    constexpr int RS = row_stride<RB, NW>();
-    constexpr int CB = 8 * RS;
-    constexpr int M0 = O::first(KB);
-    const int kend = (8 * KB + 8 < c.n) ? 8 * KB + 8 : c.n;
-    for (int k = 8 * KB; k < kend; ++k) {
-        const double* cp = c.col + (k & 1) * CB;
-        double* cn = c.col + ((k + 1) & 1) * CB;
-        const double d = cp[(k & 7) * RS + slot_of<RB, NW>(KB)];
-        // row values of this wave's rows, column values of every block
-        RowT<RB, NW, W> rs;
-        double cv[RB];
-        {
-            const double2* p = reinterpret_cast<const double2*>(cp + c.lr * RS + W * H);
+            for (int m = 0; m < O::na; ++m) rv[m] = pr[PLD * 8 * O::arow(m) + t];
 #pragma unroll
-            for (int m = M0 & ~1; m < O::na; m += 2) {
-                const double2 t = p[m / 2];
-                if (m >= M0) rs[m] = t.x;
-                if (m + 1 < O::na) rs[m + 1 < O::na ? m + 1 : 0] = t.y;
-            }
-        }
+            for (int bb = 1; bb < RB; ++bb) cv[bb] = pc[PLD * 8 * bb + t];
 #pragma unroll
-        for (int h = 0; h < NW; ++h) {
-            const double2* p = reinterpret_cast<const double2*>(cp + c.lc * RS + h * H);
+            for (int bb = RB - 1; bb >= 1; --bb) {
+                if (bb > p) {
 #pragma unroll
-            for (int s = 0; s < H; s += 2) {
-                const int b0 = h + NW * s, b1 = h + NW * (s + 1);
-                const bool u0 = b0 >= KB && b0 < RB, u1 = b1 >= KB && b1 < RB;
-                if (u0 || u1) {
-                    const double2 t = p[s / 2];
-                    if (u0) cv[u0 ? b0 : 0] = t.x;
-                    if (u1) cv[u1 ? b1 : 0] = t.y;
+                    for (int m = 0; m < O::na; ++m) {
+                        if (O::arow(m) >= bb) {
+                            double& a = acc[O::idx(m, O::arow(m) >= bb ? bb : 0)];
+                            a = __builtin_fma(rv[m], cv[bb], a);
+                        }
+                    }
                 }
             }
         }
-        fail = (fail == 0 && !(d > 0.0)) ? k + 1 : fail;
-        const double rk = fast_rcp(d);
-        if (W == 0 && c.lane == 0) {
-            const double y = fast_rsqrt(d);
-            c.rdg[k] = y;
-            c.dgl[k] = d * y;
-        }
-#pragma unroll
-        for (int m = M0; m < O::na; ++m) rs[m] *= rk;
-        // Rows need no mask: an element (i, j) with i <= k < j lies in the
-        // unused upper part of a diagonal block, and rows beyond the matrix
-        // are never stored.  Finished columns j <= k must not change.
-        if (c.lc + 8 * KB <= k) cv[KB] = 0.0;
-        fast_update_col<RB, NW, W, KB>(acc, rs, cv);
-        if ((k & 7) != 7) {
-            fast_publish<RB, NW, W, KB>(acc, c, k + 1, cn);
-            if constexpr (KB + 1 < RB) fast_update_col<RB, NW, W, KB + 1>(acc, rs, cv);
-        } else {
-            if constexpr (KB + 1 < RB) {
-                fast_update_col<RB, NW, W, KB + 1>(acc, rs, cv);
-                fast_publish<RB, NW, W, KB + 1>(acc, c, k + 1, cn);
-            }
-        }
-        fast_update<RB, NW, W, KB + 2>(acc, rs, cv);
-        __syncthreads();
-    }
-    if constexpr (KB + 1 < RB) {
-        if (8 * KB + 8 < c.n) fast_block_steps<RB, NW, W, KB + 1>(acc, c, fail);
     }
 }
 
-// Back substitution L^T x = y, columns j = min(8*KB+7, n-1) .. 8*KB
-// (descending), then block KB-1: x_j = y_j / L[j,j], then y_r -= L[j,r] x_j
-// for r < j by the 8 lanes that hold row j.
-template <int RB, int NW, int W, int KB>
-__device__ __forceinline__ void fast_back_steps(const AccT<RB, NW, W>& acc, const FastCtx& c,
-                                                double* yv, double* xv)
+// Factors panel p in wavefront 0: rows 8p + 64k + lane for k < R (the
+// rows of the panel: R = 1, 2 or 3 per lane), columns 8p .. 8p + 7.
+// Unscaled right-looking steps with the pivot and the row values broadcast
+// by v_readlane; each column is scaled by 1/sqrt(d) as it finishes.  The
+// buffer holds the panel negated (-P) and receives -L: the negated rows back
+// and the diagonal block also to dgb[p] (-L_pp), and, for the SPD solve,
+// 1/L[j][j] in rdg and the negated row of b (row n: -y, y = L^-1 b) in yv.
+// Columns past n (nc <= t < 8) are run with a unit pivot: they hold zeros
+// and finite garbage only, and their rdg and yv entries are written as 0.
+// Returns the first failing row + 1 (or the fail passed in).
+template <int RB, int R, bool POSV>
+__device__ __forceinline__ int panel_factor(double* pb, double* dgb, double* rdg, double* yv, int p, int n, int lane,
+                                            int fail)
+{
+    constexpr int NR = 8 * RB;
+    const int nc = n - 8 * p < 8 ? n - 8 * p : 8;
+    double v[R][8];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const int r = 8 * p + 64 * k + lane;
+        const double* src = pb + PLD * (r < NR ? r : 0);
+#pragma unroll
+        for (int t = 0; t < 8; ++t) v[k][t] = src[t];
+    }
+    double ymine = 0.0;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+        // the buffer holds -a: d = a_tt, s[u] = -a_ut; each row keeps -a
+        // and ends with -L
+        double d = -readlane_d(v[0][t], t);
+        const bool live = t < nc;
+        fail = (fail == 0 && live && !(d > 0.0)) ? 8 * p + t + 1 : fail;
+        d = live ? d : 1.0;
+        double s[8];
+#pragma unroll
+        for (int u = t + 1; u < 8; ++u) s[u] = readlane_d(v[0][t], u);
+        const double y = fast_rsqrt(d);
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            const double l = v[k][t] * y;  // -L[r][8p + t]
+            const double w = l * y;
+            v[k][t] = l;
+#pragma unroll
+            for (int u = t + 1; u < 8; ++u) v[k][u] = __builtin_fma(w, s[u], v[k][u]);
+        }
+        ymine = (lane == t) ? (live ? y : 0.0) : ymine;
+    }
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const int r = 8 * p + 64 * k + lane;
+        if (r < NR) {
+            double* dst = pb + PLD * r;
+#pragma unroll
+            for (int t = 0; t < 8; ++t) dst[t] = v[k][t];
+        }
+        if constexpr (POSV) {
+            if (r == n) {
+#pragma unroll
+                for (int t = 0; t < 8; ++t) yv[8 * p + t] = t < nc ? v[k][t] : 0.0;
+            }
+        }
+    }
+    if (lane < 8) {
+        if constexpr (POSV) rdg[8 * p + lane] = ymine;
+#pragma unroll
+        for (int t = 0; t < 8; ++t) dgb[64 * p + 8 * lane + t] = v[0][t];
+    }
+    return fail;
+}
+
+// Stores the factored panel P of the owned block rows from the panel buffer:
+// L at (i, j) and 0 at (j, i) for off-diagonal blocks; in the diagonal block
+// L on and below the diagonal and 0 above it.  The row of b (row n) and the
+// padding rows are not part of A.  offl = i0 + j*lda and offu = j + i0*lda
+// for the lane's row i0 = lr of block row 0 and column j = 8P + lc.
+template <int RB, int NW, int W, int P>
+__device__ __forceinline__ void store_panel_p(GPtr A, int lda, const double* pr, const double* dg, int offl, int offu,
+                                              int nrow, bool jin, int lr, int lc)
 {
     using O = Own<RB, NW, W>;
-    const int jhi = (8 * KB + 7 < c.n - 1) ? 8 * KB + 7 : c.n - 1;
-    for (int j = jhi; j >= 8 * KB; --j) {
-        const double xj = yv[j] * c.rdg[j];
-        if (c.tid == 0) xv[j] = xj;
-        if constexpr (KB % NW == W) {
-            constexpr int m = KB / NW;
-            if (c.lr == (j & 7)) {
 #pragma unroll
-                for (int bb = 0; bb <= KB; ++bb) {
-                    const int r = c.lc + 8 * bb;
-                    if (r < j) yv[r] -= acc[O::idx(m, bb)] * xj;
-                }
+    for (int m = 0; m < O::na; ++m) {
+        const int a = O::arow(m);
+        if (a > P) {
+            if (jin && 8 * a < nrow) {
+                gstore(A, offl + 8 * a, -pr[PLD * 8 * a]);
+                gstore(A, offu + 8 * a * lda, 0.0);
             }
+        } else if (a == P) {
+            // the diagonal block, from its copy dg = dgb[P] (element (lr, lc))
+            if (jin && 8 * a < nrow) gstore(A, offl + 8 * a, lr >= lc ? -dg[8 * lr + lc] : 0.0);
         }
-        __syncthreads();
     }
-    if constexpr (KB > 0) fast_back_steps<RB, NW, W, KB - 1>(acc, c, yv, xv);
 }
 
-template <int RB, int NW, int W, int KB>
-__device__ __forceinline__ void fast_back_from(const AccT<RB, NW, W>& acc, const FastCtx& c,
-                                               double* yv, double* xv, int nb)
+// Sum over the eight lanes of a lane group (lanes 8g .. 8g + 7) through DPP:
+// quad swaps, then the mirror within each half row.
+__device__ __forceinline__ double group8_sum(double v)
 {
-    // start at the block that holds row n-1
-    if (nb == KB) {
-        fast_back_steps<RB, NW, W, KB>(acc, c, yv, xv);
-        return;
-    }
-    if constexpr (KB > 0) fast_back_from<RB, NW, W, KB - 1>(acc, c, yv, xv, nb);
+    auto dpp = [](double x, int ctrl) {
+        const int lo = __double2loint(x), hi = __double2hiint(x);
+        int rl, rh;
+        switch (ctrl) {
+        case 0:
+            rl = __builtin_amdgcn_update_dpp(0, lo, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
+            rh = __builtin_amdgcn_update_dpp(0, hi, 0xB1, 0xF, 0xF, false);
+            break;
+        case 1:
+            rl = __builtin_amdgcn_update_dpp(0, lo, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
+            rh = __builtin_amdgcn_update_dpp(0, hi, 0x4E, 0xF, 0xF, false);
+            break;
+        default:
+            rl = __builtin_amdgcn_update_dpp(0, lo, 0x141, 0xF, 0xF, false);  // row_half_mirror
+            rh = __builtin_amdgcn_update_dpp(0, hi, 0x141, 0xF, 0xF, false);
+            break;
+        }
+        return __hiloint2double(rh, rl);
+    };
+    v += dpp(v, 0);
+    v += dpp(v, 1);
+    v += dpp(v, 2);
+    return v;
+}
+
+// Contribution of the owned blocks (m, K), arow(m) > K, to -z_K: lane
+// (lr, lc) adds (-P)[8a + lr][8K + lc] x[8a + lr], where P is the tile as it
+// stood before panel K was factored.
+template <int RB, int NW, int W, int K>
+__device__ __forceinline__ double colsum_p(const AccT<RB, NW, W>& acc, const double* xr)
+{
+    using O = Own<RB, NW, W>;
+    double s = 0.0;
+#pragma unroll
+    for (int m = O::na - 1; m >= 0; --m)
+        if (O::arow(m) > K) s = __builtin_fma(acc[O::idx(m, O::arow(m) > K ? K : 0)], xr[8 * O::arow(m)], s);
+    return s;
+}
+
+// Pointer read from a pointer array at a uniform index, made provably
+// uniform so that it sits in scalar registers.
+template <typename T>
+__device__ __forceinline__ T* uniform_ptr(T* p)
+{
+    const unsigned long long v = (unsigned long long)p;
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+    return (T*)(((unsigned long long)hi << 32) | lo);
 }
 
 // One matrix, wave W's share.
 template <int RB, int NW, int W, bool POSV>
-__device__ __forceinline__ void fast_matrix(const MfhCholBatch& B, FastCtx& c, int b, double* yv, double* xv)
+__device__ __forceinline__ void fast_matrix(const MfhCholBatch& B, double* lds, int b, int lane0)
 {
     using O = Own<RB, NW, W>;
-    const int lr = c.lr, lc = c.lc;
+    using LY = FastLds<RB, NW>;
+    constexpr int NR = 8 * RB;
+    const int lr0 = lane0 & 7, lc0 = lane0 >> 3;
     const int n = __builtin_amdgcn_readfirstlane(B.n[b]);
     const int lda = __builtin_amdgcn_readfirstlane(B.lda[b]);
-    c.n = n;
-    GPtr A = (GPtr)B.a[b];
+    const int rows = POSV ? n + 1 : n;
+    const int np = (n + 7) >> 3;  // panels
+    GPtr A = (GPtr)uniform_ptr(B.a[b]);
+    double* rdg = lds + LY::rdg;
+    double* yv = lds + LY::yv;
 
     // Load element (i, j), i >= j: the upper-triangle entry A[j + i*lda].
-    // Offsets are clamped into the matrix so every load is unconditional,
-    // and the value selected.
-    double acc[O::size];
+    // A row past the matrix reads element j of column 0 (in bounds: j < n)
+    // and keeps 0; the diagonal block clamps j to i.
+    int lr = lr0, lc = lc0;
+    asm volatile("" : "+v"(lr), "+v"(lc));
+    AccT<RB, NW, W> acc;
 #pragma unroll
     for (int m = 0; m < O::na; ++m) {
         const int i = lr + 8 * O::arow(m);
-        const int ic = i < n ? i : n - 1;
+        const bool rin = i < n;
+        const int roff = rin ? i * lda : 0;
 #pragma unroll
         for (int bb = 0; bb < RB; ++bb) {
             if (bb > O::arow(m)) continue;
             const int j = lc + 8 * bb;
-            const int jc = j < ic ? j : ic;
-            const double v = gload(A, jc + ic * lda);
-            acc[O::idx(m, bb)] = (i < n && j <= i) ? v : 0.0;
+            if (bb < O::arow(m)) {
+                const double v = gload(A, j + roff);
+                acc[O::idx(m, bb)] = rin ? -v : 0.0;
+            } else {
+                const int jc = j < i ? j : i;
+                const double v = gload(A, (rin ? jc : 0) + roff);
+                acc[O::idx(m, bb)] = (rin && j <= i) ? -v : 0.0;
+            }
         }
-        // one block row of loads at a time bounds the live addresses
+        // one block row of addresses live at a time
         __builtin_amdgcn_sched_barrier(0);
     }
-    if (POSV) {
+    if constexpr (POSV) {
         // Row n of the triangle holds b (lanes with lr == n % 8, block n / 8).
-        GPtr X = (GPtr)B.x[b];
-        double xb[RB];
-#pragma unroll
-        for (int bb = 0; bb < RB; ++bb) {
-            const int j = lc + 8 * bb;
-            xb[bb] = gload(X, j < n ? j : n - 1);
-        }
+        GPtr X = (GPtr)uniform_ptr(B.x[b]);
         const bool rown = lr == (n & 7);
 #pragma unroll
         for (int m = 0; m < O::na; ++m) {
@@ -329,117 +392,146 @@ __device__ __forceinline__ void fast_matrix(const MfhCholBatch& B, FastCtx& c, i
             for (int bb = 0; bb < RB; ++bb) {
                 if (bb > O::arow(m)) continue;
                 const int j = lc + 8 * bb;
-                acc[O::idx(m, bb)] = (rown && j < n) ? xb[bb] : acc[O::idx(m, bb)];
+                const double xb = gload(X, j < n ? j : n - 1);
+                acc[O::idx(m, bb)] = (rown && j < n) ? -xb : acc[O::idx(m, bb)];
             }
         }
     }
-    fast_publish<RB, NW, W, 0>(acc, c, 0, c.col);
-    __syncthreads();
+    // per-lane bases of the panel-buffer reads and of the stores of L
+    const int offl0 = lr + lc * lda, offu0 = lc + lr * lda;
+    const int nrow = n - lr, ncol = n - lc;
+
     int fail = 0;
-    fast_block_steps<RB, NW, W, 0>(acc, c, fail);
-    fail = __builtin_amdgcn_readfirstlane(fail);
-    if (fail) {
-        // FAST mode leaves a matrix that is not positive definite unchanged.
-        if (c.tid == 0) B.info[b] = fail;
-        return;
-    }
-    // Scale the columns: a_ij / L[j,j] (the diagonal is stored from dgl).
-#pragma unroll
-    for (int bb = 0; bb < RB; ++bb) {
-        if (bb > O::arow(O::na - 1)) continue;
-        const double r = c.rdg[lc + 8 * bb];
-#pragma unroll
-        for (int m = 0; m < O::na; ++m)
-            if (bb <= O::arow(m)) acc[O::idx(m, bb <= O::arow(m) ? bb : 0)] *= r;
-    }
-    // Store L, the diagonal and the zeroed strict upper triangle: in a
-    // diagonal block each lane writes its own (i, j); off-diagonal blocks
-    // write L at (i, j) and 0 at (j, i).  One exec-masked region per block row.
-#pragma unroll
-    for (int m = 0; m < O::na; ++m) {
-        const int a = O::arow(m);
-        const int i = lr + 8 * a;
-        if (i < n) {
-#pragma unroll
-            for (int bb = 0; bb < RB; ++bb) {
-                if (bb >= a) continue;
-                const int j = lc + 8 * bb;
-                gstore(A, i + j * lda, acc[O::idx(m, bb)]);
-                gstore(A, j + i * lda, 0.0);
-            }
-            const int j = lc + 8 * a;
-            if (j < n) {
-                const double v = acc[O::idx(m, a)];
-                gstore(A, i + j * lda, (i > j) ? v : (i == j ? c.dgl[i] : 0.0));
-            }
-        }
-        __builtin_amdgcn_sched_barrier(0);
-    }
-    if (c.tid == 0) B.info[b] = 0;
-    if (POSV) {
-        // y = row n of the triangle, then the Lower/Trans sweep
-        // (Blas.SolveTriangular.cs:224-231 order) with x in xv.
-        if (lr == (n & 7)) {
-#pragma unroll
-            for (int m = 0; m < O::na; ++m) {
-                if (O::arow(m) != (n >> 3)) continue;
-#pragma unroll
-                for (int bb = 0; bb < RB; ++bb) {
-                    if (bb > O::arow(m)) continue;
-                    const int j = lc + 8 * bb;
-                    if (j < n) yv[j] = acc[O::idx(m, bb)];
-                }
-            }
+    // not unrolled: one copy of the panel step keeps the code small
+#pragma clang loop unroll(disable)
+    for (int p = 0; p < np; ++p) {
+        double* pb = lds + (LY::NPB > 1 ? (p & 1) * LY::PB : 0);
+        const double* pr = pb + PLD * lr + lc;  // rows 8a + lr, column lc (publish, stores)
+        for_panel<RB>(p, [&](auto P) { publish_p<RB, NW, W, decltype(P)::value>(acc, pb + PLD * lr + lc); });
+        __syncthreads();
+        if constexpr (W == 0) {
+            // the panel is the critical path of the whole matrix: its
+            // dependent chain issues ahead of other waves' bulk updates
+            __builtin_amdgcn_s_setprio(3);
+            constexpr int RMAX = (NR + 63) / 64;
+            const int rp = rows - 8 * p;  // rows of the panel
+            if (RMAX >= 3 && rp > 128)
+                fail = panel_factor<RB, (RMAX >= 3 ? 3 : RMAX), POSV>(pb, lds + LY::dgb, rdg, yv, p, n, lane0, fail);
+            else if (RMAX >= 2 && rp > 64)
+                fail = panel_factor<RB, (RMAX >= 2 ? 2 : 1), POSV>(pb, lds + LY::dgb, rdg, yv, p, n, lane0, fail);
+            else
+                fail = panel_factor<RB, 1, POSV>(pb, lds + LY::dgb, rdg, yv, p, n, lane0, fail);
+            if (lane0 == 0) lds[LY::flag] = (double)fail;
+            __builtin_amdgcn_s_setprio(0);
         }
         __syncthreads();
-        fast_back_from<RB, NW, W, RB - 1>(acc, c, yv, xv, (n - 1) >> 3);
-        GPtr Xo = (GPtr)B.x[b];
-        for (int r = c.tid; r < n; r += 64 * NW) gstore(Xo, r, xv[r]);
+        fail = __builtin_amdgcn_readfirstlane((int)lds[LY::flag]);
+        if (fail) break;
+        const int offl = offl0 + 8 * p * lda, offu = offu0 + 8 * p;
+        const bool jin = 8 * p < ncol;
+        for_panel<RB>(p, [&](auto P) {
+            store_panel_p<RB, NW, W, decltype(P)::value>(A, lda, pr, lds + LY::dgb + 64 * decltype(P)::value, offl,
+                                                         offu, nrow, jin, lr, lc);
+        });
+        trailing<RB, NW, W>(acc, pb + PLD * lr, pb + PLD * lc, p);
+    }
+    if (fail) {
+        // columns of L before the failing panel have been written
+        if (W == 0 && lane0 == 0) B.info[b] = fail;
+        return;
+    }
+    if (W == 0 && lane0 == 0) B.info[b] = 0;
+    if constexpr (POSV) {
+        // Back substitution L^T x = y by block columns K, descending.  The
+        // tile still holds each panel as it was before it was factored
+        // (P_JK, with L_JK = P_JK L_KK^-T), so
+        //   L_KK^T x_K = y_K - L_KK^-1 z_K,   z_K = sum over J > K of P_JK^T x_J.
+        double* xw = lds + LY::xw + W * NR;
+        // a chain of dependent steps per block column
+        __builtin_amdgcn_s_setprio(2);
+        for (int i = lane0; i < NR; i += 64) xw[i] = 0.0;
+        const double* dgb = lds + LY::dgb;
+        double* part = lds + LY::part;
+        const int t = lane0 & 7;
+#pragma clang loop unroll(disable)
+        for (int K = np - 1; K >= 0; --K) {
+            double s = 0.0;
+            for_panel<RB>(K, [&](auto KK) { s = colsum_p<RB, NW, W, decltype(KK)::value>(acc, xw + lr); });
+            s = group8_sum(s);
+            double* pk = part + (K & 1) * 8 * NW;
+            if (lr == 0) pk[8 * W + lc] = s;
+            if constexpr (NW > 1) __syncthreads();
+            // the 8 x 8 diagonal block, one column (and row) per lane t
+            double z = 0.0;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) z += pk[8 * w + t];
+            const double rd = rdg[8 * K + t];
+            double lrow[8], lcol[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                lrow[k] = dgb[64 * K + 8 * t + k];  // -L[t][k]
+                lcol[k] = dgb[64 * K + 8 * k + t];  // -L[k][t]
+            }
+            // The tile, dgb and yv hold negated values (-P, -L_KK, -y), so z
+            // here is -z and the forward sweep gives -w; then
+            // y - w = (-w) - (-y).  Columns past n have rd = 0 and yv = 0,
+            // so their x is 0.
+            double wmine = 0.0;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const double v = z * rd;
+                wmine = (t == k) ? v : wmine;
+                z = __builtin_fma(lrow[k], readlane_d(v, k), z);
+            }
+            double r = wmine - yv[8 * K + t];
+            double xmine = 0.0;
+#pragma unroll
+            for (int k = 7; k >= 0; --k) {
+                const double v = r * rd;
+                xmine = (t == k) ? v : xmine;
+                r = __builtin_fma(lcol[k], readlane_d(v, k), r);
+            }
+            if (lane0 < 8) xw[8 * K + lane0] = xmine;
+        }
+        __builtin_amdgcn_s_setprio(0);
+        if constexpr (W == 0) {
+            GPtr Xo = (GPtr)uniform_ptr(B.x[b]);
+            for (int r = lane0; r < n; r += 64) gstore(Xo, r, xw[r]);
+        }
     }
 }
 
-// Waves per SIMD the register allocation of the one-wavefront kernel must
-// leave room for.  Where the compiler's own choice lands just above a
-// register-file step (256, 168 or 128 registers), a cap that costs a few
-// spilled registers doubles (or raises by a half) the matrices in flight
-// per CU: at 8*RB = 56 rows 132 -> 128 registers (2 spill instructions), at
-// 72 rows 182 -> 168 (6), at 96 rows 284 -> 256 (9), at 104 rows 310 ->
-// 256 (268).  Measured on MI355X (8,000-9,000 SPD solves per bucket): 49..56
-// rows 0.188 -> 0.170 ms, 65..72 0.315 -> 0.27, 89..96 0.76 (two
-// wavefronts) -> 0.50, 97..104 0.63 -> 0.56.  Caps at 64, 80 and 88 rows
-// (86 to 869 spill instructions) made those buckets 1.2x to 2.4x slower.
+template <int RB, int NW, int W, bool POSV>
+__device__ __forceinline__ void fast_wave(const MfhCholBatch& B, double* lds, int b, int lane, int w)
+{
+    if constexpr (W + 1 < NW) {
+        if (w == W) fast_matrix<RB, NW, W, POSV>(B, lds, b, lane);
+        else fast_wave<RB, NW, W + 1, POSV>(B, lds, b, lane, w);
+    } else {
+        fast_matrix<RB, NW, W, POSV>(B, lds, b, lane);
+    }
+}
+
+// Waves per SIMD the register allocation must leave room for: the smallest
+// allocation whose register-file step still fits the tile.
 template <int RB, int NW>
 __host__ __device__ constexpr int fast_min_waves()
 {
-    return NW > 1 ? 1 : RB == 7 ? 4 : RB == 9 ? 3 : (RB == 12 || RB == 13) ? 2 : 1;
+    return 1;
 }
 
 template <int RB, int NW, bool POSV>
 __global__ __launch_bounds__(64 * NW, (fast_min_waves<RB, NW>())) void k_chol_fast(MfhCholBatch B, MfhQueue q)
 {
     extern __shared__ double lds[];
-    constexpr int NR = 8 * RB;
-    FastCtx c;
-    c.col = lds;
-    c.dgl = lds + 16 * row_stride<RB, NW>();
-    c.rdg = c.dgl + NR;
-    double* yv = c.rdg + NR;
-    double* xv = yv + NR;
-    c.tid = threadIdx.x;
-    c.lane = threadIdx.x & 63;
-    c.lr = c.lane & 7;
-    c.lc = c.lane >> 3;
+    const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-
     for (int it = blockIdx.x;; it += gridDim.x) {
         // One matrix per workgroup: its id and sizes are uniform, which keeps
-        // every step and block test below a scalar branch.
+        // every panel and block test below a scalar branch.
         const int b = __builtin_amdgcn_readfirstlane(mfh_item(q, it));
         if (b < 0) break;
-        if (NW == 1 || w == 0)
-            fast_matrix<RB, NW, 0, POSV>(B, c, b, yv, xv);
-        else
-            fast_matrix<RB, NW, (NW > 1 ? 1 : 0), POSV>(B, c, b, yv, xv);
+        fast_wave<RB, NW, 0, POSV>(B, lds, b, lane, w);
         __syncthreads();
     }
 }
@@ -447,7 +539,7 @@ __global__ __launch_bounds__(64 * NW, (fast_min_waves<RB, NW>())) void k_chol_fa
 template <int RB, int NW, bool POSV>
 int launch_fast(mfh_handle* h, const MfhCholBatch& B, MfhQueue q, long count)
 {
-    const size_t lds = (size_t)fast_lds_doubles<RB, NW>() * 8;
+    const size_t lds = (size_t)FastLds<RB, NW>::total * 8;
     int per_cu = 0;
     MFH_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_chol_fast<RB, NW, POSV>, 64 * NW, lds));
     if (per_cu < 1) per_cu = 1;
@@ -465,6 +557,9 @@ template <int RB, bool POSV>
 int launch_fast_nw(mfh_handle* h, const MfhCholBatch& B, MfhQueue q, long count)
 {
     const int nw = h->fast_waves[RB] > 0 ? h->fast_waves[RB] : mfh_fast_default_waves(RB);
+    if constexpr (RB >= 4) {
+        if (nw == 4) return launch_fast<RB, 4, POSV>(h, B, q, count);
+    }
     if constexpr (RB >= 2) {
         if (nw == 2) return launch_fast<RB, 2, POSV>(h, B, q, count);
     }

@@ -85,7 +85,7 @@ class Handle:
         return m.value
 
     def set_fast_waves(self, row_blocks: int, waves: int) -> None:
-        """FAST Cholesky: wavefronts per matrix (1, 2; 0 = built-in) for up to 8*row_blocks rows."""
+        """FAST Cholesky: wavefronts per matrix (1, 2, 4; 0 = built-in) for up to 8*row_blocks rows."""
         _check(self.lib.mfh_set_fast_waves(self.h, int(row_blocks), int(waves)))
 
     def set_bucket_streams(self, streams: int) -> None:

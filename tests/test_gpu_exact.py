@@ -647,7 +647,7 @@ def test_matflat_mulmatvec_on_device():
 
 
 # ------------------------------------------------------------------ FAST math mode
-@pytest.mark.parametrize("waves", [0, 1, 2])
+@pytest.mark.parametrize("waves", [0, 1, 2, 4])
 @pytest.mark.parametrize("posv", [False, True])
 def test_fast_mode_cholesky_residuals(posv, waves):
     """FAST mode (FMA, right-looking) agrees with MatFlat to rounding: the
@@ -667,7 +667,7 @@ def test_fast_mode_cholesky_residuals(posv, waves):
     h.set_math_mode(hip.MATH_FAST)
     assert h.math_mode() == hip.MATH_FAST
     if waves:
-        for rb in range(2, 18):
+        for rb in range(2 if waves <= 2 else 4, 18):
             h.set_fast_waves(rb, waves)
             assert h.fast_waves(rb) == waves
     A, X = Packed(bufs), Packed(xs)
@@ -761,14 +761,19 @@ def test_fast_mode_bucket_streams_same_bits(use_plan):
     h.close()
 
 
-@pytest.mark.parametrize("waves", [1, 2])
-def test_fast_mode_not_positive_definite(waves):
+@pytest.mark.parametrize("waves", [1, 2, 4])
+@pytest.mark.parametrize("posv", [False, True])
+def test_fast_mode_not_positive_definite(waves, posv):
     """FAST mode reports the failing pivot like MatFlat (info = j+1 for a
-    clearly indefinite leading block) and leaves such a matrix unchanged;
-    its neighbours in the batch are factored normally."""
+    clearly indefinite leading block) and stops there: the columns of the
+    8-column panels before the failing one hold L (equal to the oracle's
+    partial factor on the rows MatFlat had finished, within rounding), the
+    right-hand side is left unchanged, and the neighbours in the batch are
+    factored normally."""
+    from helpers import view
     hip = hipmod()
-    ns = [5, 40, 70, 90, 12]
-    bad = {1: 17, 3: 60}  # matrix index -> row whose diagonal is made negative
+    ns = [5, 40, 70, 90, 12, 33]
+    bad = {1: 17, 3: 60, 5: 7}  # matrix index -> row whose diagonal is made negative
     bufs = []
     for i, n in enumerate(ns):
         b = spd_upper_buf(1300 + i, n, n)
@@ -776,22 +781,34 @@ def test_fast_mode_not_positive_definite(waves):
             j = bad[i]
             b[j + j * n] = -1.0e3 * n
         bufs.append(b)
+    xs = [np.random.default_rng(1400 + i).standard_normal(n) for i, n in enumerate(ns)]
     h = hip.Handle()
     h.bind_stream()
     h.set_math_mode(hip.MATH_FAST)
     for rb in range(2, 18):
-        h.set_fast_waves(rb, waves)
-    A = Packed(bufs)
+        if waves <= 2 or rb >= 4:
+            h.set_fast_waves(rb, waves)
+    A, X = Packed(bufs), Packed(xs)
     info = i32(np.full(len(ns), 99))
-    hip.dpotrf_vbatched(i32(ns), A.ptrs, i32(ns), info, max(ns), handle=h)
+    if posv:
+        hip.dposv_vbatched(i32(ns), A.ptrs, i32(ns), X.ptrs, info, max(ns), handle=h)
+    else:
+        hip.dpotrf_vbatched(i32(ns), A.ptrs, i32(ns), info, max(ns), handle=h)
     torch.cuda.synchronize()
     got = info.cpu().numpy()
-    ga = A.all()
+    ga, gx = A.all(), X.all()
     for i, n in enumerate(ns):
         if i in bad:
-            assert got[i] == bad[i] + 1, (i, got)
-            assert_bits_equal(ga[i], bufs[i])
+            j = bad[i]
+            assert got[i] == j + 1, (i, got)
+            want = bufs[i].copy()
+            assert oracle.dpotrf(n, want, n) == j + 1
+            c = 8 * (j // 8)  # columns of the panels before the failing one
+            lg, lw = view(ga[i], n, n, n), view(want, n, n, n)
+            if c:
+                sub_g, sub_w = np.tril(lg[:j, :c]), np.tril(lw[:j, :c])
+                assert np.max(np.abs(sub_g - sub_w)) <= 1e-12 * np.max(np.abs(sub_w)) * n, i
+            assert_bits_equal(gx[i], xs[i], f"x of failed matrix {i}")
         else:
             assert got[i] == 0
     h.close()
-
