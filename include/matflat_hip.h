@@ -238,17 +238,35 @@ int mfh_zgemm_vbatched(mfh_handle* h, int count, const int* transa, const int* t
  * the update and the test in FP64 until
  * ||b - A x||_inf / (||A||_inf ||x||_inf + ||b||_inf) <= tol_per_n * n or
  * max_iter refinements.  iters[b] = refinements taken, berr[b] = final
- * backward error; info[b] = 0, or n+1 if it did not converge.  A and b are
- * not modified; x receives the solution. */
+ * backward error; info[b] = 0, j+1 for an exactly zero pivot in column j,
+ * n+1 if the backward error did not reach the tolerance (a NaN backward
+ * error counts as not reached), or n+2 (iters[b] = -2, x untouched) if an
+ * entry of A is NaN or beyond the FP32 range -- such a matrix cannot be
+ * factored in FP32 (LAPACK dsgesv's check) and is for mfh_dgesv_vbatched.
+ * A and b are not modified; x receives the solution. */
 int mfh_dgesv_mixed_vbatched(mfh_handle* h, int count, const int* n, const double* const* a,
                              const int* lda, const double* const* b, double* const* x,
                              int max_iter, double tol_per_n, int* iters, double* berr,
                              int* info, int max_n);
 /* Plain FP64 blocked LU solve for the same inputs (the comparison path of
- * the mixed-precision solver), with berr as above. */
+ * the mixed-precision solver), with berr as above; info[b] = n+1 when the
+ * backward error is not finite. */
 int mfh_dgesv_vbatched(mfh_handle* h, int count, const int* n, const double* const* a,
                        const int* lda, const double* const* b, double* const* x,
                        double* berr, int* info, int max_n);
+/* The factorization of that solver on its own: Factorization.Lu
+ * (Factorization.Lu.cs:122-161) for n <= MFH_BLOCKED_MAX_N in FAST math --
+ * blocked right-looking partial pivoting with the FP64 trailing update on
+ * matrix cores.  P*A = L*U in place (unit lower L below the diagonal, U on
+ * and above it) and piv[i] = the original row now at row i, as MatFlat
+ * returns them.  The pivot of a column is its largest |a| to 2^-20 relative
+ * (|a| compared in FP32), lowest row on ties, so on near-ties piv may differ
+ * from MatFlat's order; the factors meet ||PA - LU|| / (||A|| n eps) <= 10.
+ * info[b] = j+1 for the first exactly zero pivot (MatFlat does not fail
+ * there either: the factorization completes), else 0.  Square matrices only;
+ * piv[b] holds n ints. */
+int mfh_dgetrf_blocked_vbatched(mfh_handle* h, int count, const int* n, double* const* a,
+                                const int* lda, int* const* piv, int* info, int max_n);
 
 /* ---- adjoint of the SPD solve ----------------------------------------------
  * No MatFlat counterpart (MatFlat has no derivatives).  For x = A^-1 b with

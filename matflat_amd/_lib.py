@@ -63,6 +63,7 @@ SIGNATURES = {
     "mfh_dgemv_vbatched": [_p, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i, _i],
     "mfh_dgesv_mixed_vbatched": [_p, _i, _p, _p, _p, _p, _p, _i, _d, _p, _p, _p, _i],
     "mfh_dgesv_vbatched": [_p, _i, _p, _p, _p, _p, _p, _p, _p, _i],
+    "mfh_dgetrf_blocked_vbatched": [_p, _i, _p, _p, _p, _p, _p, _i],
     "mfh_dposv_backward_vbatched": [_p, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i],
     "mfh_plan_create_dposv": [_p, _i, _p, _p, _p, _p, _p, _i, _p],
     "mfh_plan_create_dgemm": [_p, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i, _i, _i, _p],

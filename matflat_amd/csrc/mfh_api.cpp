@@ -496,6 +496,21 @@ int mfh_dgesv_vbatched(mfh_handle* h, int count, const int* n, const double* con
     return mfh_launch_gesv_mixed(h, count, n, a, lda, b, x, 0, 1.0, nullptr, berr, info, max_n, false);
 }
 
+int mfh_dgetrf_blocked_vbatched(mfh_handle* h, int count, const int* n, double* const* a, const int* lda,
+                                int* const* piv, int* info, int max_n)
+{
+    MFH_REQUIRE(h != nullptr, 1, "handle must not be null");
+    MFH_REQUIRE(count >= 0, 2, "count must be >= 0");
+    if (count == 0) return MFH_OK;
+    MFH_REQUIRE(n != nullptr, 3, "n must not be null");
+    MFH_REQUIRE(a != nullptr, 4, "a must not be null");
+    MFH_REQUIRE(lda != nullptr, 5, "lda must not be null");
+    MFH_REQUIRE(piv != nullptr, 6, "piv must not be null");
+    MFH_REQUIRE(info != nullptr, 7, "info must not be null");
+    MFH_REQUIRE(max_n >= 1 && max_n <= MFH_BLOCKED_MAX_N, 8, "max_n must be in [1, MFH_BLOCKED_MAX_N]");
+    return mfh_launch_getrf_blocked(h, count, n, a, lda, piv, info, max_n);
+}
+
 int mfh_dposv_backward_vbatched(mfh_handle* h, int count, const int* n, const double* const* l, const int* lda,
                                 const double* const* x, const double* const* xbar, double* const* abar,
                                 const int* ldabar, double* const* bbar, int* info, int max_n)

@@ -200,3 +200,5 @@ int mfh_launch_gesv_mixed(mfh_handle* h, int count, const int* n, const double* 
                           const int* lda, const double* const* b, double* const* x, int max_iter,
                           double tol_per_n, int* iters, double* berr, int* info, int max_n,
                           bool mixed);
+int mfh_launch_getrf_blocked(mfh_handle* h, int count, const int* n, double* const* a, const int* lda,
+                             int* const* piv, int* info, int max_n);

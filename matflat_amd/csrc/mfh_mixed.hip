@@ -84,6 +84,9 @@ struct GesvArgs {
     double* norms;     // [count][2] ||A||_inf, ||b||_inf
     int* zero;         // [count] first zero pivot (1-based) or 0
     int* state;        // [count] 0 active, 1 finished, -1 invalid
+    int factor_only;   // no right-hand side: factor, then export the factors
+    double* const* lu_out;  // factor_only: where the factors go (lda of a), and ...
+    int* const* piv_out;    // ... the row order (position -> original row)
 };
 
 // LDS layout (bytes), for row stride L = ldw:
@@ -225,31 +228,37 @@ __device__ __forceinline__ double uniform(double v)
     return __hiloint2double(hi, lo);
 }
 
-// Block-wide max of a non-negative double (all threads get the result).
+// Maximum that propagates NaN: a NaN entry makes the norm NaN (and the
+// backward error NaN, which is then reported as a failure) instead of being
+// dropped by the comparison.
+__device__ __forceinline__ double nan_max(double a, double m) { return (a > m || a != a) ? a : m; }
+
+// Block-wide max of a non-negative double, NaN propagating (all threads get
+// the result).
 template <typename T>
 __device__ __forceinline__ double block_max(const Lds<T>& L, double v)
 {
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        const double o = __shfl_xor(v, off, 64);
-        v = o > v ? o : v;
-    }
+    for (int off = 32; off > 0; off >>= 1) v = nan_max(__shfl_xor(v, off, 64), v);
     __syncthreads();
     if ((threadIdx.x & 63) == 0) L.rkey[threadIdx.x >> 6] = v;
     __syncthreads();
     double m = L.rkey[0];
 #pragma unroll
-    for (int w = 1; w < NWAVE; ++w) m = L.rkey[w] > m ? L.rkey[w] : m;
+    for (int w = 1; w < NWAVE; ++w) m = nan_max(L.rkey[w], m);
     return uniform(m);
 }
 
 // ------------------------------------------------------------------- copy
 // W[r][c] = (T) A[r + c*lda] through 128 x 32 LDS tiles (both sides
 // coalesced, eight independent loads per thread and tile), with the row sums
-// of |A| on the way.  Returns ||A||_inf (wave-uniform).
+// of |A| on the way.  Returns ||A||_inf and sets amax = max |a_ij| (both
+// wave-uniform, NaN propagating).
 template <typename T>
-__device__ __forceinline__ double copy_in(const double* A, int lda, T* W, int ldw, int n, const Lds<T>& L)
+__device__ __forceinline__ double copy_in(const double* A, int lda, T* W, int ldw, int n, const Lds<T>& L,
+                                          double& amax)
 {
+    double emax = 0.0;
     T* tile = L.ut;
     const int t = threadIdx.x;
     const int rr = t & 127, cq = t >> 7;  // read: row, column quarter
@@ -268,6 +277,7 @@ __device__ __forceinline__ double copy_in(const double* A, int lda, T* W, int ld
 #pragma unroll
             for (int h = 0; h < 8; ++h) {
                 rowsum += __builtin_fabs(v[h]);
+                emax = nan_max(__builtin_fabs(v[h]), emax);
                 tile[rr * 33 + cq + 4 * h] = (T)v[h];
             }
             __syncthreads();
@@ -285,10 +295,11 @@ __device__ __forceinline__ double copy_in(const double* A, int lda, T* W, int ld
         __syncthreads();
         if (cq == 0 && r < n) {
             const double sum = part[rr] + part[128 + rr] + part[256 + rr] + part[384 + rr];
-            best = sum > best ? sum : best;
+            best = nan_max(sum, best);
         }
         __syncthreads();
     }
+    amax = block_max(L, emax);
     return block_max(L, best);
 }
 
@@ -953,7 +964,7 @@ __device__ __forceinline__ double residual(const double* A, int lda, const doubl
         for (int q = 0; q < G; ++q) s += part[q * n + rr];
         L.rv[rr] = s;
         const double a = __builtin_fabs(s);
-        m = a > m ? a : m;
+        m = nan_max(a, m);
     }
     return block_max(L, m);
 }
@@ -969,19 +980,33 @@ __device__ __forceinline__ void gesv_factor(const GesvArgs& G, const Lds<T>& L, 
     const int ldw = G.ldw;
     const int t = threadIdx.x;
     const double* A = G.a[bi];
-    const double* bvec = G.b[bi];
+    const double* bvec = G.factor_only ? nullptr : G.b[bi];
     T* W = (T*)(G.ws + (long long)bi * G.wstride);
 
     // ||b||_inf; ||A||_inf comes with the copy
     double bn = 0.0;
-    for (int r = t; r < n; r += BT) {
+    for (int r = t; r < (G.factor_only ? 0 : n); r += BT) {
         const double v = __builtin_fabs(gmem(bvec)[r]);
-        bn = v > bn ? v : bn;
+        bn = nan_max(v, bn);
     }
     const double bnorm = block_max(L, bn);
-    const double anorm = copy_in(A, lda, W, ldw, n, L);
+    double amax = 0.0;
+    const double anorm = copy_in(A, lda, W, ldw, n, L, amax);
     for (int r = t; r < n; r += BT) L.perm[r] = r;
     __syncthreads();
+    // Mixed precision: an entry beyond the FP32 range (or a NaN) cannot be
+    // factored in FP32 (LAPACK dsgesv's dlag2s check).  Such a matrix is not
+    // solved here: info = n + 2, iters = -2, x untouched; the plain FP64
+    // solver takes it.  The test is on a wave-uniform value.
+    if (MIXED && !(amax <= 3.4028234663852886e38)) {
+        if (t == 0) {
+            G.info[bi] = n + 2;
+            G.iters[bi] = -2;
+            G.berr[bi] = __builtin_inf();
+            G.state[bi] = 1;
+        }
+        return;
+    }
 
     int zero = 0;
     for (int k0 = 0; k0 < n; k0 += NB) {
@@ -1037,21 +1062,28 @@ __device__ __forceinline__ void gesv_step(const GesvArgs& G, const Lds<T>& L, in
         L.xv[r] = v;
         gmem_w(xo)[r] = v;
         const double a = __builtin_fabs(v);
-        xm = a > xm ? a : xm;
+        xm = nan_max(a, xm);
     }
     const double xnorm = block_max(L, xm);  // its barriers also publish xv
     const double rnorm = residual<T>(A, lda, bvec, n, L);
     for (int r = t; r < n; r += BT) gmem_w(rs)[r] = L.rv[r];
     const double anorm = G.norms[2 * bi], bnorm = G.norms[2 * bi + 1];
     const double den = anorm * xnorm + bnorm;
-    const double eta = uniform(den > 0.0 ? rnorm / den : (rnorm > 0.0 ? 1.0 : 0.0));
+    // NaN norms (a NaN in A, b or x) give a NaN eta; a zero denominator
+    // (A = 0 and b = 0) gives 0 for r = 0
+    const double eta = uniform((den != den || rnorm != rnorm) ? __builtin_nan("")
+                                                              : (den > 0.0 ? rnorm / den : (rnorm > 0.0 ? 1.0 : 0.0)));
+    // A NaN backward error stops the refinement and is a failure, like one
+    // that stays above the tolerance; for the plain FP64 solve a backward
+    // error that is not finite is one.
     const bool done = !MIXED || !(eta > G.tol_per_n * n) || it >= G.max_iter;
     if (t == 0) {
         G.berr[bi] = eta;
         if (MIXED) G.iters[bi] = it;
         if (done) {
             const int zero = G.zero[bi];
-            G.info[bi] = zero ? zero : ((MIXED && eta > G.tol_per_n * n) ? n + 1 : 0);
+            const bool bad = MIXED ? !(eta <= G.tol_per_n * n) : !(eta <= __builtin_huge_val());
+            G.info[bi] = zero ? zero : (bad ? n + 1 : 0);
             G.state[bi] = 1;
         }
     }
@@ -1077,8 +1109,9 @@ __global__ __launch_bounds__(BT) void k_gesv(GesvArgs G)
         if (n < 1 || n > G.max_n) bad = -3;
         else if (G.a[bi] == nullptr) bad = -4;
         else if (lda < n || (long long)lda * n > MFH_MAX_MATRIX_ELEMS) bad = -5;
-        else if (G.b[bi] == nullptr) bad = -6;
-        else if (G.x[bi] == nullptr) bad = -7;
+        else if (!G.factor_only && G.b[bi] == nullptr) bad = -6;
+        else if (!G.factor_only && G.x[bi] == nullptr) bad = -7;
+        else if (G.factor_only && (G.lu_out[bi] == nullptr || G.piv_out[bi] == nullptr)) bad = -6;
         bad = __builtin_amdgcn_readfirstlane(bad);
         if (bad == 0) gesv_factor<T, MIXED>(G, L, bi, n, lda);
         else if (t == 0) {
@@ -1146,6 +1179,45 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(MFH_STEP_WAV
 
 __global__ void k_zero_counter(int* c) { *c = 0; }
 
+// Factor-only mode: writes the FP64 factors from the row-major workspace W
+// back to the caller's column-major matrix (through 32 x 33 LDS tiles, both
+// sides coalesced), the row order to piv and the first zero pivot to info.
+__global__ __launch_bounds__(BT) void k_gesv_export(GesvArgs G)
+{
+    __shared__ double tile[32 * 33];
+    const int t = threadIdx.x;
+    for (int bi = blockIdx.x; bi < G.count; bi += gridDim.x) {
+        const int st = __builtin_amdgcn_readfirstlane(G.state[bi]);
+        if (st == 0) {
+            const int n = __builtin_amdgcn_readfirstlane(G.n[bi]);
+            const int lda = __builtin_amdgcn_readfirstlane(G.lda[bi]);
+            const double* W = (const double*)(G.ws + (long long)bi * G.wstride);
+            double* out = G.lu_out[bi];
+            for (int r0 = 0; r0 < n; r0 += 32)
+                for (int c0 = 0; c0 < n; c0 += 32) {
+                    // read rows of W (columns fastest), write columns of out
+                    for (int e = t; e < 32 * 32; e += BT) {
+                        const int rr = e >> 5, cc = e & 31;
+                        const int r = r0 + rr, c = c0 + cc;
+                        tile[rr * 33 + cc] = (r < n && c < n) ? gmem(W)[(size_t)r * G.ldw + c] : 0.0;
+                    }
+                    __syncthreads();
+                    for (int e = t; e < 32 * 32; e += BT) {
+                        const int cc = e >> 5, rr = e & 31;
+                        const int r = r0 + rr, c = c0 + cc;
+                        if (r < n && c < n) gmem_w(out)[r + (size_t)c * lda] = tile[rr * 33 + cc];
+                    }
+                    __syncthreads();
+                }
+            const int* perm = G.perm + (long long)bi * G.ldw;
+            int* piv = G.piv_out[bi];
+            for (int r = t; r < n; r += BT) gmem_w(piv)[r] = gmem(perm)[r];
+            if (t == 0) G.info[bi] = G.zero[bi];
+        }
+        __syncthreads();
+    }
+}
+
 template <typename T, bool MIXED>
 int launch_gesv(mfh_handle* h, GesvArgs G)
 {
@@ -1165,6 +1237,16 @@ int launch_gesv(mfh_handle* h, GesvArgs G)
     st = mfh_timing_end(h);
     if (st) return st;
 
+    if (G.factor_only) {
+        per_cu = 0;
+        MFH_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_gesv_export, BT, 0));
+        if (per_cu < 1) per_cu = 1;
+        grid = (long)per_cu * h->num_cus;
+        if (grid > G.count) grid = G.count;
+        hipLaunchKernelGGL(k_gesv_export, dim3((unsigned)grid), dim3(BT), 0, h->stream, G);
+        MFH_HIP_CHECK(hipGetLastError());
+        return MFH_OK;
+    }
     const size_t slds = step_lds_bytes<T>(G.ldw);
     MFH_HIP_CHECK(hipFuncSetAttribute((const void*)k_gesv_step<T, MIXED>,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)slds));
@@ -1212,9 +1294,10 @@ GesvLayout gesv_layout(int count, int max_n, bool mixed)
 
 size_t mfh_gesv_ws_bytes(int count, int max_n, bool mixed) { return gesv_layout(count, max_n, mixed).total; }
 
-int mfh_launch_gesv_mixed(mfh_handle* h, int count, const int* n, const double* const* a, const int* lda,
-                          const double* const* b, double* const* x, int max_iter, double tol_per_n, int* iters,
-                          double* berr, int* info, int max_n, bool mixed)
+static int launch_gesv_common(mfh_handle* h, int count, const int* n, const double* const* a, const int* lda,
+                              const double* const* b, double* const* x, int max_iter, double tol_per_n, int* iters,
+                              double* berr, int* info, int max_n, bool mixed, double* const* lu_out,
+                              int* const* piv_out)
 {
     const int ldw = (max_n + 7) & ~7;
     const GesvLayout lay = gesv_layout(count, max_n, mixed);
@@ -1243,5 +1326,23 @@ int mfh_launch_gesv_mixed(mfh_handle* h, int count, const int* n, const double* 
     G.norms = (double*)(base + lay.norms);
     G.zero = (int*)(base + lay.zero);
     G.state = (int*)(base + lay.state);
+    G.factor_only = lu_out != nullptr;
+    G.lu_out = lu_out;
+    G.piv_out = piv_out;
     return mixed ? launch_gesv<float, true>(h, G) : launch_gesv<double, false>(h, G);
+}
+
+int mfh_launch_gesv_mixed(mfh_handle* h, int count, const int* n, const double* const* a, const int* lda,
+                          const double* const* b, double* const* x, int max_iter, double tol_per_n, int* iters,
+                          double* berr, int* info, int max_n, bool mixed)
+{
+    return launch_gesv_common(h, count, n, a, lda, b, x, max_iter, tol_per_n, iters, berr, info, max_n, mixed, nullptr,
+                              nullptr);
+}
+
+int mfh_launch_getrf_blocked(mfh_handle* h, int count, const int* n, double* const* a, const int* lda,
+                             int* const* piv, int* info, int max_n)
+{
+    return launch_gesv_common(h, count, n, (const double* const*)a, lda, nullptr, nullptr, 0, 1.0, nullptr, nullptr,
+                              info, max_n, false, a, piv);
 }

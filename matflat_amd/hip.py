@@ -236,6 +236,13 @@ def dgesv_vbatched(n, a_ptrs, lda, b_ptrs, x_ptrs, berr, info, max_n, handle: Ha
                                     _p(berr), _p(info), int(max_n)))
 
 
+def dgetrf_blocked_vbatched(n, a_ptrs, lda, piv_ptrs, info, max_n, handle: Handle | None = None):
+    """FAST-mode LU with partial pivoting for n <= 512 (mfh_dgetrf_blocked_vbatched)."""
+    h = handle or default_handle()
+    _check(h.lib.mfh_dgetrf_blocked_vbatched(h.h, int(n.numel()), _p(n), _p(a_ptrs), _p(lda), _p(piv_ptrs), _p(info),
+                                             int(max_n)))
+
+
 def dposv_backward_vbatched(n, l_ptrs, lda, x_ptrs, xbar_ptrs, abar_ptrs, ldabar, bbar_ptrs, info, max_n,
                             handle: Handle | None = None):
     h = handle or default_handle()

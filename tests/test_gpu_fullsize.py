@@ -13,6 +13,9 @@ on the batch size (the oracle is far too slow for every matrix):
   mode and within 4 k eps (|alpha| ||A|| ||B|| + |beta| ||C0||) in FAST mode.
 * 50,000 SPD solves of order 32 and their adjoint: a random sample of 200
   matrices bit for bit against the oracle (factor, x, dL/dA, dL/db).
+* 8,000 mixed-precision and plain FP64 LU solves, n in {128, 256, 384, 512}:
+  every backward error recomputed in FP64, a sample against LAPACK, and the
+  blocked FP64 factors of a sample by the dget01 ratio.
 """
 import numpy as np
 import pytest
@@ -195,3 +198,99 @@ def test_spd_solve_adjoint_50k_n32_sample_against_oracle():
         want_abar, want_bbar = oracle.dposv_backward(n, la, n, lx, hw[i * n:(i + 1) * n])
         assert_bits_equal(hab[i * nn:(i + 1) * nn], want_abar, f"abar {i}")
         assert_bits_equal(hbb[i * n:(i + 1) * n], want_bbar, f"bbar {i}")
+
+
+def _group_residuals(buf, offsets, ns, b, boff, x):
+    """eta = ||b - A x||_inf / (||A||_inf ||x||_inf + ||b||_inf) of every
+    solve, recomputed on the GPU in FP64 from the original A (column-major,
+    lda = n), per order."""
+    dev = torch.device("cuda")
+    eta = np.zeros(ns.size)
+    for n in sorted(set(ns.tolist())):
+        idx = np.flatnonzero(ns == n)
+        ao = torch.from_numpy(offsets[idx]).to(dev)
+        bo = torch.from_numpy(boff[idx]).to(dev)
+        ar = torch.arange(n * n, device=dev)
+        vr = torch.arange(n, device=dev)
+        for s0 in range(0, idx.size, 500):
+            sl = slice(s0, s0 + 500)
+            a = buf[ao[sl, None] + ar[None, :]].view(-1, n, n).transpose(1, 2).contiguous()  # row-major A
+            xv = x[bo[sl, None] + vr[None, :]]
+            bv = b[bo[sl, None] + vr[None, :]]
+            r = bv - torch.bmm(a, xv.unsqueeze(2)).squeeze(2)
+            den = a.abs().sum(dim=2).amax(dim=1) * xv.abs().amax(dim=1) + bv.abs().amax(dim=1)
+            eta[idx[sl]] = (r.abs().amax(dim=1) / den).cpu().numpy()
+    return eta
+
+
+def test_mixed_precision_8000_solves_backward_error():
+    """The mixed-precision workload at full size (8,000 LU solves, n uniform
+    over {128, 256, 384, 512}, cond 10^U(2,6), built by
+    bench.make_conditioned_batch with seed 3), both solvers: every solve has
+    info = 0 and a backward error, recomputed here in FP64 from the original
+    A, <= 1e-15 n; the mixed path took <= 10 refinement steps; each device
+    berr agrees with the recomputation; a sample of 50 solutions matches
+    LAPACK (numpy.linalg.solve) within cond * n * 20 eps; a sample of 24
+    matrices factored by the blocked FP64 LU meets
+    ||PA - LU||_1 / (||A||_1 n eps) <= 10."""
+    hip = hipmod()
+    dev = torch.device("cuda")
+    count, seed = 8000, 3
+    buf, offsets, ns, b, boff = bench.make_conditioned_batch(count, seed)
+    rng = np.random.default_rng(seed)
+    rng.choice(np.array(bench.MIXED_SIZES), size=count)
+    kappa = 10.0 ** rng.uniform(2.0, 6.0, size=count)
+    n_t = torch.from_numpy(ns).to(dev)
+    a_ptrs = (buf.data_ptr() + 8 * torch.from_numpy(offsets).to(dev)).contiguous()
+    b_ptrs = (b.data_ptr() + 8 * torch.from_numpy(boff).to(dev)).contiguous()
+    a0 = buf.clone()
+    sample = np.random.default_rng(17).choice(count, size=50, replace=False)
+    for mixed in (True, False):
+        x = torch.zeros_like(b)
+        x_ptrs = (x.data_ptr() + 8 * torch.from_numpy(boff).to(dev)).contiguous()
+        info = torch.full((count,), 99, dtype=torch.int32, device=dev)
+        iters = torch.full((count,), -1, dtype=torch.int32, device=dev)
+        berr = torch.full((count,), -1.0, dtype=torch.float64, device=dev)
+        if mixed:
+            hip.dgesv_mixed_vbatched(n_t, a_ptrs, n_t, b_ptrs, x_ptrs, 10, 1e-15, iters, berr, info, 512)
+        else:
+            hip.dgesv_vbatched(n_t, a_ptrs, n_t, b_ptrs, x_ptrs, berr, info, 512)
+        torch.cuda.synchronize()
+        assert bool(torch.equal(buf, a0)), "A must not be modified"
+        inf = info.cpu().numpy()
+        assert np.all(inf == 0), (mixed, np.flatnonzero(inf)[:10])
+        eta = _group_residuals(buf, offsets, ns, b, boff, x)
+        assert np.all(eta <= 1e-15 * ns), (mixed, float((eta / ns).max()))
+        be = berr.cpu().numpy()
+        assert np.all(np.abs(be - eta) <= 0.5 * eta + 4 * EPS), (mixed, float(np.abs(be - eta).max()))
+        if mixed:
+            it = iters.cpu().numpy()
+            assert np.all((it >= 0) & (it <= 10)), (int(it.min()), int(it.max()))
+        xh, bh, ah = x.cpu().numpy(), b.cpu().numpy(), buf.cpu().numpy()
+        for i in sample:
+            n = int(ns[i])
+            a = ah[offsets[i]:offsets[i] + n * n].reshape(n, n).T
+            want = np.linalg.solve(a, bh[boff[i]:boff[i] + n])
+            got = xh[boff[i]:boff[i] + n]
+            fwd = np.abs(got - want).max() / np.abs(want).max()
+            assert fwd <= kappa[i] * n * 20 * EPS, (mixed, i, n, kappa[i], fwd)
+    # the blocked FP64 factors themselves, on copies of 24 of the matrices
+    fsample = np.random.default_rng(23).choice(count, size=24, replace=False)
+    ah = a0.cpu().numpy()
+    sub = [ah[offsets[i]:offsets[i] + int(ns[i]) ** 2].copy() for i in fsample]
+    from gpu_util import Packed, i32
+    A = Packed(sub)
+    P = Packed([np.full(int(ns[i]), -1, dtype=np.int32) for i in fsample], dtype=torch.int32)
+    sn = [int(ns[i]) for i in fsample]
+    finfo = i32(np.full(len(sn), 99))
+    hip.dgetrf_blocked_vbatched(i32(sn), A.ptrs, i32(sn), P.ptrs, finfo, 512)
+    torch.cuda.synchronize()
+    assert np.all(finfo.cpu().numpy() == 0)
+    for k, (lu_buf, piv) in enumerate(zip(A.all(), P.all())):
+        n = sn[k]
+        a = sub[k].reshape(n, n).T
+        lu = lu_buf.reshape(n, n).T
+        low, up = np.tril(lu, -1) + np.eye(n), np.triu(lu)
+        assert sorted(piv.tolist()) == list(range(n))
+        ratio = np.abs(a[piv, :] - low @ up).sum(axis=0).max() / (np.abs(a).sum(axis=0).max() * n * EPS)
+        assert ratio <= 10.0, (k, n, ratio)

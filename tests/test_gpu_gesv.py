@@ -157,3 +157,86 @@ def test_gesv_argument_errors():
     with pytest.raises(hip.ArgumentError) as e:
         hip.dgesv_mixed_vbatched(one, ptr, one, ptr, ptr, 10, 0.0, one, f64([0.0]), one, 8)
     assert e.value.position == 9
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_gesv_nan_and_fp32_range(mixed):
+    """A NaN in A gives a NaN backward error, reported as a failure (info =
+    n+1 on the FP64 path), never as convergence.  An entry beyond the FP32
+    range cannot be factored in FP32: the mixed path reports info = n+2,
+    iters = -2 and leaves x untouched (LAPACK dsgesv's dlag2s check); the
+    FP64 path solves the same matrix to backward error <= n eps.  The
+    neighbours in the batch are solved."""
+    hip = hipmod()
+    n = 48
+    good = conditioned(21, n, 1e3)
+    nanm = good.copy()
+    nanm[5, 9] = np.nan
+    big = good * 1e41  # entries of this one are beyond the FP32 range (3.4e38)
+    mats = [good, nanm, big, good]
+    bufs = [col_major(m, n) for m in mats]
+    bs = [np.random.default_rng(70 + k).standard_normal(n) for k in range(4)]
+    bs[2] = bs[2] * 1e41
+    A, B = Packed(bufs), Packed(bs)
+    X = Packed([np.full(n, 7.0) for _ in range(4)])
+    info, berr, iters = i32([99] * 4), f64([-1.0] * 4), i32([-1] * 4)
+    if mixed:
+        hip.dgesv_mixed_vbatched(i32([n] * 4), A.ptrs, i32([n] * 4), B.ptrs, X.ptrs, 10, 1e-15, iters, berr, info, n)
+    else:
+        hip.dgesv_vbatched(i32([n] * 4), A.ptrs, i32([n] * 4), B.ptrs, X.ptrs, berr, info, n)
+    torch.cuda.synchronize()
+    got, be, it, xs = info.cpu().numpy(), berr.cpu().numpy(), iters.cpu().numpy(), X.all()
+    for k in (0, 3):
+        assert got[k] == 0, got
+        assert backward_error(good, xs[k], bs[k]) <= (1e-15 * n if mixed else n * EPS)
+    if mixed:
+        # the NaN entry is caught by the same range test (it is not finite)
+        assert got[1] == n + 2 and it[1] == -2, (got, it)
+        assert got[2] == n + 2 and it[2] == -2, (got, it)
+        assert np.all(xs[1] == 7.0) and np.all(xs[2] == 7.0)
+    else:
+        assert got[1] == n + 1, got
+        assert np.isnan(be[1])
+        assert got[2] == 0, got
+        assert backward_error(big, xs[2], bs[2]) <= n * EPS
+
+
+def test_getrf_blocked_factors_and_pivots():
+    """mfh_dgetrf_blocked_vbatched factors in place with MatFlat's piv
+    convention ((PA)[i, :] = A[piv[i], :], Factorization.Lu.cs:296-350):
+    piv is a permutation, |L| <= 1, ||PA - LU||_1 / (||A||_1 n eps) <= 10
+    (LAPACK dget01), padding rows untouched; a zero column gives info =
+    (first zero pivot) + 1 and the factorization still completes."""
+    hip = hipmod()
+    cases = [(1, 1, 1e1), (7, 9, 1e2), (64, 64, 1e4), (129, 131, 1e3), (256, 256, 1e5), (383, 384, 1e6),
+             (512, 512, 1e6)]
+    mats, bufs, _ = build(cases, 301)
+    sing = conditioned(9, 40, 1e2)
+    sing[:, 11] = 0.0
+    mats.append(sing)
+    bufs.append(col_major(sing, 40))
+    ns = [c[0] for c in cases] + [40]
+    ldas = [c[1] for c in cases] + [40]
+    A = Packed(bufs)
+    P = Packed([np.full(n, -1, dtype=np.int32) for n in ns], dtype=torch.int32)
+    info = i32(np.full(len(ns), 99))
+    hip.dgetrf_blocked_vbatched(i32(ns), A.ptrs, i32(ldas), P.ptrs, info, 512)
+    torch.cuda.synchronize()
+    got, ga, gp = info.cpu().numpy(), A.all(), P.all()
+    for i, (n, lda) in enumerate(zip(ns, ldas)):
+        piv = gp[i]
+        assert sorted(piv.tolist()) == list(range(n)), i
+        lu = view(ga[i], n, n, lda)
+        low = np.tril(lu, -1) + np.eye(n)
+        up = np.triu(lu)
+        # pivots are the largest |a| to 2^-20 relative
+        assert np.all(np.abs(np.tril(lu, -1)) <= 1.0 + 1e-5), i
+        pa = mats[i][piv, :]
+        ratio = np.abs(pa - low @ up).sum(axis=0).max() / (np.abs(mats[i]).sum(axis=0).max() * n * EPS)
+        assert ratio <= 10.0, (i, n, ratio)
+        for j in range(n):
+            assert np.all(np.isnan(ga[i][j * lda + n:(j + 1) * lda])), i
+        if i == len(ns) - 1:
+            assert got[i] == 12, got
+        else:
+            assert got[i] == 0, (i, got[i])
