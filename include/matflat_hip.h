@@ -151,6 +151,20 @@ int mfh_dgetrf_strided_batched(mfh_handle* h, int count, int m, int n, double* a
 int mfh_dpotrf_vbatched(mfh_handle* h, int count, const int* n, double* const* a,
                         const int* lda, int* info, int max_n);
 
+/* The SPD solve above, allocating nothing.  No counterpart in the reference:
+ * it is for callers that cannot call hipMalloc (code under hipGraph capture,
+ * tight serving loops, harnesses that forbid allocation).  Same arguments,
+ * results and errors as mfh_dposv_vbatched, for count <= MFH_NOALLOC_MAX_COUNT
+ * (a larger count returns -2).  Its scratch (the size sort's 2 ints per
+ * matrix and histograms, 8 MiB + 4 KiB) is a static device array of the
+ * library, one copy per GPU, reset by a kernel at the start of every call:
+ * calls that overlap on one GPU (on different streams or handles) race on
+ * it.  It creates no stream or event: the size buckets run in order unless
+ * the handle already owns its side streams (from an earlier FAST call). */
+#define MFH_NOALLOC_MAX_COUNT 1048576
+int mfh_dposv_vbatched_noalloc(mfh_handle* h, int count, const int* n, double* const* a,
+                               const int* lda, double* const* x, int* info, int max_n);
+
 /* ---- triangular solve ----------------------------------------------------
  * Replaces Blas.SolveTriangular(Uplo, Transpose, int n, double* a, int lda,
  * double* x, int incx) (Blas.SolveTriangular.cs:162-242).  diag MFH_UNIT is

@@ -23,6 +23,7 @@ UPPER, LOWER = 121, 122
 NONUNIT, UNIT = 131, 132
 EXACT_MAX_N = 128
 MATH_EXACT, MATH_FAST = 0, 1
+NOALLOC_MAX_COUNT = 1 << 20
 BLOCKED_MAX_N = 512
 KERNEL_SORT, KERNEL_CHOL_EXACT, KERNEL_CHOL_FAST, KERNEL_GETRF_EXACT, KERNEL_TRSV_EXACT, KERNEL_GETRS_EXACT = \
     1, 2, 3, 4, 5, 6
@@ -56,6 +57,7 @@ SIGNATURES = {
     "mfh_dpotrf_vbatched": [_p, _i, _p, _p, _p, _p, _i],
     "mfh_dtrsv_vbatched": [_p, _i, _i, _i, _i, _p, _p, _p, _p, _p, _p, _i],
     "mfh_dposv_vbatched": [_p, _i, _p, _p, _p, _p, _p, _i],
+    "mfh_dposv_vbatched_noalloc": [_p, _i, _p, _p, _p, _p, _p, _i],
     "mfh_dgetrs_vbatched": [_p, _i, _p, _p, _p, _p, _p, _p, _p, _i],
     "mfh_dgemm_vbatched": [_p, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i, _i, _i],
     "mfh_zgemm_vbatched": [_p, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i, _i, _i],

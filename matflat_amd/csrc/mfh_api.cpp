@@ -6,12 +6,44 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <mutex>
 #include <string>
 
 #include "mfh_internal.h"
 
 namespace {
 thread_local std::string g_error;
+}
+
+// Static device scratch of mfh_dposv_vbatched_noalloc: the size sort's keys
+// and item lists (2 ints per matrix) and its histograms, for up to
+// MFH_NOALLOC_MAX_COUNT matrices.  Every GPU the library runs on holds its
+// own copy (8 MiB + 4 KiB) until the process ends.  Host code uses only the
+// device address that hipGetSymbolAddress returns for the current device,
+// never the array's name.
+#define MFH_NOALLOC_SCRATCH_BYTES ((size_t)8 * MFH_NOALLOC_MAX_COUNT + 4096)
+static __device__ __align__(256) unsigned char scratch[MFH_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[64];  // per device: address of this device's scratch, looked up once
+}  // namespace
+
+// The scratch address on the current device (looked up on first use per
+// device, with that device current).
+static int scratch_address(int device, void** ptr)
+{
+    if (device < 0 || device >= 64) return mfh_fail(MFH_ERR_UNSUPPORTED, "device index out of range for the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || p == nullptr)
+            return mfh_fail(MFH_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[device] = p;
+    }
+    *ptr = g_scratch_ptr[device];
+    return MFH_OK;
 }
 
 void mfh_set_error(const std::string& msg) { g_error = msg; }
@@ -25,6 +57,7 @@ int mfh_fail(int status, const std::string& msg)
 int mfh_ws_reserve(mfh_handle* h, size_t bytes)
 {
     if (h->ws_bytes >= bytes) return MFH_OK;
+    if (h->no_alloc) return mfh_fail(MFH_ERR_STATE, "the static scratch is too small for this call");
     if (h->capturing) return mfh_fail(MFH_ERR_STATE, "workspace must be reserved before graph capture");
     size_t want = bytes < (1u << 20) ? (1u << 20) : bytes + bytes / 4;
     if (h->ws) {
@@ -42,6 +75,7 @@ int mfh_ws_reserve(mfh_handle* h, size_t bytes)
 int mfh_aux_reserve(mfh_handle* h, size_t bytes)
 {
     if (h->aux_bytes >= bytes) return MFH_OK;
+    if (h->no_alloc) return mfh_fail(MFH_ERR_STATE, "no second buffer in a call that allocates nothing");
     if (h->capturing) return mfh_fail(MFH_ERR_STATE, "workspace must be reserved before graph capture");
     size_t want = bytes < (1u << 20) ? (1u << 20) : bytes + bytes / 4;
     if (h->aux) {
@@ -86,7 +120,7 @@ int take_event(mfh_handle* h, hipEvent_t* ev)
 
 int mfh_timing_begin(mfh_handle* h, int kind, int rows)
 {
-    if (!h->timing || h->capturing) return MFH_OK;
+    if (!h->timing || h->capturing || h->no_alloc) return MFH_OK;
     MfhTimedLaunch t{nullptr, nullptr, kind, rows};
     int st = take_event(h, &t.start);
     if (st) return st;
@@ -102,7 +136,7 @@ int mfh_timing_begin(mfh_handle* h, int kind, int rows)
 
 int mfh_timing_end(mfh_handle* h)
 {
-    if (!h->timing || h->capturing || h->timed.empty()) return MFH_OK;
+    if (!h->timing || h->capturing || h->no_alloc || h->timed.empty()) return MFH_OK;
     MFH_HIP_CHECK(hipEventRecord(h->timed.back().stop, h->stream));
     return MFH_OK;
 }
@@ -129,6 +163,13 @@ int mfh_create(mfh_handle** handle)
         delete h;
         return mfh_fail(MFH_ERR_HIP, std::string("no usable HIP device: ") + hipGetErrorString(e));
     }
+    // set-up of the static scratch on this device (device current)
+    int st = scratch_address(h->device, &h->scratch);
+    if (st) {
+        delete h;
+        return st;
+    }
+    h->scratch_bytes = MFH_NOALLOC_SCRATCH_BYTES;
     *handle = h;
     return MFH_OK;
 }
@@ -331,6 +372,29 @@ int mfh_dposv_vbatched(mfh_handle* h, int count, const int* n, double* const* a,
     MFH_REQUIRE(max_n >= 1 && max_n <= MFH_EXACT_MAX_N, 8, "max_n must be in [1, MFH_EXACT_MAX_N]");
     MfhCholBatch B{n, a, lda, x, info};
     return mfh_launch_potrf(h, count, B, max_n, true);
+}
+
+int mfh_dposv_vbatched_noalloc(mfh_handle* h, int count, const int* n, double* const* a, const int* lda,
+                               double* const* x, int* info, int max_n)
+{
+    MFH_REQUIRE(h != nullptr, 1, "handle must not be null");
+    MFH_REQUIRE(count >= 0, 2, "count must be >= 0");
+    MFH_REQUIRE(count <= MFH_NOALLOC_MAX_COUNT, 2, "count must be <= MFH_NOALLOC_MAX_COUNT (the static scratch size)");
+    if (count == 0) return MFH_OK;
+    MFH_REQUIRE(h->scratch != nullptr, 1, "the handle has no scratch address");
+    // The scratch replaces the handle workspace for this call only; with
+    // no_alloc set, nothing below allocates, creates a stream or an event:
+    // the size buckets run in order unless the handle's side streams exist.
+    void* saved_ws = h->ws;
+    const size_t saved_bytes = h->ws_bytes;
+    h->ws = h->scratch;
+    h->ws_bytes = h->scratch_bytes;
+    h->no_alloc = true;
+    const int st = mfh_dposv_vbatched(h, count, n, a, lda, x, info, max_n);
+    h->no_alloc = false;
+    h->ws = saved_ws;
+    h->ws_bytes = saved_bytes;
+    return st;
 }
 
 int mfh_dtrsv_vbatched(mfh_handle* h, int uplo, int trans, int diag, int count, const int* n, const double* const* a,

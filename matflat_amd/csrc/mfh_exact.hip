@@ -919,7 +919,10 @@ int mfh_launch_getrf(mfh_handle* h, int count, const MfhGetrfBatch& B, int max_m
 template <typename F>
 int fork_buckets(mfh_handle* h, int nb, F launch)
 {
-    const int k = h->timing ? 1 : (h->bucket_streams < nb ? h->bucket_streams : nb);
+    int k = h->timing ? 1 : (h->bucket_streams < nb ? h->bucket_streams : nb);
+    // a call that allocates nothing uses only side streams that exist already
+    if (h->no_alloc && k > (int)h->side.size()) k = (int)h->side.size();
+    if (h->no_alloc && (int)h->side_ev.size() < k + 1) k = 1;
     if (k <= 1) {
         for (int i = nb - 1; i >= 0; --i) {
             int st = launch(i);

@@ -36,6 +36,9 @@ struct mfh_handle {
     void* aux = nullptr;  // second device buffer: staging of calls that run another entry's kernels on ws
     size_t aux_bytes = 0;
     bool capturing = false;  // plans pre-size the workspace; no allocation while set
+    bool no_alloc = false;   // inside mfh_dposv_vbatched_noalloc: ws is the static scratch, nothing is created
+    void* scratch = nullptr; // this device's copy of the library's static scratch array (device address)
+    size_t scratch_bytes = 0;
     int math_mode = MFH_MATH_EXACT;
     int fast_waves[18] = {};              // FAST Cholesky waves per matrix by row blocks (0 = default)
     bool timing = false;                  // bracket launches with events

@@ -236,6 +236,14 @@ def dgesv_vbatched(n, a_ptrs, lda, b_ptrs, x_ptrs, berr, info, max_n, handle: Ha
                                     _p(berr), _p(info), int(max_n)))
 
 
+def dposv_vbatched_noalloc(n, a_ptrs, lda, x_ptrs, info, max_n, handle: Handle | None = None):
+    """The SPD solve of dposv_vbatched on the library's static scratch: no
+    device allocation (mfh_dposv_vbatched_noalloc; count <= NOALLOC_MAX_COUNT)."""
+    h = handle or default_handle()
+    _check(h.lib.mfh_dposv_vbatched_noalloc(h.h, int(n.numel()), _p(n), _p(a_ptrs), _p(lda), _p(x_ptrs), _p(info),
+                                            int(max_n)))
+
+
 def dgetrf_blocked_vbatched(n, a_ptrs, lda, piv_ptrs, info, max_n, handle: Handle | None = None):
     """FAST-mode LU with partial pivoting for n <= 512 (mfh_dgetrf_blocked_vbatched)."""
     h = handle or default_handle()
