@@ -5,10 +5,12 @@ Workload (one step): 100,000 SPD matrices, order n ~ U{10..100}, leading
 dimension 8*ceil(n/8), A = B B^T + n I with B ~ N(0,1) (upper triangle
 stored, strict lower triangle NaN: it is never read), one N(0,1) right-hand
 side each.  A step is one replay of the hipGraph that holds the whole batch
-(device bucketing + the persistent solve kernels); inputs are resident in
-HBM before timing starts.  Each step factors and solves in place; the
-kernels are data-independent (no pivoting, no early exits on SPD input), so
-every step performs the same work on the same bytes.
+(device bucketing + the solve kernels) on a pristine copy of the inputs,
+resident in HBM.  The solve works in place, so the timed steps rotate over
+working copies that are restored from the pristine batch between cycles,
+outside the timing events (two copies of the 3.1 GB batch: no input is in
+the 256 MB last-level cache when its step starts); the outputs of the last
+timed step are checked (every info zero, residuals of a sample).
 
 Prints one JSON line (see README for the fields).  --gpus N with N > 1 exits:
 this build runs on one GPU.
@@ -33,6 +35,96 @@ HBM_PEAK_GBS = 8000.0
 
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def _plain(v):
+    """JSON form of the numpy scalars the result dictionaries may hold."""
+    if isinstance(v, np.generic):
+        return v.item()
+    raise TypeError(f"{type(v).__name__} is not JSON serializable")
+
+
+def host_cpu() -> dict:
+    """The host CPU the baselines run on and the threads they use: the OpenMP
+    thread count the environment sets (the host's CPU share for this GPU),
+    capped by the cores this process may run on; every allowed core when the
+    environment sets none."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    allowed = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    env = os.environ.get("OMP_NUM_THREADS", "").strip()
+    threads = min(allowed, int(env)) if env.isdigit() and int(env) > 0 else allowed
+    return {"cpu_model": model, "logical_cores_visible": os.cpu_count(), "cores_allowed": allowed,
+            "threads": threads, "omp_num_threads": env or None}
+
+
+def cpu_entry(value: float, unit: str, sample: str, reps: int) -> dict:
+    h = host_cpu()
+    return {"value": round(value, 1), "unit": unit, "cores": h["threads"], "kind": "port", "sample": sample,
+            "runs": reps, "cpu_model": h["cpu_model"], "cores_allowed": h["cores_allowed"],
+            "logical_cores_visible": h["logical_cores_visible"],
+            "thread_policy": "OMP_NUM_THREADS (the host's CPU share for this GPU) capped by the affinity mask"}
+
+
+def timed_cpu(run, max_seconds: float, min_reps: int = 3, max_reps: int = 5):
+    """Median wall time of run() (each call does its own untimed setup and
+    returns its timed seconds) over min_reps..max_reps runs, stopping once
+    max_seconds have passed."""
+    elapsed = []
+    t_start = time.perf_counter()
+    while len(elapsed) < max_reps:
+        elapsed.append(run())
+        if len(elapsed) >= min_reps and time.perf_counter() - t_start > max_seconds:
+            break
+    return float(np.median(elapsed)), len(elapsed)
+
+
+def rotated_calls(call, restore, copies: int, steps: int, warmup: int, stream):
+    """Per-call durations (ms) of `steps` timed calls, each bracketed by its
+    own HIP events on `stream`.  The calls cycle over `copies` working copies
+    of the inputs: a cycle first restores every copy (restore(i), untimed and
+    outside the events; nothing when restore is None), then calls copy 0, 1,
+    ... in the order they were restored, so the other copies' bytes pass
+    through the last-level cache between a copy's restore and its call.
+    Returns (per-call ms list, index of the copy the last timed call used)."""
+    import torch
+
+    def cycle(n_calls, timed):
+        if restore is not None:
+            for i in range(copies):
+                restore(i)
+        evs = []
+        for i in range(n_calls):
+            if timed:
+                s0, s1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                s0.record(stream)
+                call(i)
+                s1.record(stream)
+                evs.append((s0, s1))
+            else:
+                call(i)
+        torch.cuda.synchronize()
+        return [a.elapsed_time(b) for a, b in evs]
+
+    done = 0
+    while done < warmup:
+        k = min(copies, warmup - done)
+        cycle(k, False)
+        done += k
+    times = []
+    last = 0
+    while len(times) < steps:
+        k = min(copies, steps - len(times))
+        times += cycle(k, True)
+        last = k - 1
+    return times, last
 
 
 def make_spd_batch(count: int, nmin: int, nmax: int, seed: int, device: str = "cuda"):
@@ -78,20 +170,22 @@ def alg_bytes_posv(ns: np.ndarray) -> float:
     return float(np.sum(8.0 * (n * (n + 1) / 2 + n * n + 2 * n) + 4.0))
 
 
-def kernel_pass(handle, mode, n_t, a_ptrs, lda_t, x_ptrs, info, nmax, ns, steps):
-    """Times every kernel launch of `steps` eager batched solves with HIP
-    events recorded by the library on the stream it launches on.  Returns
-    one entry per distinct launch: kind, row bound, matrices it processes,
-    their algorithmic bytes and the mean duration."""
-    import torch
-
+def kernel_pass(handle, mode, batch, nmax, ns, steps, stream):
+    """Times every kernel launch of `steps` eager batched solves, each on a
+    pristine working copy (restored between cycles, outside the kernels),
+    with HIP events recorded by the library on the stream it launches on.
+    Returns one entry per distinct launch: kind, row bound, matrices it
+    processes, their algorithmic bytes and the mean duration."""
     from matflat_amd import _lib, hip
 
     handle.set_math_mode(mode)
     handle.set_timing(True)
-    torch.cuda.synchronize()
-    for _ in range(steps):
-        hip.dposv_vbatched(n_t, a_ptrs, lda_t, x_ptrs, info, nmax, handle=handle)
+
+    def call(i):
+        hip.dposv_vbatched(batch.n_t, batch.a_ptrs[i], batch.lda_t, batch.x_ptrs[i], batch.info[i], nmax,
+                           handle=handle)
+
+    rotated_calls(call, batch.restore, batch.copies, steps, 0, stream)
     recs = handle.timing()
     handle.set_timing(False)
     per: dict = {}
@@ -212,15 +306,95 @@ def make_conditioned_batch(count: int, seed: int, chunk: int = 200):
     return buf, offsets, ns, b, boff
 
 
-def mixed_leg(steps: int, warmup: int, count: int = 8000, seed: int = 3):
+MFMA_PROFILE = os.path.join(ROOT, "profiles", "k_gesv_8000_n128-512_mfma.md")
+
+
+def mfma_busy(kernel: str):
+    """MFMA busy fraction (SQ_VALU_MFMA_BUSY_CYCLES / (GRBM_GUI_ACTIVE x 1024
+    SIMDs)) of a factorization kernel of the 8,000-matrix mixed-precision
+    workload, from the committed rocprofv3 --pmc summary; None when absent."""
+    if not os.path.exists(MFMA_PROFILE):
+        return None
+    for line in open(MFMA_PROFILE):
+        if line.startswith("mfma_busy:"):
+            parts = dict(kv.split("=", 1) for kv in line.split()[1:])
+            if parts.get("kernel") == kernel:
+                return float(parts["frac"])
+    return None
+
+
+def mixed_cpu_baseline(buf, offsets, ns, b, boff, per_size=100, max_seconds=12.0):
+    """The oracle's LU (Factorization.Lu) + LU solve over a bounded sample of
+    the mixed-precision batch: the first `per_size` matrices of each order,
+    one threaded batch per order, OpenMP dynamic over the host's threads."""
+    import oracle
+
+    threads = host_cpu()["threads"]
+    groups = []
+    b_host = b.cpu().numpy()
+    for n in MIXED_SIZES:
+        idx = np.flatnonzero(ns == n)[:per_size]
+        if idx.size == 0:
+            continue
+        o = int(offsets[idx[0]])
+        assert np.array_equal(offsets[idx], o + n * n * np.arange(idx.size))  # contiguous per order
+        a = buf[o:o + idx.size * n * n].cpu().numpy()
+        rhs = np.concatenate([b_host[boff[j]:boff[j] + n] for j in idx])
+        groups.append((n, idx.size, a, rhs))
+    count = sum(g[1] for g in groups)
+
+    def run():
+        work = [(n, c, a.copy(), rhs) for n, c, a, rhs in groups]
+        t0 = time.perf_counter()
+        for n, c, a, rhs in work:
+            oracle.dgesv_batch(c, n, n, n * n, a, rhs, threads=threads)
+        return time.perf_counter() - t0
+
+    med, reps = timed_cpu(run, max_seconds, min_reps=3, max_reps=5)
+    return cpu_entry(count / med, "solves/s",
+                     f"{count} of the 8,000 matrices (the first {per_size} of each order in {list(MIXED_SIZES)}, the "
+                     "batch's own size mix), FP64 LU with partial pivoting + LU solve in MatFlat's order "
+                     "(oracle/matflat_oracle.c, Factorization.Lu.cs:296-350 and Factorization.LuInverse.cs:169-192), "
+                     f"OpenMP dynamic over {threads} threads, median of {reps} runs", reps)
+
+
+def save_conditioned_batch(path: str, count: int = 8000, seed: int = 3) -> None:
+    """Generates the mixed-precision workload and writes it to `path` (a
+    directory) as .npy files, for runs that must not generate it themselves
+    (a counter-collecting profiler run, which fails inside torch's QR)."""
+    buf, offsets, ns, b, boff = make_conditioned_batch(count, seed)
+    os.makedirs(path, exist_ok=True)
+    np.save(os.path.join(path, "a.npy"), buf.cpu().numpy())
+    np.save(os.path.join(path, "b.npy"), b.cpu().numpy())
+    np.savez(os.path.join(path, "meta.npz"), offsets=offsets, ns=ns, boff=boff, seed=seed)
+    log(f"saved the {count}-matrix batch to {path}")
+
+
+def load_conditioned_batch(path: str):
+    import torch
+
+    meta = np.load(os.path.join(path, "meta.npz"))
+    buf = torch.from_numpy(np.load(os.path.join(path, "a.npy"), mmap_mode="r")[:].copy()).to("cuda")
+    b = torch.from_numpy(np.load(os.path.join(path, "b.npy"))).to("cuda")
+    return buf, meta["offsets"], meta["ns"], b, meta["boff"]
+
+
+def mixed_leg(steps: int, warmup: int, count: int = 8000, seed: int = 3, cpu: bool = True, batch_dir=None):
     """FP32/BF16-factor + FP64-refinement LU solve against the plain FP64
-    blocked LU solve on the same batch, both to backward error <= 1e-15 n."""
+    blocked LU solve on the same batch, both to backward error <= 1e-15 n.
+    The inputs are read, never written, so every step sees the same bytes;
+    the 7.9 GB batch is 30x the last-level cache."""
     import torch
 
     from matflat_amd import _lib, hip
 
-    log(f"mixed precision: generating {count} matrices n in {MIXED_SIZES}, cond 1e2..1e6")
-    buf, offsets, ns, b, boff = make_conditioned_batch(count, seed)
+    if batch_dir:
+        log(f"mixed precision: loading the batch from {batch_dir}")
+        buf, offsets, ns, b, boff = load_conditioned_batch(batch_dir)
+        count = ns.size
+    else:
+        log(f"mixed precision: generating {count} matrices n in {MIXED_SIZES}, cond 1e2..1e6")
+        buf, offsets, ns, b, boff = make_conditioned_batch(count, seed)
     torch.cuda.synchronize()
     dev = torch.device("cuda")
     n_t = torch.from_numpy(ns).to(dev)
@@ -244,18 +418,12 @@ def mixed_leg(steps: int, warmup: int, count: int = 8000, seed: int = 3):
             hip.dgesv_vbatched(n_t, a_ptrs, n_t, b_ptrs, x_ptrs, berr, info, max_n, handle=handle)
 
     res = {}
-    flops = float(np.sum(2.0 / 3.0 * ns.astype(np.float64) ** 3 + 1.5 * ns.astype(np.float64) ** 2))
+    nf = ns.astype(np.float64)
+    flops = float(np.sum(2.0 / 3.0 * nf ** 3 + 1.5 * nf ** 2))
+    factor_flops = float(np.sum(2.0 / 3.0 * nf ** 3))
     for name, mixed in (("fp64", False), ("mixed", True)):
-        for _ in range(warmup):
-            call(mixed)
-        torch.cuda.synchronize()
-        start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        start.record(stream)
-        for _ in range(steps):
-            call(mixed)
-        stop.record(stream)
-        torch.cuda.synchronize()
-        ms = start.elapsed_time(stop) / steps
+        times, _ = rotated_calls(lambda i: call(mixed), None, 1, steps, warmup, stream)
+        ms = float(np.mean(times))
         inf = info.cpu().numpy()
         be = berr.cpu().numpy()
         ratio = be / ns
@@ -266,16 +434,46 @@ def mixed_leg(steps: int, warmup: int, count: int = 8000, seed: int = 3):
         if mixed:
             it = iters.cpu().numpy()
             entry.update({"mean_iters": round(float(it.mean()), 3), "max_iters": int(it.max())})
+        # one call with the library's per-launch events: the factorization kernel and the solve steps
+        handle.set_timing(True)
+        call(mixed)
+        recs = handle.timing()
+        handle.set_timing(False)
+        fkind = _lib.KERNEL_GESV_MIXED if mixed else _lib.KERNEL_GESV_F64
+        fact = [t for k, _, t in recs if k == fkind]
+        step = [t for k, _, t in recs if k == _lib.KERNEL_GESV_STEP]
+        kname = "k_gesv<float, true>" if mixed else "k_gesv<double, false>"
+        fms = float(np.mean(fact))
+        tf = factor_flops / (fms / 1e3) / 1e12
+        busy = mfma_busy(kname)
+        entry["kernels"] = {"factor_ms": round(fms, 4), "solve_step_launches": len(step),
+                            "solve_steps_ms": round(float(np.sum(step)), 4)}
+        entry["roofline"] = {"bound": "mfma", "achieved": round(tf, 3), "peak": 78.6, "unit": "TFLOP/s",
+                             "frac": round(tf / 78.6, 4), "traffic": None,
+                             "kernel": f"{kname} (factorization of all {count} matrices, one launch)",
+                             "kernel_avg_ms": round(fms, 5), "alg_flops_per_launch": factor_flops,
+                             "peak_basis": "FP64 dense, 78.6 TFLOP/s; flops are the LU's 2/3 n^3 (HPL count) "
+                                           "whatever precision computes them",
+                             "mfma_busy": busy,
+                             "mfma_busy_source": ("profiles/k_gesv_8000_n128-512_mfma.md (rocprofv3 --pmc "
+                                                  "SQ_VALU_MFMA_BUSY_CYCLES / (GRBM_GUI_ACTIVE x 1024 SIMDs))"
+                                                  if busy is not None else None)}
         log(f"  {name}: {ms:.3f} ms/step, {entry['solves_per_s']:.0f} solves/s, {entry['hpl_tflops']} TFLOP/s (HPL "
-            f"count), max berr/n {entry['max_berr_over_n']:.2e}" +
+            f"count), max berr/n {entry['max_berr_over_n']:.2e}; factor kernel {fms:.3f} ms ({tf:.2f} TFLOP/s), "
+            f"{len(step)} solve launches {float(np.sum(step)):.3f} ms" +
             (f", iters mean {entry['mean_iters']} max {entry['max_iters']}" if mixed else ""))
         res[name] = entry
     handle.close()
     res["speedup_mixed_over_fp64"] = round(res["fp64"]["ms_per_step"] / res["mixed"]["ms_per_step"], 3)
+    res["roofline"] = dict(res["mixed"]["roofline"])
     res["workload"] = (f"{count} LU solves, n uniform over {list(MIXED_SIZES)}, A = U diag(s) V^T (Haar U, V), "
                        "cond 10^U(2,6), b ~ N(0,1); refine to ||b-Ax||/(||A|| ||x|| + ||b||) <= 1e-15 n or 10 "
                        "iterations")
     res["seed"] = seed
+    if cpu:
+        log("  mixed-precision workload: cpu baseline")
+        res["cpu_baseline"] = mixed_cpu_baseline(buf, offsets, ns, b, boff)
+        log(f"  cpu: {res['cpu_baseline']['value']:.1f} solves/s")
     del buf, b, x
     torch.cuda.empty_cache()
     return res
@@ -310,10 +508,13 @@ def hbm_entry(alg_bytes: float, ms: float) -> dict:
             "frac": round(gbs / HBM_PEAK_GBS, 4), "alg_bytes": alg_bytes}
 
 
-def lu_leg(steps: int, warmup: int, count: int = 1024, n: int = 64, seed: int = 1):
+def lu_leg(steps: int, warmup: int, count: int = 1024, n: int = 64, seed: int = 1, copies: int = 16,
+           cpu: bool = True):
     """Exact (bit-identical to MatFlat) partial-pivot LU of `count` column-major
-    n x n matrices, lda = n, N(0,1) entries; each timed call factors a fresh
-    copy of the batch."""
+    n x n matrices, lda = n, N(0,1) entries; then LuInverse and the LU solve
+    of the same factors.  Each timed call works on its own pristine working
+    copy: the calls rotate over `copies` copies (16 x 33.5 MB = 537 MB, twice
+    the last-level cache), restored between cycles outside the events."""
     import torch
 
     from matflat_amd import hip
@@ -321,44 +522,66 @@ def lu_leg(steps: int, warmup: int, count: int = 1024, n: int = 64, seed: int = 
     dev = torch.device("cuda")
     g = torch.Generator(device=dev)
     g.manual_seed(seed)
-    src = torch.randn(count * n * n, dtype=torch.float64, device=dev, generator=g)
-    a = torch.empty_like(src)
-    piv = torch.zeros(count * n, dtype=torch.int32, device=dev)
-    sign = torch.zeros(count, dtype=torch.int32, device=dev)
-    info = torch.zeros(count, dtype=torch.int32, device=dev)
+    per = count * n * n
+    src = torch.randn(per, dtype=torch.float64, device=dev, generator=g)
+    work = torch.empty((copies, per), dtype=torch.float64, device=dev)
+    piv = torch.zeros((copies, count * n), dtype=torch.int32, device=dev)
+    sign = torch.zeros((copies, count), dtype=torch.int32, device=dev)
+    info = torch.zeros((copies, count), dtype=torch.int32, device=dev)
     stream = torch.cuda.current_stream()
     handle = hip.Handle()
     handle.bind_stream(stream)
     handle.set_math_mode(hip.MATH_EXACT)
-    ms = timed_calls(lambda: hip.dgetrf_strided_batched(count, n, n, a, n, n * n, piv, n, sign, info, handle=handle),
-                     lambda: a.copy_(src), steps, warmup, stream)
-    torch.cuda.synchronize()
-    ok = bool(torch.all(info == 0).item())
-    # Factorization.LuInverse of the factored batch; each timed call restores the factors
-    lu_src = a.clone()
+
+    def restore_src(i):
+        work[i].copy_(src)
+        info[i].fill_(-99)
+
+    times, last = rotated_calls(
+        lambda i: hip.dgetrf_strided_batched(count, n, n, work[i], n, n * n, piv[i], n, sign[i], info[i],
+                                             handle=handle), restore_src, copies, steps, warmup, stream)
+    ms = float(np.mean(times))
+    ok = bool(torch.all(info[last] == 0).item())
+    got_lu, got_piv = work[last].cpu().numpy(), piv[last].cpu().numpy()
+    # Factorization.LuInverse of the factored batch: every copy restored to the factors
+    lu_src = work[last].clone()
+    piv_src = piv[last].clone()
     idx = torch.arange(count, dtype=torch.int64, device=dev)
-    a_ptrs = a.data_ptr() + 8 * n * n * idx
-    piv_ptrs = piv.data_ptr() + 4 * n * idx
+    a_ptrs = [(work[i].data_ptr() + 8 * n * n * idx).contiguous() for i in range(copies)]
+    piv_ptrs = [(piv[i].data_ptr() + 4 * n * idx).contiguous() for i in range(copies)]
     n_t = torch.full((count,), n, dtype=torch.int32, device=dev)
-    inv_info = torch.zeros(count, dtype=torch.int32, device=dev)
-    ms_inv = timed_calls(lambda: hip.dgetri_vbatched(n_t, a_ptrs, n_t, piv_ptrs, inv_info, n, handle=handle),
-                         lambda: a.copy_(lu_src), steps, warmup, stream)
-    inv_ok = bool(torch.all(inv_info == 0).item())
-    # LU solve (the LuInverse composition for one right-hand side) with the factors
-    a.copy_(lu_src)
-    bvec = torch.randn(count * n, dtype=torch.float64, device=dev, generator=g)
+    inv_info = torch.zeros((copies, count), dtype=torch.int32, device=dev)
+
+    def restore_lu(i):
+        work[i].copy_(lu_src)
+        piv[i].copy_(piv_src)
+        inv_info[i].fill_(-99)
+
+    inv_times, inv_last = rotated_calls(
+        lambda i: hip.dgetri_vbatched(n_t, a_ptrs[i], n_t, piv_ptrs[i], inv_info[i], n, handle=handle),
+        restore_lu, copies, steps, warmup, stream)
+    ms_inv = float(np.mean(inv_times))
+    inv_ok = bool(torch.all(inv_info[inv_last] == 0).item())
+    # LU solve (the LuInverse composition for one right-hand side): read-only factors, rotated
+    for i in range(copies):
+        restore_lu(i)
+    bvec = torch.randn((copies, count * n), dtype=torch.float64, device=dev, generator=g)
     xvec = torch.zeros_like(bvec)
-    b_ptrs = bvec.data_ptr() + 8 * n * idx
-    x_ptrs = xvec.data_ptr() + 8 * n * idx
-    rs_info = torch.zeros(count, dtype=torch.int32, device=dev)
-    ms_rs = timed_calls(lambda: hip.dgetrs_vbatched(n_t, a_ptrs, n_t, piv_ptrs, b_ptrs, x_ptrs, rs_info, n,
-                                                    handle=handle), None, steps, warmup, stream)
-    rs_ok = bool(torch.all(rs_info == 0).item())
+    b_ptrs = [(bvec[i].data_ptr() + 8 * n * idx).contiguous() for i in range(copies)]
+    x_ptrs = [(xvec[i].data_ptr() + 8 * n * idx).contiguous() for i in range(copies)]
+    rs_info = torch.zeros((copies, count), dtype=torch.int32, device=dev)
+    rs_times, rs_last = rotated_calls(
+        lambda i: hip.dgetrs_vbatched(n_t, a_ptrs[i], n_t, piv_ptrs[i], b_ptrs[i], x_ptrs[i], rs_info[i], n,
+                                      handle=handle), None, copies, steps, warmup, stream)
+    ms_rs = float(np.mean(rs_times))
+    rs_ok = bool(torch.all(rs_info[rs_last] == 0).item())
     handle.close()
     alg = count * (16.0 * n * n + 4 * n + 4)
     res = {"workload": f"{count} LU factorizations (partial pivoting), n = {n}, lda = {n}, N(0,1), exact mode "
                        "(bit-identical to MatFlat's Factorization.Lu)",
-           "ms_per_call": round(ms, 5), "factorizations_per_s": round(count / (ms / 1e3), 1),
+           "working_copies": copies, "working_set_mb": round(copies * per * 8 / 1e6, 1),
+           "ms_per_call": round(ms, 5), "ms_per_call_median": round(float(np.median(times)), 5),
+           "factorizations_per_s": round(count / (ms / 1e3), 1),
            "all_info_zero": ok, "roofline": hbm_entry(alg, ms),
            "inverse": {"workload": f"Factorization.LuInverse of the same {count} factored matrices, exact mode "
                                    "(bit-identical)",
@@ -369,14 +592,41 @@ def lu_leg(steps: int, warmup: int, count: int = 1024, n: int = 64, seed: int = 
                      "ms_per_call": round(ms_rs, 5), "solves_per_s": round(count / (ms_rs / 1e3), 1),
                      "all_info_zero": rs_ok,
                      "roofline": hbm_entry(count * (8.0 * n * n + 4 * n + 16 * n + 4), ms_rs)}}
+    if cpu:
+        import oracle
+
+        threads = host_cpu()["threads"]
+        a_host = src.cpu().numpy()
+        out = {}
+
+        def run():
+            a = a_host.copy()
+            t0 = time.perf_counter()
+            out["piv"], _ = oracle.dgetrf_batch(count, n, n, n, n * n, a, threads=threads)
+            dt = time.perf_counter() - t0
+            out["lu"] = a
+            return dt
+
+        med, reps = timed_cpu(run, 3.0, min_reps=5, max_reps=50)
+        res["cpu_baseline"] = cpu_entry(count / med, "factorizations/s",
+                                        f"the full {count}-matrix batch, Factorization.Lu in MatFlat's order "
+                                        f"(oracle/matflat_oracle.c), OpenMP dynamic over {threads} threads, "
+                                        f"median of {reps} runs", reps)
+        # the timed call's outputs against the oracle's: MatFlat's pivots and bits
+        res["output_check"] = {"pivots_bit_exact": bool(np.array_equal(got_piv + 0, out["piv"])),
+                               "factors_bit_exact": bool(np.array_equal(got_lu, out["lu"]))}
     log(f"  LU n={n} x{count}: {ms:.4f} ms/call, {res['factorizations_per_s']:.0f} LU/s, "
-        f"{res['roofline']['achieved']} GB/s; inverse {ms_inv:.4f} ms/call; solve {ms_rs:.4f} ms/call")
+        f"{res['roofline']['achieved']} GB/s; inverse {ms_inv:.4f} ms/call; solve {ms_rs:.4f} ms/call"
+        + (f"; cpu {res['cpu_baseline']['value']:.0f} LU/s, check {res['output_check']}" if cpu else ""))
     return res
 
 
-def adjoint_leg(steps: int, warmup: int, count: int = 50_000, n: int = 32, seed: int = 4):
+def adjoint_leg(steps: int, warmup: int, count: int = 50_000, n: int = 32, seed: int = 4, copies: int = 2,
+                cpu: bool = True):
     """Forward (exact-mode SPD solve) and hand-written backward of the batched
-    SPD solve, n = 32, A = B B^T + n I."""
+    SPD solve, n = 32, A = B B^T + n I.  The forward works in place, so its
+    calls rotate over `copies` pristine working copies (2 x 410 MB) restored
+    between cycles; the backward reads the forward's outputs of each copy."""
     import torch
 
     from matflat_amd import hip
@@ -389,7 +639,8 @@ def adjoint_leg(steps: int, warmup: int, count: int = 50_000, n: int = 32, seed:
     del bm
     b0 = torch.randn((count, n), dtype=torch.float64, device=dev, generator=g)
     w = torch.randn((count, n), dtype=torch.float64, device=dev, generator=g)
-    a, x = torch.empty_like(src), torch.empty_like(b0)
+    a = [torch.empty_like(src) for _ in range(copies)]
+    x = [torch.empty_like(b0) for _ in range(copies)]
     abar, bbar = torch.empty_like(src), torch.empty_like(b0)
     ns = torch.full((count,), n, dtype=torch.int32, device=dev)
     step = torch.arange(count, dtype=torch.int64, device=dev)
@@ -402,31 +653,66 @@ def adjoint_leg(steps: int, warmup: int, count: int = 50_000, n: int = 32, seed:
     handle = hip.Handle()
     handle.bind_stream(stream)
     handle.set_math_mode(hip.MATH_EXACT)
-    pa, px, pw, pab, pbb = ptrs(a, n * n), ptrs(x, n), ptrs(w, n), ptrs(abar, n * n), ptrs(bbar, n)
+    pa = [ptrs(t, n * n) for t in a]
+    px = [ptrs(t, n) for t in x]
+    pw, pab, pbb = ptrs(w, n), ptrs(abar, n * n), ptrs(bbar, n)
 
-    def restore():
-        a.copy_(src)
-        x.copy_(b0)
+    def restore(i):
+        a[i].copy_(src)
+        x[i].copy_(b0)
 
-    fwd = timed_calls(lambda: hip.dposv_vbatched(ns, pa, ns, px, info, n, handle=handle), restore, steps, warmup,
-                      stream)
+    ft, _ = rotated_calls(lambda i: hip.dposv_vbatched(ns, pa[i], ns, px[i], info, n, handle=handle), restore,
+                          copies, steps, warmup, stream)
+    fwd = float(np.mean(ft))
     ok_f = bool(torch.all(info == 0).item())
-    bwd = timed_calls(lambda: hip.dposv_backward_vbatched(ns, pa, ns, px, pw, pab, ns, pbb, info, n, handle=handle),
-                      None, steps, warmup, stream)
+    for i in range(copies):
+        restore(i)
+        hip.dposv_vbatched(ns, pa[i], ns, px[i], info, n, handle=handle)
+    bt, _ = rotated_calls(lambda i: hip.dposv_backward_vbatched(ns, pa[i], ns, px[i], pw, pab, ns, pbb, info, n,
+                                                                handle=handle), None, copies, steps, warmup, stream)
+    bwd = float(np.mean(bt))
     ok_b = bool(torch.all(info == 0).item())
     handle.close()
     fb = count * 8.0 * (n * (n + 1) / 2 + n * n + 2 * n)
     bb = count * 8.0 * (2 * n * n + 3 * n)
     res = {"workload": f"{count} SPD solves n = {n} (exact mode) and their adjoint (dL/dA, dL/db for L = w.x)",
-           "forward_ms": round(fwd, 5), "backward_ms": round(bwd, 5),
+           "working_copies": copies, "forward_ms": round(fwd, 5), "backward_ms": round(bwd, 5),
            "forward_plus_backward_per_s": round(count / ((fwd + bwd) / 1e3), 1), "all_info_zero": ok_f and ok_b,
            "forward_roofline": hbm_entry(fb, fwd), "backward_roofline": hbm_entry(bb, bwd)}
+    if cpu:
+        import oracle
+
+        threads = host_cpu()["threads"]
+        a_host = src.cpu().numpy().reshape(-1)
+        b_host = b0.cpu().numpy().reshape(-1)
+        w_host = w.cpu().numpy().reshape(-1)
+        nv = np.full(count, n, dtype=np.int32)
+        offs = np.arange(count, dtype=np.int64) * n * n
+        xoffs = np.arange(count, dtype=np.int64) * n
+
+        def run():
+            aa, xx = a_host.copy(), b_host.copy()
+            t0 = time.perf_counter()
+            info_c = oracle.dposv_batch(nv, nv, offs, aa, xoffs, xx, threads=threads)
+            oracle.dposv_backward_batch(count, n, n, n * n, aa, xx, w_host, threads=threads)
+            dt = time.perf_counter() - t0
+            assert np.all(info_c == 0)
+            return dt
+
+        med, reps = timed_cpu(run, 4.0, min_reps=3, max_reps=10)
+        res["cpu_baseline"] = cpu_entry(count / med, "forward+backward/s",
+                                        f"the full {count}-matrix batch: Cholesky + two solves in MatFlat's order, "
+                                        "then the adjoint's two solves and symmetric outer product "
+                                        f"(oracle/matflat_oracle.c), OpenMP dynamic over {threads} threads, "
+                                        f"median of {reps} runs", reps)
     log(f"  adjoint n={n} x{count}: forward {fwd:.4f} ms ({res['forward_roofline']['achieved']} GB/s), backward "
-        f"{bwd:.4f} ms ({res['backward_roofline']['achieved']} GB/s)")
+        f"{bwd:.4f} ms ({res['backward_roofline']['achieved']} GB/s)"
+        + (f"; cpu {res['cpu_baseline']['value']:.0f}/s" if cpu else ""))
     return res
 
 
-def gemm_leg(steps: int, warmup: int, count: int = 200_000, seed: int = 5, complex_: bool = False):
+def gemm_leg(steps: int, warmup: int, count: int = 200_000, seed: int = 5, complex_: bool = False,
+             cpu: bool = True, sample: int = 20_000):
     """C = alpha op(A) op(B) + beta C over `count` products with m, n, k iid
     U{8..128}, op uniform over {N, T} ({N, T, C} for complex), leading
     dimensions padded to a multiple of 4, alpha, beta ~ U(-1, 1), exact mode
@@ -453,11 +739,11 @@ def gemm_leg(steps: int, warmup: int, count: int = 200_000, seed: int = 5, compl
         g = torch.Generator(device=dev)
         g.manual_seed(int(rng.integers(1 << 30)))
         t = torch.randn(int(sizes.sum()) * w, dtype=torch.float64, device=dev, generator=g)
-        return t, (t.data_ptr() + 8 * torch.from_numpy(off).to(dev)).contiguous()
+        return t, (t.data_ptr() + 8 * torch.from_numpy(off).to(dev)).contiguous(), off
 
-    A, pa = slab(size_a)
-    Bt, pb = slab(size_b)
-    C, pc = slab(size_c)
+    A, pa, oa = slab(size_a)
+    Bt, pb, ob = slab(size_b)
+    C, pc, oc = slab(size_c)
     alpha = torch.from_numpy(rng.uniform(-1, 1, size=count * w)).to(dev)
     beta = torch.from_numpy(rng.uniform(-1, 1, size=count * w)).to(dev)
 
@@ -477,7 +763,43 @@ def gemm_leg(steps: int, warmup: int, count: int = 200_000, seed: int = 5, compl
     name = "complex128" if complex_ else "fp64"
     res = {"workload": f"{count} {name} products C = alpha op(A) op(B) + beta C, m, n, k ~ U{{8..128}}, "
                        f"op in {{{'N, T, C' if complex_ else 'N, T'}}}, ld padded to 4, alpha, beta ~ U(-1, 1)",
-           "model_ceiling_ms": round(ceiling_ms, 4), "alg_bytes": float(alg.sum()), "flops": float(flops.sum())}
+           "model_ceiling_ms": round(ceiling_ms, 4), "alg_bytes": float(alg.sum()), "flops": float(flops.sum()),
+           "note": "C is overwritten by each call, so later calls read the previous result (the same work)"}
+    if cpu:
+        # the first `sample` products from the pristine inputs: the oracle's time, and the bits of
+        # one exact-mode GPU call on the same inputs
+        import oracle
+
+        threads = host_cpu()["threads"]
+        S = min(sample, count)
+        ea, eb, ec = int(oa[S - 1] + size_a[S - 1] * w), int(ob[S - 1] + size_b[S - 1] * w), \
+            int(oc[S - 1] + size_c[S - 1] * w)
+        ha, hb, hc = A[:ea].cpu().numpy(), Bt[:eb].cpu().numpy(), C[:ec].cpu().numpy()
+        al_h, be_h = alpha.cpu().numpy()[:S * w], beta.cpu().numpy()[:S * w]
+        handle.set_math_mode(hip.MATH_EXACT)
+        hip.gemm_vbatched(*args, info, 128, 128, 128, complex_=complex_, handle=handle)
+        gpu_c = C[:ec].cpu().numpy()
+        out = {}
+
+        def run():
+            cc = hc.copy()
+            t0 = time.perf_counter()
+            inf_c = oracle.gemm_vbatch(int(complex_), ta[:S], tb[:S], m[:S], n[:S], k[:S], al_h, ha, oa[:S],
+                                       lda[:S], hb, ob[:S], ldb[:S], be_h, cc, oc[:S], ldc[:S], threads=threads)
+            dt = time.perf_counter() - t0
+            assert np.all(inf_c == 0)
+            out["c"] = cc
+            return dt
+
+        med, reps = timed_cpu(run, 4.0, min_reps=3, max_reps=5)
+        res["cpu_baseline"] = cpu_entry(S / med, "products/s",
+                                        f"the first {S} of the {count} products (the batch's own shape mix), "
+                                        "Blas.MulMatMat in MatFlat's order (oracle/matflat_oracle.c), OpenMP dynamic "
+                                        f"over {threads} threads, median of {reps} runs", reps)
+        res["output_check"] = {"exact_mode_bit_exact_on_sample": bool(np.array_equal(gpu_c, out["c"])),
+                               "sample": S}
+        log(f"  gemm {name}: cpu {res['cpu_baseline']['value']:.0f} products/s; exact-mode bits on the sample "
+            f"{'match' if res['output_check']['exact_mode_bit_exact_on_sample'] else 'DIFFER'}")
     for mode_name, mode in (("fast", hip.MATH_FAST), ("exact", hip.MATH_EXACT)):
         handle.set_math_mode(mode)
         ms = timed_calls(lambda: hip.gemm_vbatched(*args, info, 128, 128, 128, complex_=complex_, handle=handle),
@@ -604,35 +926,105 @@ def gemv_leg(steps: int, warmup: int, count: int = 200_000, seed: int = 6):
     return res
 
 
-def other_legs(steps: int) -> dict:
+LEGS = ("lu", "adjoint", "gemm", "zgemm", "gemv", "trsv")
+
+
+def other_legs(steps: int, cpu: bool = True, which=LEGS) -> dict:
     log("other configurations")
-    return {"lu_n64": lu_leg(steps, 2), "spd_solve_adjoint_n32": adjoint_leg(steps, 2),
-            "gemm_fp64": gemm_leg(max(3, steps // 4), 1),
-            "gemm_complex128": gemm_leg(max(3, steps // 4), 1, complex_=True),
-            "gemv_fp64": gemv_leg(steps, 2), "trsv": trsv_leg(steps, 2)}
+    run = {"lu": ("lu_n64", lambda: lu_leg(steps, 2, cpu=cpu)),
+           "adjoint": ("spd_solve_adjoint_n32", lambda: adjoint_leg(steps, 2, cpu=cpu)),
+           "gemm": ("gemm_fp64", lambda: gemm_leg(max(3, steps // 4), 1, cpu=cpu)),
+           "zgemm": ("gemm_complex128", lambda: gemm_leg(max(3, steps // 4), 1, complex_=True, cpu=cpu)),
+           "gemv": ("gemv_fp64", lambda: gemv_leg(steps, 2)),
+           "trsv": ("trsv", lambda: trsv_leg(steps, 2))}
+    out = {}
+    for w in which:
+        key, fn = run[w]
+        out[key] = fn()
+    return out
 
 
 def cpu_baseline(buf_host, offsets, ns, ldas, xoff, b_host, max_seconds=20.0):
+    """The oracle's Cholesky + two solves (MatFlat's operation order) over the
+    whole batch on the host's cores: OpenMP dynamic scheduling, one matrix per
+    iteration."""
     import oracle
 
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = host_cpu()["threads"]
     count = ns.size
-    reps, elapsed = 0, []
-    t_start = time.perf_counter()
-    while True:
+
+    def run():
         a = buf_host.copy()
         x = b_host.copy()
         t0 = time.perf_counter()
         info = oracle.dposv_batch(ns, ldas, offsets, a, xoff, x, threads=threads)
-        elapsed.append(time.perf_counter() - t0)
-        reps += 1
+        dt = time.perf_counter() - t0
         assert np.all(info == 0)
-        if reps >= 5 or time.perf_counter() - t_start > max_seconds:
-            break
-    med = float(np.median(elapsed))
-    return {"value": count / med, "unit": "solves/s", "cores": threads, "kind": "port",
-            "sample": f"the full {count}-matrix batch, C restatement of MatFlat (oracle/matflat_oracle.c, "
-                      f"-O2 -ffp-contract=off) with OpenMP dynamic scheduling, median of {reps} runs"}
+        return dt
+
+    med, reps = timed_cpu(run, max_seconds, min_reps=3, max_reps=5)
+    return cpu_entry(count / med, "solves/s",
+                     f"the full {count}-matrix batch, C restatement of MatFlat (oracle/matflat_oracle.c, -O2 "
+                     f"-ffp-contract=off) with OpenMP dynamic scheduling over {threads} threads, median of {reps} runs",
+                     reps)
+
+
+class SpdCopies:
+    """Working copies of the SPD batch for in-place solves: `copies` buffers
+    of the matrices and right-hand sides with their pointer arrays and info
+    vectors, restored from the pristine batch by restore(i)."""
+
+    def __init__(self, buf, b, offsets, ns, ldas, xoff, copies):
+        import torch
+
+        dev = buf.device
+        self.src, self.b = buf, b
+        self.copies = copies
+        self.n_t = torch.from_numpy(ns).to(dev)
+        self.lda_t = torch.from_numpy(ldas).to(dev)
+        offs = 8 * torch.from_numpy(offsets).to(dev)
+        xoffs = 8 * torch.from_numpy(xoff).to(dev)
+        self.work = [torch.empty_like(buf) for _ in range(copies)]
+        self.x = [torch.empty_like(b) for _ in range(copies)]
+        self.a_ptrs = [(w.data_ptr() + offs).contiguous() for w in self.work]
+        self.x_ptrs = [(x.data_ptr() + xoffs).contiguous() for x in self.x]
+        self.info = [torch.zeros(ns.size, dtype=torch.int32, device=dev) for _ in range(copies)]
+
+    def restore(self, i):
+        self.work[i].copy_(self.src)
+        self.x[i].copy_(self.b)
+        self.info[i].fill_(-99)
+
+
+def check_spd_outputs(batch, i, offsets, ns, ldas, xoff, sample=256, seed=11):
+    """Checks the outputs a solve left in working copy i: every info zero, and
+    for a seeded sample of matrices the factor residual ||L L^T - A||_1 /
+    (n ||A||_1 eps) <= 10, an exactly zero strict upper triangle and a solve
+    backward error ||b - A x||_inf / (||A||_inf ||x||_inf + ||b||_inf) <= n
+    eps, with A rebuilt from the pristine batch."""
+    eps = np.finfo(np.float64).eps
+    info = batch.info[i].cpu().numpy()
+    rng = np.random.default_rng(seed)
+    pick = rng.choice(ns.size, size=min(sample, ns.size), replace=False)
+    worst_f, worst_b, upper_ok = 0.0, 0.0, True
+    for j in pick:
+        n, lda, o, xo = int(ns[j]), int(ldas[j]), int(offsets[j]), int(xoff[j])
+        a = batch.src[o:o + n * lda].cpu().numpy().reshape(n, lda)[:, :n].T  # a[r, c], upper triangle valid
+        a = np.triu(a) + np.triu(a, 1).T
+        lf = batch.work[i][o:o + n * lda].cpu().numpy().reshape(n, lda)[:, :n].T
+        upper_ok = upper_ok and bool(np.all(np.triu(lf, 1) == 0.0))
+        lo = np.tril(lf)
+        worst_f = max(worst_f, float(np.linalg.norm(lo @ lo.T - a, 1) / (n * np.linalg.norm(a, 1) * eps)))
+        x = batch.x[i][xo:xo + n].cpu().numpy()
+        b = batch.b[xo:xo + n].cpu().numpy()
+        r = b - a @ x
+        be = float(np.abs(r).max() / (np.abs(a).sum(1).max() * np.abs(x).max() + np.abs(b).max()))
+        worst_b = max(worst_b, float(be / (n * eps)))
+    res = {"info_all_zero": bool(np.all(info == 0)), "sampled": int(pick.size),
+           "max_factor_residual_ratio": round(worst_f, 4), "max_backward_error_over_n_eps": round(worst_b, 4),
+           "strict_upper_zero": upper_ok}
+    res["ok"] = bool(res["info_all_zero"] and upper_ok and worst_f <= 10.0 and worst_b <= 1.0)
+    return res
 
 
 def main() -> int:
@@ -642,6 +1034,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--count", type=int, default=100_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copies", type=int, default=2,
+                    help="working copies of the headline batch the timed steps rotate over (at least 2)")
     ap.add_argument("--math", choices=["fast", "exact"], default="fast",
                     help="math mode of the headline run (the other mode is timed as a secondary line)")
     ap.add_argument("--no-secondary", action="store_true")
@@ -649,9 +1043,13 @@ def main() -> int:
                     help="skip the mixed-precision LU-solve comparison (n = 128..512)")
     ap.add_argument("--no-legs", action="store_true",
                     help="skip the LU (n = 64), adjoint (n = 32) and GEMM legs")
-    ap.add_argument("--only", choices=["mixed", "legs"], default=None,
+    ap.add_argument("--only", choices=["mixed", "legs"] + list(LEGS), default=None,
                     help="run only the mixed-precision comparison or only the other-configuration legs and print "
                          "their JSON (for profiling them alone); the default runs everything")
+    ap.add_argument("--save-mixed", default=None, metavar="DIR",
+                    help="generate the mixed-precision batch, write it to DIR and exit")
+    ap.add_argument("--load-mixed", default=None, metavar="DIR",
+                    help="with --only mixed: read the batch --save-mixed wrote instead of generating it")
     ap.add_argument("--bucket-streams", type=int, default=0,
                     help="streams the FAST size-bucket launches are forked over (0 = the library default)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -666,11 +1064,20 @@ def main() -> int:
 
     from matflat_amd import hip
 
-    if args.only == "mixed":
-        print(json.dumps({"mixed_precision": mixed_leg(args.steps, args.warmup)}), flush=True)
+    cpu_on = not args.no_cpu_baseline
+    if args.only is not None and args.calibrate_copy:
+        copy_calibration()
+    if args.save_mixed:
+        save_conditioned_batch(args.save_mixed)
         return 0
-    if args.only == "legs":
-        print(json.dumps({"other_configs": other_legs(args.steps)}), flush=True)
+    if args.only == "mixed":
+        print(json.dumps({"mixed_precision": mixed_leg(args.steps, args.warmup, cpu=cpu_on,
+                                                       batch_dir=args.load_mixed)}, default=_plain), flush=True)
+        return 0
+    if args.only is not None:
+        which = LEGS if args.only == "legs" else (args.only,)
+        print(json.dumps({"other_configs": other_legs(args.steps, cpu=cpu_on, which=which)}, default=_plain),
+              flush=True)
         return 0
 
     seed, nmin, nmax = 2, 10, 100
@@ -680,12 +1087,9 @@ def main() -> int:
     buf_host = buf.cpu().numpy() if not args.no_cpu_baseline else None
     b_host = b.cpu().numpy() if not args.no_cpu_baseline else None
 
-    dev = torch.device("cuda")
-    n_t = torch.from_numpy(ns).to(dev)
-    lda_t = torch.from_numpy(ldas).to(dev)
-    a_ptrs = (buf.data_ptr() + 8 * torch.from_numpy(offsets).to(dev)).contiguous()
-    x_ptrs = (b.data_ptr() + 8 * torch.from_numpy(xoff).to(dev)).contiguous()
-    info = torch.zeros(args.count, dtype=torch.int32, device=dev)
+    copies = max(2, args.copies)
+    batch = SpdCopies(buf, b, offsets, ns, ldas, xoff, copies)
+    log(f"{copies} working copies of {buf.numel() * 8 / 1e9:.2f} GB")
     stream = torch.cuda.current_stream()
     handle = hip.Handle()
     handle.bind_stream(stream)
@@ -693,58 +1097,63 @@ def main() -> int:
         handle.set_bucket_streams(args.bucket_streams)
 
     def run_mode(mode: int, steps: int, warmup: int):
+        """Per-step ms of plan replays on rotated pristine copies (one plan per
+        copy) and the copy the last timed step used."""
         handle.set_math_mode(mode)
-        plan = hip.DposvPlan(n_t, a_ptrs, lda_t, x_ptrs, info, nmax, handle=handle)
-        for _ in range(warmup):
-            plan.execute()
-        torch.cuda.synchronize()
-        if int((info != 0).sum().item()) != 0:
-            raise RuntimeError("factorization failures during warmup")
-        start = torch.cuda.Event(enable_timing=True)
-        stop = torch.cuda.Event(enable_timing=True)
+        plans = [hip.DposvPlan(batch.n_t, batch.a_ptrs[i], batch.lda_t, batch.x_ptrs[i], batch.info[i], nmax,
+                               handle=handle) for i in range(copies)]
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        start.record(stream)
-        for _ in range(steps):
-            plan.execute()
-        stop.record(stream)
-        torch.cuda.synchronize()
+        times, last = rotated_calls(lambda i: plans[i].execute(), batch.restore, copies, steps, warmup, stream)
         wall = time.perf_counter() - t0
-        plan.close()
-        return start.elapsed_time(stop) / steps, wall * 1e3 / steps
+        for p in plans:
+            p.close()
+        return times, last, wall * 1e3 / (steps + warmup)
 
     main_mode = hip.MATH_FAST if args.math == "fast" else hip.MATH_EXACT
-    ms, wall_ms = run_mode(main_mode, args.steps, args.warmup)
-    log(f"{args.math}: {args.steps} steps: {ms:.4f} ms/step (events), wall {wall_ms:.4f} ms/step")
+    times, last, wall_ms = run_mode(main_mode, args.steps, args.warmup)
+    ms = float(np.mean(times))
+    check = check_spd_outputs(batch, last, offsets, ns, ldas, xoff)
+    log(f"{args.math}: {args.steps} steps: {ms:.4f} ms/step mean, {float(np.median(times)):.4f} median (events); "
+        f"wall incl. restores {wall_ms:.3f} ms/step; output check {check}")
+    if not check["ok"]:
+        raise RuntimeError(f"the outputs of the last timed step failed their check: {check}")
     secondary = None
     if not args.no_secondary:
         other = hip.MATH_EXACT if main_mode == hip.MATH_FAST else hip.MATH_FAST
-        oms, _ = run_mode(other, max(3, args.steps // 4), 1)
+        otimes, olast, _ = run_mode(other, max(4, args.steps // 4), 2)
+        oms = float(np.mean(otimes))
+        ocheck = check_spd_outputs(batch, olast, offsets, ns, ldas, xoff, sample=64)
         oname = "exact" if other == hip.MATH_EXACT else "fast"
-        log(f"{oname}: {oms:.4f} ms/step")
+        log(f"{oname}: {oms:.4f} ms/step; output check {ocheck}")
         secondary = {"math": oname, "value": round(args.count / (oms / 1e3), 1), "unit": "solves/s",
-                     "ms_per_step": round(oms, 5),
+                     "ms_per_step": round(oms, 5), "steps": len(otimes), "output_check": ocheck,
                      "note": ("bit-identical to MatFlat's CPU results" if oname == "exact"
                               else "FMA / right-looking, backward error O(n eps)")}
 
     value = args.count / (ms / 1e3)
     log("kernel timing pass")
-    kern = kernel_pass(handle, main_mode, n_t, a_ptrs, lda_t, x_ptrs, info, nmax, ns, args.steps)
+    kern = kernel_pass(handle, main_mode, batch, nmax, ns, args.steps, stream)
     roofline = roofline_of(kern, ns, default_workload=args.count == 100_000)
+    handle.close()
     if args.calibrate_copy:
         copy_calibration()
+    del batch
+    torch.cuda.empty_cache()
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline")
         cpu = cpu_baseline(buf_host, offsets, ns, ldas, xoff, b_host)
+        log(f"  cpu: {cpu['value']:.0f} solves/s on {cpu['cores']} threads ({cpu['cpu_model']})")
+    del buf_host, b_host
     mixed = None
     if not args.no_mixed:
-        mixed = mixed_leg(max(3, args.steps // 4), 1)
+        mixed = mixed_leg(max(3, args.steps // 4), 1, cpu=not args.no_cpu_baseline)
     legs = None
     if not args.no_legs:
         del buf, b
         torch.cuda.empty_cache()
-        legs = other_legs(args.steps)
+        legs = other_legs(args.steps, cpu=not args.no_cpu_baseline)
     out = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -760,8 +1169,10 @@ def main() -> int:
         "data": "synthetic (seeded N(0,1); A = B B^T + n I)",
         "config": {"workload": "batched FP64 Cholesky + forward/back triangular solve, "
                                f"{args.count} SPD matrices, n~U{{{nmin}..{nmax}}}, lda=8*ceil(n/8), one rhs each, "
-                               "one hipGraph replay per step",
-                   "count": args.count, "n_min": nmin, "n_max": nmax, "seed": seed},
+                               "one hipGraph replay per step on a pristine working copy",
+                   "count": args.count, "n_min": nmin, "n_max": nmax, "seed": seed, "working_copies": copies},
+        "ms_per_step_median": round(float(np.median(times)), 5),
+        "output_check": check,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "math": args.math,
@@ -769,7 +1180,7 @@ def main() -> int:
         "mixed_precision": mixed,
         "other_configs": legs,
     }
-    print(json.dumps(out), flush=True)
+    print(json.dumps(out, default=_plain), flush=True)
     return 0
 
 

@@ -84,6 +84,12 @@ def lib() -> ctypes.CDLL:
         _lib.mfo_dposv_batch.argtypes = [i, p, p, p, p, p, p, p, i]
         _lib.mfo_dgetrf_batch.restype = None
         _lib.mfo_dgetrf_batch.argtypes = [i, i, i, i, i64, p, p, p, i]
+        _lib.mfo_dgesv_batch.restype = None
+        _lib.mfo_dgesv_batch.argtypes = [i, i, i, i64, p, p, p, p, i]
+        _lib.mfo_gemm_vbatch.restype = None
+        _lib.mfo_gemm_vbatch.argtypes = [i, i] + [p] * 17 + [i]
+        _lib.mfo_dposv_backward_batch.restype = None
+        _lib.mfo_dposv_backward_batch.argtypes = [i, i, i, i64, p, p, p, p, p, i]
         _lib.mfo_dposv_backward.restype = None
         _lib.mfo_dposv_backward.argtypes = [i, p, i, p, p, p, i, p]
     return _lib
@@ -172,6 +178,49 @@ def dgetrf_batch(count, m, n, lda, stride, a, threads=1):
     sign = np.zeros(count, dtype=np.int32)
     lib().mfo_dgetrf_batch(count, m, n, lda, stride, _ptr(a), _ptr(piv), _ptr(sign), threads)
     return piv, sign
+
+
+def dgesv_batch(count, n, lda, stride, a, b, threads=1):
+    """LU + LU solve of `count` order-n matrices stride elements apart in a
+    (overwritten by the factors); b holds the right-hand sides n apart.
+    Returns (piv, x)."""
+    piv = np.zeros(count * n, dtype=np.int32)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.zeros(count * n, dtype=np.float64)
+    lib().mfo_dgesv_batch(count, n, lda, stride, _ptr(a), _ptr(piv), _ptr(b), _ptr(x), threads)
+    return piv, x
+
+
+def gemm_vbatch(cplx, ta, tb, m, n, k, alpha, a, offa, lda, b, offb, ldb, beta, c, offc, ldc, threads=1):
+    """mfo_dgemm / mfo_zgemm over a batch; a, b, c are float64 buffers (complex
+    entries as (re, im) pairs), offsets count doubles.  Returns info."""
+    def i32(v):
+        return np.ascontiguousarray(v, dtype=np.int32)
+
+    def i64(v):
+        return np.ascontiguousarray(v, dtype=np.int64)
+
+    ta, tb, m, n, k, lda, ldb, ldc = map(i32, (ta, tb, m, n, k, lda, ldb, ldc))
+    offa, offb, offc = map(i64, (offa, offb, offc))
+    alpha = np.ascontiguousarray(alpha, dtype=np.float64)
+    beta = np.ascontiguousarray(beta, dtype=np.float64)
+    info = np.zeros(m.size, dtype=np.int32)
+    lib().mfo_gemm_vbatch(int(cplx), m.size, _ptr(ta), _ptr(tb), _ptr(m), _ptr(n), _ptr(k), _ptr(alpha), _ptr(a),
+                          _ptr(offa), _ptr(lda), _ptr(b), _ptr(offb), _ptr(ldb), _ptr(beta), _ptr(c), _ptr(offc),
+                          _ptr(ldc), _ptr(info), threads)
+    return info
+
+
+def dposv_backward_batch(count, n, lda, stride, l, x, w, threads=1):
+    """mfo_dposv_backward over `count` order-n factors stride elements apart.
+    Returns (abar, bbar) with abar laid out like l."""
+    abar = np.zeros(count * stride, dtype=np.float64)
+    bbar = np.zeros(count * n, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    lib().mfo_dposv_backward_batch(count, n, lda, stride, _ptr(l), _ptr(x), _ptr(w), _ptr(abar), _ptr(bbar),
+                                   threads)
+    return abar, bbar
 
 
 def dposv_backward(n, l, lda, x, w, ldab=None):

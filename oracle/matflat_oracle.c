@@ -373,6 +373,66 @@ void mfo_dgetrf_batch(int count, int m, int n, int lda, int64_t stride, double* 
     (void)threads;
 }
 
+/* LU solve over a fixed-size batch: per matrix mfo_dgetrf then mfo_dgetrs
+ * (the LuInverse composition for one right-hand side), a overwritten by the
+ * factors, piv[i*n..], x[i*n..] from b[i*n..]. */
+void mfo_dgesv_batch(int count, int n, int lda, int64_t stride, double* a, int* piv,
+                     const double* b, double* x, int threads)
+{
+#ifdef _OPENMP
+    if (threads < 1) threads = 1;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads)
+#endif
+    for (int i = 0; i < count; i++) {
+        double* ai = a + (size_t)i * stride;
+        int* pi = piv + (size_t)i * n;
+        mfo_dgetrf(n, n, ai, lda, pi);
+        mfo_dgetrs(n, ai, lda, pi, b + (size_t)i * n, x + (size_t)i * n);
+    }
+    (void)threads;
+}
+
+/* Products C_i = alpha_i op(A_i) op(B_i) + beta_i C_i over a batch (mfo_dgemm,
+ * or mfo_zgemm when cplx): operands at a + offa[i], ... (offsets in doubles;
+ * complex scalars and entries are (re, im) pairs). */
+void mfo_gemm_vbatch(int cplx, int count, const int* ta, const int* tb, const int* m, const int* n,
+                     const int* k, const double* alpha, const double* a, const int64_t* offa,
+                     const int* lda, const double* b, const int64_t* offb, const int* ldb,
+                     const double* beta, double* c, const int64_t* offc, const int* ldc, int* info,
+                     int threads)
+{
+#ifdef _OPENMP
+    if (threads < 1) threads = 1;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(threads)
+#endif
+    for (int i = 0; i < count; i++) {
+        if (cplx)
+            info[i] = mfo_zgemm(ta[i], tb[i], m[i], n[i], k[i], alpha + 2 * (size_t)i, a + offa[i], lda[i],
+                                b + offb[i], ldb[i], beta + 2 * (size_t)i, c + offc[i], ldc[i]);
+        else
+            info[i] = mfo_dgemm(ta[i], tb[i], m[i], n[i], k[i], alpha[i], a + offa[i], lda[i], b + offb[i],
+                                ldb[i], beta[i], c + offc[i], ldc[i]);
+    }
+    (void)threads;
+}
+
+/* mfo_dposv_backward over a fixed-size batch: L, Abar stride elements
+ * apart, x, w, bbar n apart. */
+void mfo_dposv_backward_batch(int count, int n, int lda, int64_t stride, const double* l,
+                              const double* x, const double* w, double* abar, double* bbar,
+                              int threads)
+{
+#ifdef _OPENMP
+    if (threads < 1) threads = 1;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(threads)
+#endif
+    for (int i = 0; i < count; i++) {
+        mfo_dposv_backward(n, l + (size_t)i * stride, lda, x + (size_t)i * n, w + (size_t)i * n,
+                           abar + (size_t)i * stride, lda, bbar + (size_t)i * n);
+    }
+    (void)threads;
+}
+
 /* Adjoint of x = A^-1 b, A = L L^T (no reference counterpart; MatFlat has no
  * derivatives).  bbar = L^-T (L^-1 w) through the same two solves as
  * mfo_dposv; Abar = -(bbar x^T + x bbar^T)/2. */

@@ -88,6 +88,26 @@ void mfo_dposv_batch(int count, const int* n, const int* lda, const int64_t* off
 void mfo_dgetrf_batch(int count, int m, int n, int lda, int64_t stride, double* a,
                       int* piv, int* sign, int threads);
 
+/* mfo_dgetrf + mfo_dgetrs per matrix over a fixed-size batch (stride
+ * elements apart), right-hand sides and solutions n apart. */
+void mfo_dgesv_batch(int count, int n, int lda, int64_t stride, double* a, int* piv,
+                     const double* b, double* x, int threads);
+
+/* mfo_dgemm (cplx = 0) or mfo_zgemm (cplx = 1) over a batch; offsets count
+ * doubles, complex alpha/beta are (re, im) pairs.  info[i] = the call's
+ * return value. */
+void mfo_gemm_vbatch(int cplx, int count, const int* ta, const int* tb, const int* m, const int* n,
+                     const int* k, const double* alpha, const double* a, const int64_t* offa,
+                     const int* lda, const double* b, const int64_t* offb, const int* ldb,
+                     const double* beta, double* c, const int64_t* offc, const int* ldc, int* info,
+                     int threads);
+
+/* mfo_dposv_backward over a fixed-size batch (L and Abar stride elements
+ * apart with leading dimension lda; x, w and bbar n apart). */
+void mfo_dposv_backward_batch(int count, int n, int lda, int64_t stride, const double* l,
+                              const double* x, const double* w, double* abar, double* bbar,
+                              int threads);
+
 /* Adjoint of x = A^-1 b with A = L L^T: given L (lower, as produced by
  * mfo_dpotrf), x and w = dloss/dx, returns bbar = A^-1 w and
  * Abar = -(bbar x^T + x bbar^T)/2 (full symmetric n x n, leading dim ldab). */

@@ -490,3 +490,70 @@ def test_batch_drivers_match_single():
         oracle.dposv(nn, seg, l, xs)
         assert np.array_equal(np.nan_to_num(seg, nan=7.0), np.nan_to_num(a[o:o + l * nn], nan=7.0))
         assert np.array_equal(xs, x[xoffs[i]:xoffs[i] + nn])
+
+
+def test_lu_solve_batch_matches_single_and_solves():
+    """The threaded LU-solve driver gives the same bits as per-matrix
+    dgetrf + dgetrs, and x solves A x = b to a small backward error."""
+    count, n = 6, 13
+    rng = np.random.default_rng(11)
+    a = rng.standard_normal(count * n * n)
+    b = rng.standard_normal(count * n)
+    a0 = a.copy()
+    piv, x = oracle.dgesv_batch(count, n, n, n * n, a, b, threads=3)
+    for i in range(count):
+        seg = a0[i * n * n:(i + 1) * n * n].copy()
+        p, _ = oracle.dgetrf(n, n, seg, n)
+        xs = oracle.dgetrs(n, seg, n, p, b[i * n:(i + 1) * n])
+        assert np.array_equal(seg, a[i * n * n:(i + 1) * n * n])
+        assert np.array_equal(p, piv[i * n:(i + 1) * n])
+        assert np.array_equal(xs, x[i * n:(i + 1) * n])
+        am = a0[i * n * n:(i + 1) * n * n].reshape(n, n).T
+        r = b[i * n:(i + 1) * n] - am @ xs
+        assert np.abs(r).max() <= 1e-13 * (np.abs(am).sum(1).max() * np.abs(xs).max() + 1)
+
+
+def test_gemm_and_backward_batch_drivers_match_single():
+    """The threaded GEMM and adjoint drivers give the same bits as their
+    per-problem calls."""
+    rng = np.random.default_rng(12)
+    for cplx in (0, 1):
+        w = 2 if cplx else 1
+        cnt = 5
+        m, n, k = (rng.integers(1, 9, size=cnt) for _ in range(3))
+        ops = [111, 112, 113] if cplx else [111, 112]
+        ta, tb = rng.choice(ops, size=cnt), rng.choice(ops, size=cnt)
+        ra, ca = np.where(ta == 111, m, k), np.where(ta == 111, k, m)
+        rb, cb = np.where(tb == 111, k, n), np.where(tb == 111, n, k)
+        lda, ldb, ldc = ra + 1, rb + 2, m + 1
+        sa, sb, sc = lda * ca * w, ldb * cb * w, ldc * n * w
+        oa, ob, oc = (np.concatenate([[0], np.cumsum(s)[:-1]]) for s in (sa, sb, sc))
+        a, b, c = (rng.standard_normal(int(s.sum())) for s in (sa, sb, sc))
+        al, be = rng.uniform(-1, 1, cnt * w), rng.uniform(-1, 1, cnt * w)
+        c0 = c.copy()
+        info = oracle.gemm_vbatch(cplx, ta, tb, m, n, k, al, a, oa, lda, b, ob, ldb, be, c, oc, ldc, threads=2)
+        assert np.all(info == 0)
+        for i in range(cnt):
+            ci = c0[oc[i]:oc[i] + sc[i]].copy()
+            ai, bi = a[oa[i]:oa[i] + sa[i]], b[ob[i]:ob[i] + sb[i]]
+            if cplx:
+                cz = ci.view(np.complex128)
+                oracle.zgemm(int(ta[i]), int(tb[i]), int(m[i]), int(n[i]), int(k[i]), complex(al[2 * i], al[2 * i + 1]),
+                             ai.view(np.complex128), int(lda[i]), bi.view(np.complex128), int(ldb[i]),
+                             complex(be[2 * i], be[2 * i + 1]), cz, int(ldc[i]))
+            else:
+                oracle.dgemm(int(ta[i]), int(tb[i]), int(m[i]), int(n[i]), int(k[i]), al[i], ai.copy(), int(lda[i]),
+                             bi.copy(), int(ldb[i]), be[i], ci, int(ldc[i]))
+            assert np.array_equal(ci, c[oc[i]:oc[i] + sc[i]]), (cplx, i)
+    cnt, n, lda = 4, 7, 8
+    ls = np.concatenate([spd_upper_buf(40 + i, n, lda) for i in range(cnt)])
+    for i in range(cnt):
+        seg = ls[i * n * lda:(i + 1) * n * lda]
+        assert oracle.dpotrf(n, seg, lda) == 0
+    x, wv = rng.standard_normal(cnt * n), rng.standard_normal(cnt * n)
+    abar, bbar = oracle.dposv_backward_batch(cnt, n, lda, n * lda, ls, x, wv, threads=2)
+    for i in range(cnt):
+        ab, bb = oracle.dposv_backward(n, ls[i * n * lda:(i + 1) * n * lda].copy(), lda, x[i * n:(i + 1) * n],
+                                       wv[i * n:(i + 1) * n], ldab=lda)
+        assert np.array_equal(ab, abar[i * n * lda:(i + 1) * n * lda])
+        assert np.array_equal(bb, bbar[i * n:(i + 1) * n])
