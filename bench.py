@@ -315,11 +315,12 @@ def mfma_busy(kernel: str):
     workload, from the committed rocprofv3 --pmc summary; None when absent."""
     if not os.path.exists(MFMA_PROFILE):
         return None
+    import re
+
     for line in open(MFMA_PROFILE):
-        if line.startswith("mfma_busy:"):
-            parts = dict(kv.split("=", 1) for kv in line.split()[1:])
-            if parts.get("kernel") == kernel:
-                return float(parts["frac"])
+        m = re.match(r"mfma_busy: kernel=(.+) frac=(\S+)\s*$", line)
+        if m and m.group(1) == kernel:
+            return float(m.group(2))
     return None
 
 

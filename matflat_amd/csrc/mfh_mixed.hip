@@ -336,10 +336,12 @@ __device__ __forceinline__ double recip(double d)
     return __builtin_fma(__builtin_fma(-d, r, 1.0), r, r);
 }
 
-#define MFH_DPP_MAX_U32(v, CTRL, RMASK)                                                   \
-    do {                                                                                \
-        const unsigned o_ = __builtin_amdgcn_update_dpp((v), (v), CTRL, RMASK, 0xF, false); \
-        (v) = o_ > (v) ? o_ : (v);                                                      \
+// (lanes the row mask leaves out read 0, the identity of the unsigned
+// maximum, so the move and the maximum can fuse into one DPP instruction)
+#define MFH_DPP_MAX_U32(v, CTRL, RMASK)                                                 \
+    do {                                                                              \
+        const unsigned o_ = __builtin_amdgcn_update_dpp(0u, (v), CTRL, RMASK, 0xF, false); \
+        (v) = o_ > (v) ? o_ : (v);                                                    \
     } while (0)
 
 // Wave-wide maximum through DPP row shifts and row broadcasts (no LDS
@@ -374,11 +376,14 @@ __device__ __forceinline__ void panel_cols(T (&P)[Prec<T>::NB], int r, bool& liv
             T* cand = L.cand + (J & 1) * NWAVE * NB;
             unsigned* ckey = (unsigned*)(L.ckey + (J & 1) * NWAVE);
             int* cidx = L.cidx + (J & 1) * NWAVE;
-            // wavefronts with no row of the panel only keep the barrier
-            // company; wavefront 0 always has rows (k0 < n)
+            // wavefront 0 always has rows (k0 < n)
             const int nwl = (n - k0 + 63) >> 6;  // wavefronts holding rows k0 .. n-1
-            const bool wlive = wave_id() < nwl;
-            if (wlive) {
+            // Every wavefront runs the column, also those without rows in
+            // k0 .. n-1 (their rows are not live: key 0, masked below, and
+            // l = 0): a branch around it is divergent to the compiler, which
+            // then copies the panel row in and out of it and serializes the
+            // pivot-row loads behind the copies.
+            {
                 // wave argmax: the largest 32-bit key, then the lowest lane
                 // that holds it (rows are in lane order, so the lowest row
                 // wins ties)
@@ -386,23 +391,49 @@ __device__ __forceinline__ void panel_cols(T (&P)[Prec<T>::NB], int r, bool& liv
                 const unsigned m = wave_max_u32(k32);
                 const int win = __builtin_ctzll(__ballot(k32 == m));
                 if ((t & 63) == win) {
+                    // 16-byte stores from the aligned entry at or below J
+                    // (the entries below J are never read): NB / VW - J / VW
+                    // stores with immediate offsets
+                    typedef T vec_t __attribute__((ext_vector_type(16 / sizeof(T))));
+                    constexpr int VW = 16 / sizeof(T);
+                    vec_t* cv = (vec_t*)(cand + w * NB);
 #pragma unroll
-                    for (int c = J; c < NB; ++c) cand[w * NB + c] = P[c];
+                    for (int q = J / VW; q < NB / VW; ++q) {
+                        vec_t v;
+#pragma unroll
+                        for (int e = 0; e < VW; ++e) v[e] = P[q * VW + e];
+                        cv[q] = v;
+                    }
                     ckey[w] = m;
                     cidx[w] = r;
                 }
             }
             __syncthreads();
-            if (wlive) {
-                unsigned best = ckey[0];
-                for (int ww = 1; ww < nwl; ++ww) best = ckey[ww] > best ? ckey[ww] : best;
+            {
+                // every wavefront's key in one LDS round trip (a loop over
+                // the nwl live slots would wait for each load in turn); the
+                // slots of wavefronts without rows are masked
+                unsigned best = 0u;
+#pragma unroll
+                for (int ww = 0; ww < NWAVE; ++ww) {
+                    const unsigned kw = ckey[ww];
+                    best = (ww < nwl && kw > best) ? kw : best;
+                }
+                best = __builtin_amdgcn_readfirstlane(best);
                 const int wb = 7 - (int)(best & 7u);  // the winning wavefront
                 const T* prow = cand + wb * NB;
                 // a live row always wins (n - k0 - J >= 1 rows are live); the
                 // clamp only keeps the index in range whatever the keys hold
                 int p = cidx[wb];
+                // the pivot row's entries, all loaded before any is used:
+                // left to itself the scheduler pairs each load with its FMA
+                // and waits for every pair in turn
+                T pr[NB];
+#pragma unroll
+                for (int c = J; c < NB; ++c) pr[c] = prow[c];
+                __builtin_amdgcn_sched_barrier(0);
                 p = p < k0 ? k0 : (p >= n ? n - 1 : p);
-                const T d = prow[J];
+                const T d = pr[J];
                 if (t == 0) L.pivs[J] = p;
                 if (d == (T)0 && zero == 0) zero = k0 + J + 1;
                 if (r == p) live = false;
@@ -415,7 +446,7 @@ __device__ __forceinline__ void panel_cols(T (&P)[Prec<T>::NB], int r, bool& liv
                 const T l = live ? q : (T)0;
                 P[J] = live ? q : P[J];
 #pragma unroll
-                for (int c = J + 1; c < NB; ++c) P[c] -= l * prow[c];
+                for (int c = J + 1; c < NB; ++c) P[c] -= l * pr[c];
             }
             panel_cols<T, J + 1>(P, r, live, n, k0, nb, L, zero);
         }
@@ -476,12 +507,28 @@ __device__ __forceinline__ int panel(T* W, int ldw, int n, int k0, int nb, const
     const bool mine = r < n;
     T P[NB];
     {
-        // clamped addresses, selected values: branch-free loads
+        // clamped addresses, selected values: branch-free loads.  A full
+        // panel is read as 16-byte vectors (rows are 16-byte aligned: ldw and
+        // k0 are multiples of 8): every load instruction touches one cache
+        // line per lane, so a scalar load per entry would cost the address
+        // unit NB passes over the same 64 lines instead of NB / VW.
         const int rc = mine ? r : n - 1;
+        if (nb == NB) {
+            typedef T vec_t __attribute__((ext_vector_type(16 / sizeof(T))));
+            constexpr int VW = 16 / sizeof(T);
+            const vec_t* src = (const vec_t*)(W + (size_t)rc * ldw + k0);
 #pragma unroll
-        for (int c = 0; c < NB; ++c) {
-            const T v = W[(size_t)rc * ldw + (k0 + c < n ? k0 + c : n - 1)];
-            P[c] = (mine && c < nb) ? v : (T)0;
+            for (int q = 0; q < NB / VW; ++q) {
+                const vec_t v = gmem(src)[q];
+#pragma unroll
+                for (int e = 0; e < VW; ++e) P[q * VW + e] = mine ? v[e] : (T)0;
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < NB; ++c) {
+                const T v = W[(size_t)rc * ldw + (k0 + c < n ? k0 + c : n - 1)];
+                P[c] = (mine && c < nb) ? v : (T)0;
+            }
         }
     }
     int zero = 0;
@@ -494,8 +541,16 @@ __device__ __forceinline__ int panel(T* W, int ldw, int n, int k0, int nb, const
     if (mine) {
         const int d = L.dst[t];
         if (nb == NB) {
+            typedef T vec_t __attribute__((ext_vector_type(16 / sizeof(T))));
+            constexpr int VW = 16 / sizeof(T);
+            vec_t* dstp = (vec_t*)(W + (size_t)d * ldw + k0);
 #pragma unroll
-            for (int c = 0; c < NB; ++c) W[(size_t)d * ldw + k0 + c] = P[c];
+            for (int q = 0; q < NB / VW; ++q) {
+                vec_t v;
+#pragma unroll
+                for (int e = 0; e < VW; ++e) v[e] = P[q * VW + e];
+                gmem_w(dstp)[q] = v;
+            }
         } else {
             for (int c = 0; c < nb; ++c) {
                 T v = P[0];
@@ -807,25 +862,60 @@ __device__ __forceinline__ void trailing_f32_valu(float* W, int ldw, int n, int 
 }
 
 // ---------------------------------------------------------------- solve
-// sum over c = c0, c0+step, ... < c1 of row[c] * y[c], with four
-// independent partial sums (several loads in flight).
-template <typename T>
-__device__ __forceinline__ double gemv_row(const T* row, const T* y, int c0, int c1, int step)
+// The value lane j (a constant) holds, for every lane.
+__device__ __forceinline__ float readlane_t(float v, int j)
 {
-    double s[8];
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
+}
+__device__ __forceinline__ double readlane_t(double v, int j)
+{
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), j);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), j);
+    return __hiloint2double(hi, lo);
+}
+
+// sum over c in [c0, c1) of row[c] * y[c], split over TPR threads (this one
+// is q): thread q takes the VW-entry chunks at c0 + q VW + k TPR VW, loaded
+// as 16-byte vectors, GU chunks in flight (c0 is a multiple of TPR VW, so
+// every chunk is aligned); a chunk that crosses c1 is summed entry by entry.
+// GU = 4 keeps the solve launch within its 80-register budget.
+#ifndef MFH_GEMV_U
+#define MFH_GEMV_U 4
+#endif
+template <typename T, int TPR>
+__device__ __forceinline__ double gemv_row(const T* row, const T* y, int c0, int c1, int q)
+{
+    typedef T vec_t __attribute__((ext_vector_type(16 / sizeof(T))));
+    constexpr int VW = 16 / sizeof(T);
+    constexpr int S = TPR * VW;
+    constexpr int GU = MFH_GEMV_U;
+    double s[GU];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) s[u] = 0.0;
-    int c = c0;
-    for (; c + 7 * step < c1; c += 8 * step) {
-        T rv[8];
+    for (int u = 0; u < GU; ++u) s[u] = 0.0;
+    int c = c0 + q * VW;
+    for (; c + (GU - 1) * S + VW <= c1; c += GU * S) {
+        vec_t rv[GU];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) rv[u] = gmem(row)[c + u * step];
-        __builtin_amdgcn_sched_barrier(0);  // the eight loads go out together
+        for (int u = 0; u < GU; ++u) rv[u] = *(const __attribute__((address_space(1))) vec_t*)(gmem(row) + c + u * S);
+        __builtin_amdgcn_sched_barrier(0);  // the loads go out together
 #pragma unroll
-        for (int u = 0; u < 8; ++u) s[u] += (double)rv[u] * (double)y[c + u * step];
+        for (int u = 0; u < GU; ++u)
+#pragma unroll
+            for (int e = 0; e < VW; ++e) s[u] += (double)rv[u][e] * (double)y[c + u * S + e];
     }
-    for (; c < c1; c += step) s[0] += (double)row[c] * (double)y[c];
-    return ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
+    for (; c < c1; c += S) {
+        if (c + VW <= c1) {
+            const vec_t v = *(const __attribute__((address_space(1))) vec_t*)(gmem(row) + c);
+#pragma unroll
+            for (int e = 0; e < VW; ++e) s[e % GU] += (double)v[e] * (double)y[c + e];
+        } else {
+            for (int e = 0; c + e < c1; ++e) s[0] += (double)gmem(row)[c + e] * (double)y[c + e];
+        }
+    }
+    double t = 0.0;
+#pragma unroll
+    for (int u = 0; u < GU; ++u) t += s[u];
+    return t;
 }
 
 // yv <- U^-1 L^-1 yv with the factors in W (row-major, unit-lower L), in
@@ -862,19 +952,22 @@ __device__ __forceinline__ void lu_solve(const T* W, int ldw, int n, const Lds<T
         if (r0 > 0) {
             const int r = r0 + t / TPR, q = t % TPR;
             double s = 0.0;
-            if (r < r0 + rows) s = gemv_row(W + (size_t)r * ldw, L.yv, q, r0, TPR);
+            if (r < r0 + rows) s = gemv_row<T, TPR>(W + (size_t)r * ldw, L.yv, 0, r0, q);
 #pragma unroll
             for (int off = 1; off < TPR; off <<= 1) s += __shfl_xor(s, off, 64);
             if (q == 0 && r < r0 + rows) L.yv[r] = (T)((double)L.yv[r] - s);
         }
         __syncthreads();
         if (w == 0) {
+            // j is wave-uniform, so lane j's value reaches the others
+            // through v_readlane (lane index in a scalar register) instead
+            // of an LDS permute
             const int r = r0 + lane;
             T y = (lane < rows) ? L.yv[r0 + (lane < rows ? lane : 0)] : (T)0;
             const T* drow = D + lane * DL;
 #pragma unroll 8
             for (int j = 0; j < BS; ++j) {
-                const T yj = __shfl(y, j, 64);
+                const T yj = readlane_t(y, j);
                 y -= drow[j] * yj;
             }
             if (lane < rows) L.yv[r] = y;
@@ -890,22 +983,26 @@ __device__ __forceinline__ void lu_solve(const T* W, int ldw, int n, const Lds<T
         if (c1 < n) {
             const int r = r0 + t / TPR, q = t % TPR;
             double s = 0.0;
-            if (r < c1) s = gemv_row(W + (size_t)r * ldw, L.yv, c1 + q, n, TPR);
+            if (r < c1) s = gemv_row<T, TPR>(W + (size_t)r * ldw, L.yv, c1, n, q);
 #pragma unroll
             for (int off = 1; off < TPR; off <<= 1) s += __shfl_xor(s, off, 64);
             if (q == 0 && r < c1) L.yv[r] = (T)((double)L.yv[r] - s);
         }
         __syncthreads();
         if (w == 0) {
+            // each lane's reciprocal of its diagonal entry first (one division
+            // for the wavefront instead of one per column); lane j scales by
+            // it when its value is complete
             const int r = r0 + lane;
             T y = (lane < rows) ? L.yv[r0 + (lane < rows ? lane : 0)] : (T)0;
             const T* drow = D + lane * DL;
+            const T dd = drow[lane < BS ? lane : 0];
+            const T rd = (dd != (T)0) ? (T)1 / dd : (T)1;
 #pragma unroll 8
             for (int j = BS - 1; j >= 0; --j) {
-                const T dj = drow[j];
-                if (lane == j) y = (dj != (T)0) ? y / dj : y;
-                const T yj = __shfl(y, j, 64);
-                if (lane < j) y -= dj * yj;
+                if (lane == j) y *= rd;
+                const T yj = readlane_t(y, j);
+                if (lane < j) y -= drow[j] * yj;
             }
             if (lane < rows) L.yv[r] = y;
         }
@@ -1091,8 +1188,11 @@ __device__ __forceinline__ void gesv_step(const GesvArgs& G, const Lds<T>& L, in
 
 // The persistent loops have a single back edge and every branch around a
 // barrier is on a wave-uniform value.
+#ifndef MFH_GESV_WAVES
+#define MFH_GESV_WAVES 2
+#endif
 template <typename T, bool MIXED>
-__global__ __launch_bounds__(BT) void k_gesv(GesvArgs G)
+__global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(MFH_GESV_WAVES))) void k_gesv(GesvArgs G)
 {
     extern __shared__ __attribute__((aligned(16))) char lds_raw[];
     const Lds<T> L = carve<T>(lds_raw, G.ldw);
