@@ -65,6 +65,23 @@ def _dev(t: torch.Tensor) -> torch.device:
     return t.device
 
 
+_exact_handle: hip.Handle | None = None
+
+
+def _exact() -> hip.Handle:
+    """The handle every drop-in call runs on: its own, always in exact math
+    mode, so a caller that switches the default handle to FAST mode does not
+    change these results (bit-identical to MatFlat, partial writes on a
+    failed Cholesky as MatFlat leaves them before its throw at
+    Factorization.Cholesky.cs:156)."""
+    global _exact_handle
+    if _exact_handle is None:
+        _exact_handle = hip.Handle()
+        _exact_handle.set_math_mode(hip.MATH_EXACT)
+    _exact_handle.bind_stream()
+    return _exact_handle
+
+
 def _ptrs(*ts: torch.Tensor) -> torch.Tensor:
     return torch.tensor([t.data_ptr() for t in ts], dtype=torch.int64, device=ts[0].device)
 
@@ -97,7 +114,7 @@ class Factorization:
         dev = _dev(a)
         sign = torch.zeros(1, dtype=torch.int32, device=dev)
         info = torch.zeros(1, dtype=torch.int32, device=dev)
-        hip.dgetrf_strided_batched(1, m, n, a, lda, lda * n, piv, m, sign, info)
+        hip.dgetrf_strided_batched(1, m, n, a, lda, lda * n, piv, m, sign, info, handle=_exact())
         torch.cuda.synchronize()
         return int(sign.item())
 
@@ -113,7 +130,7 @@ class Factorization:
         _need(a, lda * (n - 1) + n, "a")
         dev = _dev(a)
         info = torch.zeros(1, dtype=torch.int32, device=dev)
-        hip.dpotrf_vbatched(_i32(n, dev), _ptrs(a), _i32(lda, dev), info, n)
+        hip.dpotrf_vbatched(_i32(n, dev), _ptrs(a), _i32(lda, dev), info, n, handle=_exact())
         if int(info.item()) != 0:
             raise MatrixFactorizationException(
                 "Cholesky decomposition failed. The matrix must be positive definite.")
@@ -135,7 +152,7 @@ class Factorization:
         _need(piv, n, "piv")
         dev = _dev(a)
         info = torch.zeros(1, dtype=torch.int32, device=dev)
-        hip.dgetri_vbatched(_i32(n, dev), _ptrs(a), _i32(lda, dev), _ptrs(piv), info, n)
+        hip.dgetri_vbatched(_i32(n, dev), _ptrs(a), _i32(lda, dev), _ptrs(piv), info, n, handle=_exact())
         if int(info.item()) != 0:
             raise MatrixFactorizationException("The matrix is not invertible.")
 
@@ -164,7 +181,7 @@ class Blas:
         dev = _dev(a)
         info = torch.zeros(1, dtype=torch.int32, device=dev)
         hip.dtrsv_vbatched(int(uplo), int(transa), hip.NONUNIT, _i32(n, dev), _ptrs(a), _i32(lda, dev), _ptrs(x),
-                           _i32(incx, dev), info, n)
+                           _i32(incx, dev), info, n, handle=_exact())
         torch.cuda.synchronize()
 
     @staticmethod
@@ -205,7 +222,7 @@ class Blas:
         zero = torch.zeros(2 if cplx else 1, dtype=torch.float64, device=dev)
         hip.gemm_vbatched(_i32(int(transa), dev), _i32(int(transb), dev), _i32(m, dev), _i32(n, dev), _i32(k, dev),
                           one, _ptrs(a), _i32(lda, dev), _ptrs(b), _i32(ldb, dev), zero, _ptrs(c), _i32(ldc, dev),
-                          info, m, n, k, complex_=cplx)
+                          info, m, n, k, complex_=cplx, handle=_exact())
         torch.cuda.synchronize()
 
     @staticmethod
@@ -241,5 +258,5 @@ class Blas:
         dev = _dev(a)
         info = torch.zeros(1, dtype=torch.int32, device=dev)
         hip.dgemv_vbatched(_i32(int(trans), dev), _i32(m, dev), _i32(n, dev), _ptrs(a), _i32(lda, dev), _ptrs(x),
-                           _i32(incx, dev), _ptrs(y), _i32(incy, dev), info, m, n)
+                           _i32(incx, dev), _ptrs(y), _i32(incy, dev), info, m, n, handle=_exact())
         torch.cuda.synchronize()

@@ -555,6 +555,40 @@ def test_matflat_signatures_on_device():
         Blas.MulMatMat(Transpose.NoTrans, Transpose.NoTrans, 2, 2, 2, aa, 2, aa, 2, aa, 2)
 
 
+def test_matflat_dropin_ignores_fast_default_handle():
+    """The drop-in calls run in exact mode whatever the default handle's
+    mode: a Cholesky that fails keeps MatFlat's partial writes (FAST mode
+    would leave the matrix unchanged) and an LU stays bit-identical."""
+    from matflat_amd import hip
+    from matflat_amd.matflat import Factorization, MatrixFactorizationException
+    dh = hip.default_handle()
+    old = dh.math_mode()
+    dh.set_math_mode(hip.MATH_FAST)
+    try:
+        n = 6
+        rng = np.random.default_rng(7)
+        b = rng.standard_normal((n, n))
+        s = b @ b.T + n * np.eye(n)
+        s[4, 4] = -50.0  # row 4 fails after rows 0..3 are written
+        a = np.asfortranarray(s).reshape(-1, order="F").copy()
+        want = a.copy()
+        assert oracle.dpotrf(n, want, n) == 5
+        dev = torch.from_numpy(a.copy()).cuda()
+        with pytest.raises(MatrixFactorizationException):
+            Factorization.Cholesky(n, dev, n)
+        assert_bits_equal(dev.cpu().numpy(), want, "partial writes")
+        m = 40
+        la = rand_buf(11, m, m, m)
+        dev = torch.from_numpy(la.copy()).cuda()
+        piv = torch.zeros(m, dtype=torch.int32, device="cuda")
+        sign = Factorization.Lu(m, m, dev, m, piv)
+        wpiv, wsign = oracle.dgetrf(m, m, la, m)
+        assert_bits_equal(dev.cpu().numpy(), la, "Lu")
+        assert np.array_equal(piv.cpu().numpy(), wpiv) and sign == wsign
+    finally:
+        dh.set_math_mode(old)
+
+
 # ------------------------------------------------------------------ GEMV
 @pytest.mark.parametrize("seed", [0, 1])
 def test_gemv_variable_sizes_bit_exact(seed):
