@@ -52,8 +52,11 @@ extern "C" {
 #define MFH_NONUNIT 131
 #define MFH_UNIT 132
 
-/* Largest order the exact (MatFlat-order) factorization kernels take. */
-#define MFH_EXACT_MAX_N 128
+/* Largest order of the MatFlat-order (exact) LU, Cholesky, SPD solve,
+ * triangular solve and LU solve, double and complex. */
+#define MFH_EXACT_MAX_N 1024
+/* Largest order of the LU inverse and of the SPD-solve adjoint. */
+#define MFH_SMALL_MAX_N 128
 /* Largest order of the blocked MFMA factorizations (mixed precision). */
 #define MFH_BLOCKED_MAX_N 512
 
@@ -122,6 +125,10 @@ int mfh_get_bucket_streams(mfh_handle* handle, int* streams);
 #define MFH_KERNEL_GEMV_EXACT 10
 #define MFH_KERNEL_GETRI_SETUP 11
 #define MFH_KERNEL_GETRI_EXACT 12
+#define MFH_KERNEL_GETRF_GEN 13
+#define MFH_KERNEL_POTRF_GEN 14
+#define MFH_KERNEL_TRSV_GEN 15
+#define MFH_KERNEL_GETRS_PERMUTE 16
 int mfh_set_timing(mfh_handle* handle, int enable);
 int mfh_get_timing(mfh_handle* handle, int capacity, int* n_records, int* kind, int* rows,
                    float* ms);
@@ -160,7 +167,8 @@ int mfh_dpotrf_vbatched(mfh_handle* h, int count, const int* n, double* const* a
  * library, one copy per GPU, reset by a kernel at the start of every call:
  * calls that overlap on one GPU (on different streams or handles) race on
  * it.  It creates no stream or event: the size buckets run in order unless
- * the handle already owns its side streams (from an earlier FAST call). */
+ * the handle already owns its side streams (from an earlier FAST call).
+ * max_n <= MFH_SMALL_MAX_N (larger orders need a workspace; -8 otherwise). */
 #define MFH_NOALLOC_MAX_COUNT 1048576
 int mfh_dposv_vbatched_noalloc(mfh_handle* h, int count, const int* n, double* const* a,
                                const int* lda, double* const* x, int* info, int max_n);
@@ -200,7 +208,7 @@ int mfh_dgetrs_vbatched(mfh_handle* h, int count, const int* n, const double* co
  * A matrix whose U has a zero diagonal entry gets info = j + 1 for the first
  * such j and is left unchanged (MatFlat throws "The matrix is not
  * invertible." before writing).  Padding rows are never touched.
- * n <= MFH_EXACT_MAX_N; count * max_n < 2^31.  Staging lives in a second
+ * n <= MFH_SMALL_MAX_N; count * max_n < 2^31.  Staging lives in a second
  * handle buffer of about count * max_n^2 * 8 bytes. */
 int mfh_dgetri_vbatched(mfh_handle* h, int count, const int* n, double* const* a, const int* lda,
                         const int* const* piv, int* info, int max_n);
@@ -242,6 +250,36 @@ int mfh_zgemm_vbatched(mfh_handle* h, int count, const int* transa, const int* t
                        const double* const* a, const int* lda, const double* const* b,
                        const int* ldb, const double* beta, double* const* c, const int* ldc,
                        int* info, int max_m, int max_n, int max_k);
+
+/* ---- complex128 LU, Cholesky and triangular solve ----------------------------
+ * a and x hold interleaved (re, im) doubles (System.Numerics.Complex's
+ * layout).  Bit-identical to MatFlat: products, sums and divisions restate
+ * the BCL's Complex operators (division by Smith's method), dot products
+ * keep Internals.Dot / DotConj's pairing.
+ *
+ * Factorization.Lu(int m, int n, Complex* a, int lda, int* piv)
+ * (Factorization.Lu.cs:199-238, LuCore :352-406): as mfh_dgetrf_vbatched,
+ * with the pivot chosen by |re| + |im| (Internals.FastMagnitude,
+ * Internals.cs:15-19).  m, n <= MFH_EXACT_MAX_N. */
+int mfh_zgetrf_vbatched(mfh_handle* h, int count, const int* m, const int* n,
+                        double* const* a, const int* lda, int* const* piv,
+                        int* sign, int* info, int max_m, int max_n);
+/* Factorization.Cholesky(int n, Complex* a, int lda)
+ * (Factorization.Cholesky.cs:195-249): A = L L^H from the upper triangle
+ * (only the real part of the diagonal is read); L's diagonal is real.  On
+ * failure at row j, info = j+1 and the matrix holds MatFlat's partial
+ * writes.  n <= MFH_EXACT_MAX_N. */
+int mfh_zpotrf_vbatched(mfh_handle* h, int count, const int* n, double* const* a,
+                        const int* lda, int* info, int max_n);
+/* Blas.SolveTriangular(Uplo, Transpose, int n, Complex* a, int lda,
+ * Complex* x, int incx) (Blas.SolveTriangular.cs:280-395, MulSub :397-421,
+ * MulSubConj :423-445) for a batch; trans is any of MFH_NOTRANS, MFH_TRANS,
+ * MFH_CONJTRANS, MFH_CONJNOTRANS (the Conj forms use conj(A)).  Non-unit
+ * diagonal, as the reference.  One (uplo, trans) for the whole batch;
+ * n <= MFH_EXACT_MAX_N. */
+int mfh_ztrsv_vbatched(mfh_handle* h, int uplo, int trans, int count, const int* n,
+                       const double* const* a, const int* lda, double* const* x,
+                       const int* incx, int* info, int max_n);
 
 /* ---- mixed-precision LU solve with FP64 iterative refinement --------------
  * No single MatFlat counterpart: it composes Factorization.Lu and the LU
@@ -287,7 +325,7 @@ int mfh_dgetrf_blocked_vbatched(mfh_handle* h, int count, const int* n, double* 
  * A = L L^T (l from mfh_dposv_vbatched / mfh_dpotrf_vbatched) and
  * xbar = dloss/dx: bbar = A^-1 xbar and
  * abar = -(bbar x^T + x bbar^T)/2, written as a full symmetric n x n matrix
- * with leading dimension ldabar.  n <= MFH_EXACT_MAX_N. */
+ * with leading dimension ldabar.  n <= MFH_SMALL_MAX_N. */
 int mfh_dposv_backward_vbatched(mfh_handle* h, int count, const int* n, const double* const* l,
                                 const int* lda, const double* const* x, const double* const* xbar,
                                 double* const* abar, const int* ldabar, double* const* bbar,

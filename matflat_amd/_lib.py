@@ -21,18 +21,20 @@ MFH_ERR_STATE = -1004
 NOTRANS, TRANS, CONJTRANS, CONJNOTRANS = 111, 112, 113, 114
 UPPER, LOWER = 121, 122
 NONUNIT, UNIT = 131, 132
-EXACT_MAX_N = 128
+EXACT_MAX_N = 1024
+SMALL_MAX_N = 128
+LDS_MAX_N = 128  # larger orders run the generic kernels (k_getrf_gen, k_potrf_gen, k_trsv_gen)
 MATH_EXACT, MATH_FAST = 0, 1
 NOALLOC_MAX_COUNT = 1 << 20
 BLOCKED_MAX_N = 512
 KERNEL_SORT, KERNEL_CHOL_EXACT, KERNEL_CHOL_FAST, KERNEL_GETRF_EXACT, KERNEL_TRSV_EXACT, KERNEL_GETRS_EXACT = \
     1, 2, 3, 4, 5, 6
 KERNEL_GESV_MIXED, KERNEL_GESV_F64, KERNEL_GESV_STEP, KERNEL_GEMV_EXACT, KERNEL_GETRI_SETUP = 7, 8, 9, 10, 11
-KERNEL_GETRI_EXACT = 12
+KERNEL_GETRI_EXACT, KERNEL_GETRF_GEN, KERNEL_POTRF_GEN, KERNEL_TRSV_GEN, KERNEL_GETRS_PERMUTE = 12, 13, 14, 15, 16
 KERNEL_NAMES = {1: "size sort", 2: "k_chol_packed", 3: "k_chol_fast", 4: "k_getrf_exact", 5: "k_trsv_exact",
                 6: "k_getrs_exact", 7: "k_gesv<float, mixed>", 8: "k_gesv<double>", 9: "k_gesv_step",
-                10: "k_dgemv_exact", 11: "k_getri_setup",
-                12: "k_getri_exact"}
+                10: "k_dgemv_exact", 11: "k_getri_setup", 12: "k_getri_exact", 13: "k_getrf_gen",
+                14: "k_potrf_gen", 15: "k_trsv_gen", 16: "k_getrs_permute"}
 
 _i, _d, _p, _i64 = ctypes.c_int, ctypes.c_double, ctypes.c_void_p, ctypes.c_int64
 
@@ -63,6 +65,9 @@ SIGNATURES = {
     "mfh_zgemm_vbatched": [_p, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i, _i, _i],
     "mfh_dgetri_vbatched": [_p, _i, _p, _p, _p, _p, _p, _i],
     "mfh_dgemv_vbatched": [_p, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i, _i],
+    "mfh_zgetrf_vbatched": [_p, _i, _p, _p, _p, _p, _p, _p, _p, _i, _i],
+    "mfh_zpotrf_vbatched": [_p, _i, _p, _p, _p, _p, _i],
+    "mfh_ztrsv_vbatched": [_p, _i, _i, _i, _p, _p, _p, _p, _p, _p, _i],
     "mfh_dgesv_mixed_vbatched": [_p, _i, _p, _p, _p, _p, _p, _i, _d, _p, _p, _p, _i],
     "mfh_dgesv_vbatched": [_p, _i, _p, _p, _p, _p, _p, _p, _p, _i],
     "mfh_dgetrf_blocked_vbatched": [_p, _i, _p, _p, _p, _p, _p, _i],

@@ -33,8 +33,8 @@ def _check_batch(a: torch.Tensor, b: torch.Tensor) -> None:
         raise ValueError("a must have shape (count, n, n)")
     if b.shape != a.shape[:2]:
         raise ValueError("b must have shape (count, n)")
-    if a.shape[1] < 1 or a.shape[1] > hip.EXACT_MAX_N:
-        raise ValueError(f"n must be in [1, {hip.EXACT_MAX_N}]")
+    if a.shape[1] < 1 or a.shape[1] > hip.SMALL_MAX_N:
+        raise ValueError(f"n must be in [1, {hip.SMALL_MAX_N}]")
 
 
 class SpdSolve(torch.autograd.Function):

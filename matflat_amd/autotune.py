@@ -72,8 +72,8 @@ def main(argv=None) -> int:
     h.set_math_mode(hip.MATH_FAST)
     table = {}
     for rb in range(1, args.max_rb + 1):
-        # SPD solve: n + 1 rows in (8(rb-1), 8rb]; n <= MFH_EXACT_MAX_N
-        nmin, nmax = max(1, 8 * (rb - 1)), min(8 * rb - 1, _lib.EXACT_MAX_N)
+        # SPD solve: n + 1 rows in (8(rb-1), 8rb]; n <= MFH_SMALL_MAX_N
+        nmin, nmax = max(1, 8 * (rb - 1)), min(8 * rb - 1, _lib.SMALL_MAX_N)
         batch = spd_batch(args.count, nmin, nmax, seed=rb)
         res = {}
         for waves in [w for w in (1, 2, 4) if w == 1 or (w == 2 and rb >= 2) or (w == 4 and rb >= 4)]:

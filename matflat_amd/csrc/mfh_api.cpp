@@ -382,6 +382,7 @@ int mfh_dposv_vbatched_noalloc(mfh_handle* h, int count, const int* n, double* c
     MFH_REQUIRE(count <= MFH_NOALLOC_MAX_COUNT, 2, "count must be <= MFH_NOALLOC_MAX_COUNT (the static scratch size)");
     if (count == 0) return MFH_OK;
     MFH_REQUIRE(h->scratch != nullptr, 1, "the handle has no scratch address");
+    MFH_REQUIRE(max_n >= 1 && max_n <= MFH_SMALL_MAX_N, 8, "max_n must be in [1, MFH_SMALL_MAX_N]");
     // The scratch replaces the handle workspace for this call only; with
     // no_alloc set, nothing below allocates, creates a stream or an event:
     // the size buckets run in order unless the handle's side streams exist.
@@ -446,10 +447,60 @@ int mfh_dgetri_vbatched(mfh_handle* h, int count, const int* n, double* const* a
     MFH_REQUIRE(lda != nullptr, 5, "lda must not be null");
     MFH_REQUIRE(piv != nullptr, 6, "piv must not be null");
     MFH_REQUIRE(info != nullptr, 7, "info must not be null");
-    MFH_REQUIRE(max_n >= 1 && max_n <= MFH_EXACT_MAX_N, 8, "max_n must be in [1, MFH_EXACT_MAX_N]");
+    MFH_REQUIRE(max_n >= 1 && max_n <= MFH_SMALL_MAX_N, 8, "max_n must be in [1, MFH_SMALL_MAX_N]");
     MFH_REQUIRE((long long)count * max_n <= 0x7fffffffLL, 2, "count * max_n must be < 2^31");
     MfhGetriBatch B{n, a, lda, piv, info};
     return mfh_launch_getri(h, count, B, max_n);
+}
+
+int mfh_zgetrf_vbatched(mfh_handle* h, int count, const int* m, const int* n, double* const* a, const int* lda,
+                        int* const* piv, int* sign, int* info, int max_m, int max_n)
+{
+    MFH_REQUIRE(h != nullptr, 1, "handle must not be null");
+    MFH_REQUIRE(count >= 0, 2, "count must be >= 0");
+    if (count == 0) return MFH_OK;
+    MFH_REQUIRE(m != nullptr, 3, "m must not be null");
+    MFH_REQUIRE(n != nullptr, 4, "n must not be null");
+    MFH_REQUIRE(a != nullptr, 5, "a must not be null");
+    MFH_REQUIRE(lda != nullptr, 6, "lda must not be null");
+    MFH_REQUIRE(piv != nullptr, 7, "piv must not be null");
+    MFH_REQUIRE(sign != nullptr, 8, "sign must not be null");
+    MFH_REQUIRE(info != nullptr, 9, "info must not be null");
+    MFH_REQUIRE(max_m >= 1 && max_m <= MFH_EXACT_MAX_N, 10, "max_m must be in [1, MFH_EXACT_MAX_N]");
+    MFH_REQUIRE(max_n >= 1 && max_n <= MFH_EXACT_MAX_N, 11, "max_n must be in [1, MFH_EXACT_MAX_N]");
+    return mfh_launch_zgetrf(h, count, m, n, a, lda, piv, sign, info, max_m, max_n);
+}
+
+int mfh_zpotrf_vbatched(mfh_handle* h, int count, const int* n, double* const* a, const int* lda, int* info,
+                        int max_n)
+{
+    MFH_REQUIRE(h != nullptr, 1, "handle must not be null");
+    MFH_REQUIRE(count >= 0, 2, "count must be >= 0");
+    if (count == 0) return MFH_OK;
+    MFH_REQUIRE(n != nullptr, 3, "n must not be null");
+    MFH_REQUIRE(a != nullptr, 4, "a must not be null");
+    MFH_REQUIRE(lda != nullptr, 5, "lda must not be null");
+    MFH_REQUIRE(info != nullptr, 6, "info must not be null");
+    MFH_REQUIRE(max_n >= 1 && max_n <= MFH_EXACT_MAX_N, 7, "max_n must be in [1, MFH_EXACT_MAX_N]");
+    return mfh_launch_zpotrf(h, count, n, a, lda, info, max_n);
+}
+
+int mfh_ztrsv_vbatched(mfh_handle* h, int uplo, int trans, int count, const int* n, const double* const* a,
+                       const int* lda, double* const* x, const int* incx, int* info, int max_n)
+{
+    MFH_REQUIRE(h != nullptr, 1, "handle must not be null");
+    MFH_REQUIRE(uplo == MFH_UPPER || uplo == MFH_LOWER, 2, "Invalid enum value.");
+    MFH_REQUIRE(trans >= MFH_NOTRANS && trans <= MFH_CONJNOTRANS, 3, "Invalid enum value.");
+    MFH_REQUIRE(count >= 0, 4, "count must be >= 0");
+    if (count == 0) return MFH_OK;
+    MFH_REQUIRE(n != nullptr, 5, "n must not be null");
+    MFH_REQUIRE(a != nullptr, 6, "a must not be null");
+    MFH_REQUIRE(lda != nullptr, 7, "lda must not be null");
+    MFH_REQUIRE(x != nullptr, 8, "x must not be null");
+    MFH_REQUIRE(incx != nullptr, 9, "incx must not be null");
+    MFH_REQUIRE(info != nullptr, 10, "info must not be null");
+    MFH_REQUIRE(max_n >= 1 && max_n <= MFH_EXACT_MAX_N, 11, "max_n must be in [1, MFH_EXACT_MAX_N]");
+    return mfh_launch_ztrsv(h, uplo, trans, count, n, a, lda, x, incx, info, max_n);
 }
 
 static int gemm_common(mfh_handle* h, int count, const int* transa, const int* transb, const int* m, const int* n,
@@ -591,7 +642,7 @@ int mfh_dposv_backward_vbatched(mfh_handle* h, int count, const int* n, const do
     MFH_REQUIRE(ldabar != nullptr, 9, "ldabar must not be null");
     MFH_REQUIRE(bbar != nullptr, 10, "bbar must not be null");
     MFH_REQUIRE(info != nullptr, 11, "info must not be null");
-    MFH_REQUIRE(max_n >= 1 && max_n <= MFH_EXACT_MAX_N, 12, "max_n must be in [1, MFH_EXACT_MAX_N]");
+    MFH_REQUIRE(max_n >= 1 && max_n <= MFH_SMALL_MAX_N, 12, "max_n must be in [1, MFH_SMALL_MAX_N]");
     MfhAdjointBatch B{n, l, lda, x, xbar, abar, ldabar, bbar, info};
     return mfh_launch_posv_backward(h, count, B, max_n);
 }

@@ -230,7 +230,8 @@ struct GetrfKey {
             B.info[i] = bad;
             return -1;
         }
-        return m > n ? m : n;
+        // larger orders run k_getrf_gen (mfh_general.hip)
+        return (m > n ? m : n) > MFH_LDS_MAX_N ? -1 : (m > n ? m : n);
     }
 };
 
@@ -358,6 +359,7 @@ struct CholKey {
             B.info[i] = bad;
             return -1;
         }
+        if (n > MFH_LDS_MAX_N) return -1;  // k_potrf_gen (mfh_general.hip)
         return posv ? n + 1 : n;
     }
 };
@@ -657,7 +659,7 @@ struct TrsvKey {
             B.info[i] = bad;
             return -1;
         }
-        return n;
+        return n > MFH_LDS_MAX_N ? -1 : n;  // larger: k_trsv_gen (mfh_general.hip)
     }
 };
 
@@ -777,7 +779,7 @@ struct GetrsKey {
             B.info[i] = bad;
             return -1;
         }
-        return n;
+        return n > MFH_LDS_MAX_N ? -1 : n;  // larger: mfh_general.hip
     }
 };
 
@@ -895,8 +897,8 @@ int mfh_launch_getrf(mfh_handle* h, int count, const MfhGetrfBatch& B, int max_m
     SortWs ws;
     int st = device_sort(h, GetrfKey{B, max_m, max_n}, count, &ws);
     if (st) return st;
-    int key = max_m > max_n ? max_m : max_n;
-    if (key > MFH_EXACT_MAX_N) key = MFH_EXACT_MAX_N;
+    const int top = max_m > max_n ? max_m : max_n;
+    const int key = top > MFH_LDS_MAX_N ? MFH_LDS_MAX_N : top;
     for (int b = 0; b < nbuckets_for(key); ++b) {
         const int nb = bucket_bound(b, key);
         const int t = threads_for(nb);
@@ -906,6 +908,7 @@ int mfh_launch_getrf(mfh_handle* h, int count, const MfhGetrfBatch& B, int max_m
         else st = launch_bucket(h, MFH_KERNEL_GETRF_EXACT, nb, k_getrf_exact<128>, 128, lds, count, B, q);
         if (st) return st;
     }
+    if (top > MFH_LDS_MAX_N) return mfh_launch_getrf_gen_d(h, count, B, max_m, max_n);
     return MFH_OK;
 }
 
@@ -953,14 +956,18 @@ int mfh_launch_potrf(mfh_handle* h, int count, const MfhCholBatch& B, int max_n,
     SortWs ws;
     int st = device_sort(h, CholKey{B, max_n, posv ? 1 : 0}, count, &ws);
     if (st) return st;
-    const int n_cap = max_n > MFH_EXACT_MAX_N ? MFH_EXACT_MAX_N : max_n;
+    const int n_cap = max_n > MFH_LDS_MAX_N ? MFH_LDS_MAX_N : max_n;
     const int key = posv ? n_cap + 1 : n_cap;
     if (h->math_mode == MFH_MATH_FAST) {
         // fine bucket f holds row counts (8f, 8f + 8]: register tile of f + 1 row blocks
         const int nf = (key + MFH_FINE_STEP - 1) / MFH_FINE_STEP;
-        return fork_buckets(h, nf, [&](int f) {
+        st = fork_buckets(h, nf, [&](int f) {
             return mfh_launch_chol_fast_bucket(h, B, fine_queue(ws, f), f + 1, count, posv);
         });
+        if (st) return st;
+        // orders above the register tiles run the MatFlat-order kernels
+        if (max_n > MFH_LDS_MAX_N) return mfh_launch_potrf_gen_d(h, count, B, max_n, posv);
+        return MFH_OK;
     }
     // The exact buckets (16 rows wide) stay in order: one bucket usually
     // dominates, and measured on MI355X the fork only added its waits
@@ -971,6 +978,7 @@ int mfh_launch_potrf(mfh_handle* h, int count, const MfhCholBatch& B, int max_n,
                   : run_chol_bucket<false>(h, b, nb_rows, count, B, ws);
         if (st) return st;
     }
+    if (max_n > MFH_LDS_MAX_N) return mfh_launch_potrf_gen_d(h, count, B, max_n, posv);
     return MFH_OK;
 }
 
@@ -979,7 +987,7 @@ int mfh_launch_trsv(mfh_handle* h, int count, const MfhTrsvBatch& B, int max_n)
     SortWs ws;
     int st = device_sort(h, TrsvKey{B, max_n}, count, &ws);
     if (st) return st;
-    const int key = max_n > MFH_EXACT_MAX_N ? MFH_EXACT_MAX_N : max_n;
+    const int key = max_n > MFH_LDS_MAX_N ? MFH_LDS_MAX_N : max_n;
     for (int b = 0; b < nbuckets_for(key); ++b) {
         const int nb = bucket_bound(b, key);
         const int t = threads_for(nb);
@@ -989,6 +997,7 @@ int mfh_launch_trsv(mfh_handle* h, int count, const MfhTrsvBatch& B, int max_n)
         else st = launch_bucket(h, MFH_KERNEL_TRSV_EXACT, nb, k_trsv_exact<128>, 128, lds, count, B, q);
         if (st) return st;
     }
+    if (max_n > MFH_LDS_MAX_N) return mfh_launch_trsv_gen_d(h, count, B, max_n);
     return MFH_OK;
 }
 
@@ -997,7 +1006,7 @@ int mfh_launch_getrs(mfh_handle* h, int count, const MfhGetrsBatch& B, int max_n
     SortWs ws;
     int st = device_sort(h, GetrsKey{B, max_n}, count, &ws);
     if (st) return st;
-    const int key = max_n > MFH_EXACT_MAX_N ? MFH_EXACT_MAX_N : max_n;
+    const int key = max_n > MFH_LDS_MAX_N ? MFH_LDS_MAX_N : max_n;
     for (int b = 0; b < nbuckets_for(key); ++b) {
         const int nb = bucket_bound(b, key);
         const int t = threads_for(nb);
@@ -1007,6 +1016,7 @@ int mfh_launch_getrs(mfh_handle* h, int count, const MfhGetrsBatch& B, int max_n
         else st = launch_bucket(h, MFH_KERNEL_GETRS_EXACT, nb, k_getrs_exact<128>, 128, lds, count, B, q);
         if (st) return st;
     }
+    if (max_n > MFH_LDS_MAX_N) return mfh_launch_getrs_gen_d(h, count, B, max_n);
     return MFH_OK;
 }
 

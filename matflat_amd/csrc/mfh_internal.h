@@ -18,6 +18,10 @@
 // with 32-bit byte offsets from its base pointer.
 #define MFH_MAX_MATRIX_ELEMS (1LL << 28)
 
+// Largest order of the LDS-resident exact kernels (mfh_exact.hip); larger
+// orders, up to MFH_EXACT_MAX_N, run the kernels of mfh_general.hip.
+#define MFH_LDS_MAX_N 128
+
 // The FAST kernels use finer buckets: (8f, 8f + 8] rows.
 #define MFH_FINE_STEP 8
 
@@ -203,5 +207,17 @@ int mfh_launch_gesv_mixed(mfh_handle* h, int count, const int* n, const double* 
                           const int* lda, const double* const* b, double* const* x, int max_iter,
                           double tol_per_n, int* iters, double* berr, int* info, int max_n,
                           bool mixed);
+// mfh_general.hip: double matrices above MFH_LDS_MAX_N rows (the size sort
+// has reported invalid ones), and the complex128 entry points.
+int mfh_launch_getrf_gen_d(mfh_handle* h, int count, const MfhGetrfBatch& B, int max_m, int max_n);
+int mfh_launch_potrf_gen_d(mfh_handle* h, int count, const MfhCholBatch& B, int max_n, bool posv);
+int mfh_launch_trsv_gen_d(mfh_handle* h, int count, const MfhTrsvBatch& B, int max_n);
+int mfh_launch_getrs_gen_d(mfh_handle* h, int count, const MfhGetrsBatch& B, int max_n);
+int mfh_launch_zgetrf(mfh_handle* h, int count, const int* m, const int* n, double* const* a, const int* lda,
+                      int* const* piv, int* sign, int* info, int max_m, int max_n);
+int mfh_launch_zpotrf(mfh_handle* h, int count, const int* n, double* const* a, const int* lda, int* info,
+                      int max_n);
+int mfh_launch_ztrsv(mfh_handle* h, int uplo, int trans, int count, const int* n, const double* const* a,
+                     const int* lda, double* const* x, const int* incx, int* info, int max_n);
 int mfh_launch_getrf_blocked(mfh_handle* h, int count, const int* n, double* const* a, const int* lda,
                              int* const* piv, int* info, int max_n);

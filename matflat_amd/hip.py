@@ -18,8 +18,8 @@ from dataclasses import dataclass
 import torch
 
 from . import _lib
-from ._lib import (BLOCKED_MAX_N, EXACT_MAX_N, LOWER, MATH_EXACT, MATH_FAST, NONUNIT, NOTRANS,  # noqa: F401
-                   TRANS, UNIT, UPPER)
+from ._lib import (BLOCKED_MAX_N, CONJNOTRANS, CONJTRANS, EXACT_MAX_N, LOWER, MATH_EXACT, MATH_FAST,  # noqa: F401
+                   NONUNIT, NOTRANS, SMALL_MAX_N, TRANS, UNIT, UPPER)
 
 
 class MatFlatException(Exception):
@@ -220,6 +220,26 @@ def dgemv_vbatched(trans, m, n, a_ptrs, lda, x_ptrs, incx, y_ptrs, incy, info, m
     h = handle or default_handle()
     _check(h.lib.mfh_dgemv_vbatched(h.h, int(m.numel()), _p(trans), _p(m), _p(n), _p(a_ptrs), _p(lda), _p(x_ptrs),
                                     _p(incx), _p(y_ptrs), _p(incy), _p(info), int(max_m), int(max_n)))
+
+
+def zgetrf_vbatched(m, n, a_ptrs, lda, piv_ptrs, sign, info, max_m, max_n, handle: Handle | None = None):
+    """Complex128 LU (mfh_zgetrf_vbatched); a_ptrs address interleaved (re, im) matrices."""
+    h = handle or default_handle()
+    _check(h.lib.mfh_zgetrf_vbatched(h.h, int(m.numel()), _p(m), _p(n), _p(a_ptrs), _p(lda), _p(piv_ptrs),
+                                     _p(sign), _p(info), int(max_m), int(max_n)))
+
+
+def zpotrf_vbatched(n, a_ptrs, lda, info, max_n, handle: Handle | None = None):
+    """Complex128 Cholesky A = L L^H (mfh_zpotrf_vbatched)."""
+    h = handle or default_handle()
+    _check(h.lib.mfh_zpotrf_vbatched(h.h, int(n.numel()), _p(n), _p(a_ptrs), _p(lda), _p(info), int(max_n)))
+
+
+def ztrsv_vbatched(uplo, trans, n, a_ptrs, lda, x_ptrs, incx, info, max_n, handle: Handle | None = None):
+    """Complex128 triangular solve, every transpose (mfh_ztrsv_vbatched)."""
+    h = handle or default_handle()
+    _check(h.lib.mfh_ztrsv_vbatched(h.h, int(uplo), int(trans), int(n.numel()), _p(n), _p(a_ptrs), _p(lda),
+                                    _p(x_ptrs), _p(incx), _p(info), int(max_n)))
 
 
 def dgesv_mixed_vbatched(n, a_ptrs, lda, b_ptrs, x_ptrs, max_iter, tol_per_n, iters, berr, info, max_n,

@@ -14,6 +14,11 @@ CPU fallback.
 * ``Blas.MulMatMat``          MatFlat/Blas.MulMatMat.cs:226-368 (double),
                               :407-648 (Complex)
 * LU solve composition        MatFlat/Factorization.LuInverse.cs:169-192
+* Complex LU, Cholesky, triangular solve
+                              MatFlat/Factorization.Lu.cs:352-406,
+                              Factorization.Cholesky.cs:195-249,
+                              Blas.SolveTriangular.cs:280-445, with BCL
+                              Complex division (Smith's method)
 
 compiled with ``-O2 -ffp-contract=off -fno-fast-math``.  Pinning (what makes
 these functions trustworthy, beyond being written from the reference) lives in
@@ -90,6 +95,16 @@ def lib() -> ctypes.CDLL:
         _lib.mfo_gemm_vbatch.argtypes = [i, i] + [p] * 17 + [i]
         _lib.mfo_dposv_backward_batch.restype = None
         _lib.mfo_dposv_backward_batch.argtypes = [i, i, i, i64, p, p, p, p, p, i]
+        _lib.mfo_zdiv.restype = None
+        _lib.mfo_zdiv.argtypes = [d, d, d, d, p, p]
+        _lib.mfo_zdot.restype = None
+        _lib.mfo_zdot.argtypes = [i, i, p, i, p, i, p]
+        _lib.mfo_zgetrf.restype = i
+        _lib.mfo_zgetrf.argtypes = [i, i, p, i, p]
+        _lib.mfo_zpotrf.restype = i
+        _lib.mfo_zpotrf.argtypes = [i, p, i]
+        _lib.mfo_ztrsv.restype = i
+        _lib.mfo_ztrsv.argtypes = [i, i, i, p, i, p, i]
         _lib.mfo_dposv_backward.restype = None
         _lib.mfo_dposv_backward.argtypes = [i, p, i, p, p, p, i, p]
     return _lib
@@ -160,6 +175,42 @@ def zgemm(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc) -> int:
     assert a.dtype == np.complex128 and b.dtype == np.complex128 and c.dtype == np.complex128
     return lib().mfo_zgemm(transa, transb, m, n, k, _ptr(al), _ptr(a), lda, _ptr(b), ldb,
                            _ptr(be), _ptr(c), ldc)
+
+
+def zdiv(x: complex, y: complex) -> complex:
+    """BCL Complex division x / y (Smith's method)."""
+    re, im = ctypes.c_double(), ctypes.c_double()
+    lib().mfo_zdiv(float(x.real), float(x.imag), float(y.real), float(y.imag), ctypes.byref(re),
+                   ctypes.byref(im))
+    return complex(re.value, im.value)
+
+
+def zdot(conj: int, n: int, x, incx: int, y, incy: int) -> complex:
+    x = np.ascontiguousarray(x, dtype=np.complex128)
+    y = np.ascontiguousarray(y, dtype=np.complex128)
+    out = np.zeros(1, dtype=np.complex128)
+    lib().mfo_zdot(int(conj), n, _ptr(x), incx, _ptr(y), incy, _ptr(out))
+    return complex(out[0])
+
+
+def zgetrf(m: int, n: int, a: np.ndarray, lda: int):
+    """In-place complex LU of the (lda*n,) complex128 buffer ``a``.
+    Returns (piv int32[m], sign)."""
+    assert a.dtype == np.complex128 and a.size >= lda * (n - 1) + m
+    piv = np.zeros(m, dtype=np.int32)
+    sign = lib().mfo_zgetrf(m, n, _ptr(a), lda, _ptr(piv))
+    return piv, sign
+
+
+def zpotrf(n: int, a: np.ndarray, lda: int) -> int:
+    """In-place complex Cholesky (A = L L^H); returns info (0 or j+1)."""
+    assert a.dtype == np.complex128
+    return lib().mfo_zpotrf(n, _ptr(a), lda)
+
+
+def ztrsv(uplo: int, trans: int, n: int, a: np.ndarray, lda: int, x: np.ndarray, incx: int = 1) -> int:
+    assert a.dtype == np.complex128 and x.dtype == np.complex128
+    return lib().mfo_ztrsv(uplo, trans, n, _ptr(a), lda, _ptr(x), incx)
 
 
 def dposv_batch(n, lda, off, a, xoff, x, threads=1) -> np.ndarray:

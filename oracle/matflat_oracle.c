@@ -450,3 +450,214 @@ void mfo_dposv_backward(int n, const double* l, int lda, const double* x, const 
         }
     }
 }
+
+/* ------------------------------------------------------------------ complex
+ * Complex128 factorizations and triangular solve.  Buffers hold interleaved
+ * (re, im) doubles; element (i, j) of a matrix is at 2*(i + j*lda).
+ *
+ * System.Numerics.Complex operator / (BCL): Smith's method (R. L. Smith,
+ * "Algorithm 116: Complex division", CACM 5(8):435, 1962) with the branch
+ * taken on |d| < |c|:
+ *   |d| <  |c|: r = d/c, den = c + d*r, q = ((a + b*r)/den, (b - a*r)/den)
+ *   otherwise : r = c/d, den = d + c*r, q = ((b + a*r)/den, (-a + b*r)/den)
+ * for (a + bi) / (c + di). */
+void mfo_zdiv(double a, double b, double c, double d, double* re, double* im)
+{
+    if (fabs(d) < fabs(c)) {
+        double r = d / c;
+        double den = c + d * r;
+        *re = (a + b * r) / den;
+        *im = (b - a * r) / den;
+    } else {
+        double r = c / d;
+        double den = d + c * r;
+        *re = (b + a * r) / den;
+        *im = (-a + b * r) / den;
+    }
+}
+
+/* Internals.Dot(int n, Complex* x, int incx, Complex* y, int incy),
+ * Internals.cs:294-321 (conj = 0), and Internals.DotConj(int n, Complex* x,
+ * int incx, Complex* y, int incy), Internals.cs:352-380 (conj = 1, products
+ * MulConj(x, y) = conj(x)*y).  Odd n: sum = p0 first; even n: sum = 0.  Then
+ * pairs: sum = sum + (p_k + p_{k+1}), component-wise. */
+static void zdot_pairs(int conj, int n, const double* x, int incx, const double* y, int incy,
+                       double* sre, double* sim)
+{
+    double sr, si;
+    if (n & 1) {
+        if (conj) cmulconj(x, y, &sr, &si);
+        else cmul(x[0], x[1], y[0], y[1], &sr, &si);
+        x += 2 * (size_t)incx;
+        y += 2 * (size_t)incy;
+        n--;
+    } else {
+        sr = 0.0;
+        si = 0.0;
+    }
+    while (n > 0) {
+        double r0, i0, r1, i1;
+        const double* x1 = x + 2 * (size_t)incx;
+        const double* y1 = y + 2 * (size_t)incy;
+        if (conj) {
+            cmulconj(x, y, &r0, &i0);
+            cmulconj(x1, y1, &r1, &i1);
+        } else {
+            cmul(x[0], x[1], y[0], y[1], &r0, &i0);
+            cmul(x1[0], x1[1], y1[0], y1[1], &r1, &i1);
+        }
+        sr = sr + (r0 + r1);
+        si = si + (i0 + i1);
+        x += 4 * (size_t)incx;
+        y += 4 * (size_t)incy;
+        n -= 2;
+    }
+    *sre = sr;
+    *sim = si;
+}
+
+void mfo_zdot(int conj, int n, const double* x, int incx, const double* y, int incy, double* out)
+{
+    zdot_pairs(conj, n, x, incx, y, incy, out, out + 1);
+}
+
+/* Factorization.Lu.cs:352-406 (LuCore, Complex).  As mfo_dgetrf, with the
+ * pivot chosen by Internals.FastMagnitude = |re| + |im| (Internals.cs:15-19)
+ * under the same strict '>' scan, the zero test diag != Complex.Zero, and
+ * BCL division for the multipliers (Internals.DivInplace, Internals.cs:125-147). */
+int mfo_zgetrf(int m, int n, double* a, int lda, int* piv)
+{
+    double* work = (double*)malloc(sizeof(double) * 2 * (size_t)m);
+    for (int i = 0; i < m; i++) piv[i] = i;
+    int sign = 1;
+    for (int j = 0; j < n; j++) {
+        double* colj = a + 2 * (size_t)j * lda;
+        memcpy(work, colj, sizeof(double) * 2 * (size_t)m);
+        for (int i = 0; i < m; i++) {
+            double sr, si;
+            zdot_pairs(0, imin(i, j), a + 2 * (size_t)i, lda, work, 1, &sr, &si);
+            work[2 * i] = work[2 * i] - sr;
+            work[2 * i + 1] = work[2 * i + 1] - si;
+            colj[2 * i] = work[2 * i];
+            colj[2 * i + 1] = work[2 * i + 1];
+        }
+        int p = j;
+        for (int i = j + 1; i < m; i++) {
+            double mi = fabs(work[2 * i]) + fabs(work[2 * i + 1]);
+            double mp = fabs(work[2 * p]) + fabs(work[2 * p + 1]);
+            if (mi > mp) p = i;
+        }
+        if (p != j) {
+            for (int c = 0; c < n; c++) {
+                double* x = a + 2 * (p + (size_t)c * lda);
+                double* y = a + 2 * (j + (size_t)c * lda);
+                double t0 = x[0], t1 = x[1];
+                x[0] = y[0];
+                x[1] = y[1];
+                y[0] = t0;
+                y[1] = t1;
+            }
+            int t = piv[p];
+            piv[p] = piv[j];
+            piv[j] = t;
+            sign = -sign;
+        }
+        if (j < m) {
+            double dr = colj[2 * j], di = colj[2 * j + 1];
+            if (dr != 0.0 || di != 0.0) {
+                for (int i = j + 1; i < m; i++) {
+                    double qr, qi;
+                    mfo_zdiv(colj[2 * i], colj[2 * i + 1], dr, di, &qr, &qi);
+                    colj[2 * i] = qr;
+                    colj[2 * i + 1] = qi;
+                }
+            }
+        }
+    }
+    free(work);
+    return sign;
+}
+
+/* Factorization.Cholesky.cs:195-249 (Complex, A = L L^H).  For row j and
+ * k < j: t = (A[k,j] - DotConj(k, L[j,0:], L[k,0:])) / L[k,k] by BCL
+ * division, L[j,k] = conj(t), s = s + (t.re*t.re + t.im*t.im); then
+ * s = re(A[j,j]) - s and L[j,j] = (sqrt(s), 0) if s > 0, else failure after
+ * the partial writes (returns j + 1).  The strict upper triangle is zeroed
+ * at the end. */
+int mfo_zpotrf(int n, double* a, int lda)
+{
+    for (int j = 0; j < n; j++) {
+        double* colj = a + 2 * (size_t)j * lda;
+        double s = 0.0;
+        for (int k = 0; k < j; k++) {
+            double* colk = a + 2 * (size_t)k * lda;
+            double dr, di, tr, ti;
+            zdot_pairs(1, k, a + 2 * (size_t)j, lda, a + 2 * (size_t)k, lda, &dr, &di);
+            mfo_zdiv(colj[2 * k] - dr, colj[2 * k + 1] - di, colk[2 * k], colk[2 * k + 1], &tr, &ti);
+            colk[2 * j] = tr;
+            colk[2 * j + 1] = -ti;
+            s = s + (tr * tr + ti * ti);
+        }
+        s = colj[2 * j] - s;
+        if (s > 0) {
+            colj[2 * j] = sqrt(s);
+            colj[2 * j + 1] = 0.0;
+        } else {
+            return j + 1;
+        }
+    }
+    for (int j = 1; j < n; j++) {
+        double* colj = a + 2 * (size_t)j * lda;
+        for (int i = 0; i < 2 * j; i++) colj[i] = 0.0;
+    }
+    return 0;
+}
+
+/* Blas.SolveTriangular.cs:280-395 (Complex) with MulSub (:397-421,
+ * dst -= x*y) and MulSubConj (:423-445, dst -= MulConj(x, y)).  All four
+ * transposes; the ConjNoTrans / ConjTrans forms use conj(A).  Division is
+ * BCL division (mfo_zdiv). */
+int mfo_ztrsv(int uplo, int trans, int n, const double* a, int lda, double* x, int incx)
+{
+    const int conj = (trans == MFO_CONJNOTRANS || trans == MFO_CONJTRANS);
+    const int t = conj ? remove_conj(trans) : trans;
+    if (trans < MFO_NOTRANS || trans > MFO_CONJNOTRANS) return -2;
+    if (uplo != MFO_UPPER && uplo != MFO_LOWER) return -1;
+    if ((uplo == MFO_UPPER) == (t == MFO_NOTRANS)) {
+        /* axpy forms: Upper/NoTrans (:317-326, :337-346 conj) walks i down and
+         * subtracts column i; Lower/Trans (:362-371, :381-390 conj) walks j down
+         * and subtracts row j. */
+        for (int i = n - 1; i >= 0; i--) {
+            double* xp = x + 2 * (size_t)incx * i;
+            const double* d = a + 2 * ((size_t)lda * i + i);
+            double qr, qi;
+            mfo_zdiv(xp[0], xp[1], d[0], conj ? -d[1] : d[1], &qr, &qi);
+            xp[0] = qr;
+            xp[1] = qi;
+            for (int r = 0; r < i; r++) {
+                const double* e = (uplo == MFO_UPPER) ? a + 2 * ((size_t)lda * i + r) : a + 2 * (i + (size_t)lda * r);
+                double pr, pi;
+                if (conj) cmulconj(e, xp, &pr, &pi);
+                else cmul(e[0], e[1], xp[0], xp[1], &pr, &pi);
+                double* xr = x + 2 * (size_t)incx * r;
+                xr[0] = xr[0] - pr;
+                xr[1] = xr[1] - pi;
+            }
+        }
+    } else {
+        /* dot forms: Upper/Trans (:327-334, :347-354 conj) dots column j of A
+         * with x[0:j]; Lower/NoTrans (:355-361, :372-379 conj) dots row i. */
+        for (int i = 0; i < n; i++) {
+            double* xp = x + 2 * (size_t)incx * i;
+            double sr, si;
+            if (uplo == MFO_UPPER) zdot_pairs(conj, i, a + 2 * (size_t)lda * i, 1, x, incx, &sr, &si);
+            else zdot_pairs(conj, i, a + 2 * (size_t)i, lda, x, incx, &sr, &si);
+            const double* d = a + 2 * ((size_t)lda * i + i);
+            double qr, qi;
+            mfo_zdiv(xp[0] - sr, xp[1] - si, d[0], conj ? -d[1] : d[1], &qr, &qi);
+            xp[0] = qr;
+            xp[1] = qi;
+        }
+    }
+    return 0;
+}

@@ -76,6 +76,29 @@ int mfo_zgemm(int transa, int transb, int m, int n, int k, const double* alpha,
               const double* a, int lda, const double* b, int ldb,
               const double* beta, double* c, int ldc);
 
+/* ---- complex128 (interleaved re, im doubles; element (i, j) at 2(i + j*lda)) */
+
+/* BCL Complex division (Smith's method, branch on |d| < |c|):
+ * (a + bi) / (c + di). */
+void mfo_zdiv(double a, double b, double c, double d, double* re, double* im);
+
+/* Internals.Dot / Internals.DotConj over Complex, Internals.cs:294-321 and
+ * :352-380 (conj = 1: products conj(x)*y); out = (re, im). */
+void mfo_zdot(int conj, int n, const double* x, int incx, const double* y, int incy, double* out);
+
+/* Factorization.Lu Complex, Factorization.Lu.cs:199-238 / :352-406: pivot by
+ * |re| + |im|; returns the pivot sign, piv as mfo_dgetrf. */
+int mfo_zgetrf(int m, int n, double* a, int lda, int* piv);
+
+/* Factorization.Cholesky Complex, Factorization.Cholesky.cs:195-249: A = L L^H
+ * from the upper triangle; returns 0 or j + 1 of the failing row (partial
+ * writes as the reference leaves them). */
+int mfo_zpotrf(int n, double* a, int lda);
+
+/* Blas.SolveTriangular Complex, Blas.SolveTriangular.cs:280-395: every
+ * uplo / transpose (NoTrans, Trans, ConjTrans, ConjNoTrans). */
+int mfo_ztrsv(int uplo, int trans, int n, const double* a, int lda, double* x, int incx);
+
 /* ---- batched drivers (CPU baseline; OpenMP over matrices) ---------------- */
 
 /* Cholesky + two solves over a batch stored back-to-back: matrix i at

@@ -5,7 +5,9 @@ MatFlat's own tests hold no golden vectors for this path (they compare with
 OpenBLAS at 1e-12), so these fixtures are the build's own: seeded inputs and
 the oracle's bit-exact outputs for the shapes the reference's tests use
 (LuTests.cs:103-122, CholeskyTests.cs, BlasTests.SolveTriangular*.cs,
-BlasTests.MulMatMat*.cs, BlasTests.MulMatVecDouble.cs, LuInverseTests.cs), stored with allow_pickle=False arrays only.  The
+BlasTests.MulMatMat*.cs, BlasTests.MulMatVecDouble.cs, LuInverseTests.cs; the
+complex LU, Cholesky and triangular solve on the same shapes), stored with
+allow_pickle=False arrays only.  The
 GPU tests compare the HIP path against them without running the oracle.
 
 Run from the repo root:  python tests/golden/make_golden.py
@@ -212,6 +214,73 @@ def getri_fixture(rng):
             "getri_piv": piv.astype(np.int32), "getri_piv_off": poff, "getri_inv": inv}
 
 
+def complex_fixture(rng):
+    """Complex LU on the LuTests shapes, complex Cholesky (NaN lower
+    triangle, one failing matrix) and the eight complex triangular-solve
+    forms (Factorization.Lu.cs:352-406, Factorization.Cholesky.cs:195-249,
+    Blas.SolveTriangular.cs:280-395)."""
+    def crand(k):
+        return rng.standard_normal(k) + 1j * rng.standard_normal(k)
+
+    out = {}
+    meta, ins, outs, pivs = [], [], [], []
+    for m, n, lda in LU_SHAPES:
+        a = np.full(lda * n, complex(np.nan, np.nan))
+        for j in range(n):
+            a[j * lda:j * lda + m] = crand(m)
+        lu = a.copy()
+        piv, sign = oracle.zgetrf(m, n, lu, lda)
+        meta.append((m, n, lda, sign))
+        ins.append(a)
+        outs.append(lu)
+        pivs.append(piv.astype(np.int32))
+    a_in, off = pack(ins)
+    a_out, _ = pack(outs)
+    piv, poff = pack(pivs)
+    out.update({"zlu_meta": np.array(meta, dtype=np.int64), "zlu_off": off, "zlu_in": a_in, "zlu_out": a_out,
+                "zlu_piv": piv.astype(np.int32), "zlu_piv_off": poff})
+    meta, ins, outs = [], [], []
+    for n in CHOL_N + [9]:
+        lda = (n + 7) // 8 * 8
+        bm = crand(n * n).reshape(n, n)
+        hpd = bm @ bm.conj().T + n * np.eye(n)
+        if n == 9:
+            hpd[5, 5] = -1.0  # fails at row 5, after the partial writes of rows 0..5
+        a = np.full(lda * n, complex(np.nan, np.nan))
+        for j in range(n):
+            a[j * lda:j * lda + j + 1] = hpd[:j + 1, j]
+        l = a.copy()
+        info = oracle.zpotrf(n, l, lda)
+        meta.append((n, lda, info))
+        ins.append(a)
+        outs.append(l)
+    a_in, off = pack(ins)
+    a_out, _ = pack(outs)
+    out.update({"zchol_meta": np.array(meta, dtype=np.int64), "zchol_off": off, "zchol_in": a_in,
+                "zchol_out": a_out})
+    meta, ins, xin, xout = [], [], [], []
+    for uplo in (121, 122):
+        for trans in (111, 112, 113, 114):
+            for n in TRSV_N:
+                lda, inc = n + 1, 2
+                a = crand(lda * n)
+                for j in range(n):
+                    a[j * lda + j] = (2.0 + np.sqrt(n)) * np.exp(1j * rng.uniform(0, 2 * np.pi))
+                x = crand(n * inc)
+                want = x.copy()
+                assert oracle.ztrsv(uplo, trans, n, a, lda, want, inc) == 0
+                meta.append((uplo, trans, n, lda, inc))
+                ins.append(a)
+                xin.append(x)
+                xout.append(want)
+    a, off = pack(ins)
+    x0, xoff = pack(xin)
+    x1, _ = pack(xout)
+    out.update({"ztrsv_meta": np.array(meta, dtype=np.int64), "ztrsv_off": off, "ztrsv_a": a, "ztrsv_x_off": xoff,
+                "ztrsv_x_in": x0, "ztrsv_x_out": x1})
+    return out
+
+
 def main():
     oracle.build()
     rng = np.random.default_rng(20240610)
@@ -221,6 +290,7 @@ def main():
     np.savez_compressed(os.path.join(HERE, "gemm.npz"), **gemm_fixture(rng, False), **gemm_fixture(rng, True))
     np.savez_compressed(os.path.join(HERE, "gemv.npz"), **gemv_fixture(rng))
     np.savez_compressed(os.path.join(HERE, "getri.npz"), **getri_fixture(rng))
+    np.savez_compressed(os.path.join(HERE, "complex.npz"), **complex_fixture(np.random.default_rng(20261204)))
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)), "bytes")

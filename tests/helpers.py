@@ -87,3 +87,16 @@ def backward_error(a: np.ndarray, x: np.ndarray, b: np.ndarray) -> float:
     """Normwise backward error ||b - A x||_inf / (||A||_inf ||x||_inf + ||b||_inf)."""
     r = b - a @ x
     return np.abs(r).max() / (np.abs(a).sum(1).max() * np.abs(x).max() + np.abs(b).max())
+
+
+def hpd_upper_cbuf(seed: int, n: int, lda: int, shift: float | None = None, nan_lower: bool = True) -> np.ndarray:
+    """Hermitian positive definite A = B B^H + shift*I with B complex N(0,1);
+    upper triangle (diagonal included) stored, strict lower NaN."""
+    rng = np.random.default_rng(seed)
+    b = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    a = b @ b.conj().T + (n if shift is None else shift) * np.eye(n)
+    fill = complex(np.nan, np.nan) if nan_lower else 0.0
+    buf = np.full(lda * n, fill, dtype=np.complex128)
+    for j in range(n):
+        buf[j * lda:j * lda + j + 1] = a[: j + 1, j]
+    return buf

@@ -89,3 +89,27 @@ def test_getri_fixtures():
         a = g["getri_lu"][o:o + lda * n].copy()
         assert oracle.dgetri(int(n), a, int(lda), g["getri_piv"][po:po + n]) == 0
         assert_bits_equal(a, g["getri_inv"][o:o + lda * n], f"getri {i}")
+
+
+def test_complex_fixtures():
+    g = load("complex.npz")
+    for i, (m, n, lda, sign) in enumerate(g["zlu_meta"]):
+        off, poff = g["zlu_off"][i], g["zlu_piv_off"][i]
+        a = g["zlu_in"][off:off + lda * n].copy()
+        piv, s = oracle.zgetrf(int(m), int(n), a, int(lda))
+        assert_bits_equal(a, g["zlu_out"][off:off + lda * n], f"zlu {i}")
+        assert np.array_equal(piv, g["zlu_piv"][poff:poff + m]) and s == sign
+    infos = []
+    for i, (n, lda, info) in enumerate(g["zchol_meta"]):
+        off = g["zchol_off"][i]
+        a = g["zchol_in"][off:off + lda * n].copy()
+        assert oracle.zpotrf(int(n), a, int(lda)) == info
+        assert_bits_equal(a, g["zchol_out"][off:off + lda * n], f"zchol {i}")
+        infos.append(int(info))
+    assert infos[-1] == 6 and all(v == 0 for v in infos[:-1])
+    for i, (uplo, trans, n, lda, inc) in enumerate(g["ztrsv_meta"]):
+        off, xoff = g["ztrsv_off"][i], g["ztrsv_x_off"][i]
+        a = g["ztrsv_a"][off:off + lda * n].copy()
+        x = g["ztrsv_x_in"][xoff:xoff + n * inc].copy()
+        oracle.ztrsv(int(uplo), int(trans), int(n), a, int(lda), x, int(inc))
+        assert_bits_equal(x, g["ztrsv_x_out"][xoff:xoff + n * inc], f"ztrsv {i}")
