@@ -332,15 +332,45 @@ int mfh_dposv_backward_vbatched(mfh_handle* h, int count, const int* n, const do
                                 int* info, int max_n);
 
 /* ---- plans: one batch captured into a hipGraph ------------------------------
- * op names the entry point; the plan records one call with the given
- * arguments (device arrays stay owned by the caller and must outlive the
- * plan) and replays it with mfh_plan_execute on the handle's stream. */
+ * A plan records one call of an entry point with the given arguments into a
+ * hipGraph (device arrays stay owned by the caller and must outlive the
+ * plan) and replays it with mfh_plan_execute on the handle's stream, with
+ * no host work: the size sort, every size-bucket launch (forked over the
+ * handle's side streams as graph edges) and the solve steps are nodes of the
+ * graph.  The plan owns the workspace the call needs, so later calls on the
+ * handle do not disturb it.  Replays read the per-matrix arrays afresh:
+ * sizes and data may change between replays within the captured maxima.
+ * The create functions take the arguments of the entry point they name
+ * (no counterpart in the reference, which has no batches) followed by the
+ * plan; mfh_plan_get_op returns the MFH_OP_* of a plan. */
 #define MFH_OP_DPOSV 1
 #define MFH_OP_DGETRF 2
 #define MFH_OP_DPOTRF 3
+#define MFH_OP_DTRSV 4
+#define MFH_OP_DGESV 5
+#define MFH_OP_DGESV_MIXED 6
+#define MFH_OP_DGEMM 7
+#define MFH_OP_ZGEMM 8
 int mfh_plan_create_dposv(mfh_handle* h, int count, const int* n, double* const* a,
                           const int* lda, double* const* x, int* info, int max_n,
                           mfh_plan** plan);
+int mfh_plan_create_dgetrf(mfh_handle* h, int count, const int* m, const int* n,
+                           double* const* a, const int* lda, int* const* piv, int* sign,
+                           int* info, int max_m, int max_n, mfh_plan** plan);
+int mfh_plan_create_dpotrf(mfh_handle* h, int count, const int* n, double* const* a,
+                           const int* lda, int* info, int max_n, mfh_plan** plan);
+int mfh_plan_create_dtrsv(mfh_handle* h, int uplo, int trans, int diag, int count,
+                          const int* n, const double* const* a, const int* lda,
+                          double* const* x, const int* incx, int* info, int max_n,
+                          mfh_plan** plan);
+int mfh_plan_create_dgesv(mfh_handle* h, int count, const int* n, const double* const* a,
+                          const int* lda, const double* const* b, double* const* x,
+                          double* berr, int* info, int max_n, mfh_plan** plan);
+int mfh_plan_create_dgesv_mixed(mfh_handle* h, int count, const int* n,
+                                const double* const* a, const int* lda,
+                                const double* const* b, double* const* x, int max_iter,
+                                double tol_per_n, int* iters, double* berr, int* info,
+                                int max_n, mfh_plan** plan);
 /* The batched GEMM of mfh_dgemm_vbatched / mfh_zgemm_vbatched (same
  * arguments) captured into one graph: tile-list build and the GEMM kernel,
  * in the handle's math mode at capture time.  Replays read the per-product
@@ -356,6 +386,7 @@ int mfh_plan_create_zgemm(mfh_handle* h, int count, const int* transa, const int
                           const double* const* a, const int* lda, const double* const* b,
                           const int* ldb, const double* beta, double* const* c, const int* ldc,
                           int* info, int max_m, int max_n, int max_k, mfh_plan** plan);
+int mfh_plan_get_op(mfh_plan* plan, int* op);
 int mfh_plan_execute(mfh_plan* plan);
 int mfh_plan_destroy(mfh_plan* plan);
 

@@ -25,6 +25,7 @@ EXACT_MAX_N = 1024
 SMALL_MAX_N = 128
 LDS_MAX_N = 128  # larger orders run the generic kernels (k_getrf_gen, k_potrf_gen, k_trsv_gen)
 MATH_EXACT, MATH_FAST = 0, 1
+OP_DPOSV, OP_DGETRF, OP_DPOTRF, OP_DTRSV, OP_DGESV, OP_DGESV_MIXED, OP_DGEMM, OP_ZGEMM = 1, 2, 3, 4, 5, 6, 7, 8
 NOALLOC_MAX_COUNT = 1 << 20
 BLOCKED_MAX_N = 512
 KERNEL_SORT, KERNEL_CHOL_EXACT, KERNEL_CHOL_FAST, KERNEL_GETRF_EXACT, KERNEL_TRSV_EXACT, KERNEL_GETRS_EXACT = \
@@ -75,6 +76,12 @@ SIGNATURES = {
     "mfh_plan_create_dposv": [_p, _i, _p, _p, _p, _p, _p, _i, _p],
     "mfh_plan_create_dgemm": [_p, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i, _i, _i, _p],
     "mfh_plan_create_zgemm": [_p, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i, _i, _i, _p],
+    "mfh_plan_create_dgetrf": [_p, _i, _p, _p, _p, _p, _p, _p, _p, _i, _i, _p],
+    "mfh_plan_create_dpotrf": [_p, _i, _p, _p, _p, _p, _i, _p],
+    "mfh_plan_create_dtrsv": [_p, _i, _i, _i, _i, _p, _p, _p, _p, _p, _p, _i, _p],
+    "mfh_plan_create_dgesv": [_p, _i, _p, _p, _p, _p, _p, _p, _p, _i, _p],
+    "mfh_plan_create_dgesv_mixed": [_p, _i, _p, _p, _p, _p, _p, _i, _d, _p, _p, _p, _i, _p],
+    "mfh_plan_get_op": [_p, _p],
     "mfh_plan_execute": [_p],
     "mfh_plan_destroy": [_p],
 }

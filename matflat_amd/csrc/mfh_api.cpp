@@ -652,35 +652,50 @@ int mfh_dposv_backward_vbatched(mfh_handle* h, int count, const int* n, const do
 
 struct mfh_plan {
     mfh_handle* h = nullptr;
+    int op = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    void* ws = nullptr;  // the plan's own workspace: later calls on the handle may regrow the handle's
+    void* ws = nullptr;   // the plan's own workspace and second buffer: later calls on the handle may regrow
+    void* aux = nullptr;  // the handle's, and the graph must keep pointing at live memory
 };
 
-// Records one call into a hipGraph.  The plan owns a workspace sized
-// exactly as the launcher asks for it; the capture runs on a private stream
-// (the handle's stream may be the null stream, which cannot be captured)
-// with the handle temporarily pointing at the plan's workspace.
+// Records one call into a hipGraph.  The plan owns a workspace and a second
+// buffer sized exactly as the launchers ask for them; the capture runs on a
+// private stream (the handle's stream may be the null stream, which cannot be
+// captured) with the handle temporarily pointing at the plan's buffers.
 template <typename F>
-static int capture_plan(mfh_handle* h, size_t ws_bytes, F call, mfh_plan** plan)
+static int capture_plan(mfh_handle* h, int op, size_t ws_bytes, size_t aux_bytes, F call, mfh_plan** plan)
 {
     // side streams of the bucket fork exist before the capture starts
     int sst = mfh_side_streams(h, h->bucket_streams);
     if (sst) return sst;
     void* pws = nullptr;
-    MFH_HIP_CHECK(hipMalloc(&pws, ws_bytes));
+    void* paux = nullptr;
+    MFH_HIP_CHECK(hipMalloc(&pws, ws_bytes < 256 ? 256 : ws_bytes));
+    if (aux_bytes) {
+        hipError_t ea = hipMalloc(&paux, aux_bytes);
+        if (ea != hipSuccess) {
+            hipFree(pws);
+            return mfh_fail(MFH_ERR_ALLOC, std::string("hipMalloc plan buffer: ") + hipGetErrorString(ea));
+        }
+    }
     hipStream_t cap;
     hipError_t e = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking);
     if (e != hipSuccess) {
         hipFree(pws);
+        if (paux) hipFree(paux);
         return mfh_fail(MFH_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
     }
     hipStream_t saved = h->stream;
     void* saved_ws = h->ws;
     size_t saved_bytes = h->ws_bytes;
+    void* saved_aux = h->aux;
+    size_t saved_aux_bytes = h->aux_bytes;
     h->stream = cap;
     h->ws = pws;
     h->ws_bytes = ws_bytes;
+    h->aux = paux;
+    h->aux_bytes = aux_bytes;
     h->capturing = true;
     int st = MFH_OK;
     hipGraph_t graph = nullptr;
@@ -692,6 +707,8 @@ static int capture_plan(mfh_handle* h, size_t ws_bytes, F call, mfh_plan** plan)
     h->stream = saved;
     h->ws = saved_ws;
     h->ws_bytes = saved_bytes;
+    h->aux = saved_aux;
+    h->aux_bytes = saved_aux_bytes;
     h->capturing = false;
     hipStreamDestroy(cap);
     hipGraphExec_t exec = nullptr;
@@ -699,14 +716,17 @@ static int capture_plan(mfh_handle* h, size_t ws_bytes, F call, mfh_plan** plan)
     if (st != MFH_OK || e != hipSuccess) {
         if (graph) hipGraphDestroy(graph);
         hipFree(pws);
+        if (paux) hipFree(paux);
         if (st != MFH_OK) return st;
         return mfh_fail(MFH_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
     }
     mfh_plan* p = new mfh_plan();
     p->h = h;
+    p->op = op;
     p->graph = graph;
     p->exec = exec;
     p->ws = pws;
+    p->aux = paux;
     *plan = p;
     return MFH_OK;
 }
@@ -719,8 +739,74 @@ int mfh_plan_create_dposv(mfh_handle* h, int count, const int* n, double* const*
     MFH_REQUIRE(h != nullptr, 1, "handle must not be null");
     MFH_REQUIRE(plan != nullptr, 9, "plan must not be null");
     MFH_REQUIRE(count >= 1, 2, "count must be >= 1");
+    MFH_REQUIRE(max_n >= 1 && max_n <= MFH_EXACT_MAX_N, 8, "max_n must be in [1, MFH_EXACT_MAX_N]");
     return capture_plan(
-        h, mfh_exact_ws_bytes(count), [&]() { return mfh_dposv_vbatched(h, count, n, a, lda, x, info, max_n); },
+        h, MFH_OP_DPOSV, mfh_exact_ws_bytes(count), mfh_potrf_gen_aux_bytes(h, count, max_n, false),
+        [&]() { return mfh_dposv_vbatched(h, count, n, a, lda, x, info, max_n); }, plan);
+}
+
+int mfh_plan_create_dgetrf(mfh_handle* h, int count, const int* m, const int* n, double* const* a, const int* lda,
+                           int* const* piv, int* sign, int* info, int max_m, int max_n, mfh_plan** plan)
+{
+    MFH_REQUIRE(h != nullptr, 1, "handle must not be null");
+    MFH_REQUIRE(plan != nullptr, 12, "plan must not be null");
+    MFH_REQUIRE(count >= 1, 2, "count must be >= 1");
+    return capture_plan(
+        h, MFH_OP_DGETRF, mfh_exact_ws_bytes(count), 0,
+        [&]() { return mfh_dgetrf_vbatched(h, count, m, n, a, lda, piv, sign, info, max_m, max_n); }, plan);
+}
+
+int mfh_plan_create_dpotrf(mfh_handle* h, int count, const int* n, double* const* a, const int* lda, int* info,
+                           int max_n, mfh_plan** plan)
+{
+    MFH_REQUIRE(h != nullptr, 1, "handle must not be null");
+    MFH_REQUIRE(plan != nullptr, 8, "plan must not be null");
+    MFH_REQUIRE(count >= 1, 2, "count must be >= 1");
+    MFH_REQUIRE(max_n >= 1 && max_n <= MFH_EXACT_MAX_N, 7, "max_n must be in [1, MFH_EXACT_MAX_N]");
+    return capture_plan(
+        h, MFH_OP_DPOTRF, mfh_exact_ws_bytes(count), mfh_potrf_gen_aux_bytes(h, count, max_n, false),
+        [&]() { return mfh_dpotrf_vbatched(h, count, n, a, lda, info, max_n); }, plan);
+}
+
+int mfh_plan_create_dtrsv(mfh_handle* h, int uplo, int trans, int diag, int count, const int* n,
+                          const double* const* a, const int* lda, double* const* x, const int* incx, int* info,
+                          int max_n, mfh_plan** plan)
+{
+    MFH_REQUIRE(h != nullptr, 1, "handle must not be null");
+    MFH_REQUIRE(plan != nullptr, 13, "plan must not be null");
+    MFH_REQUIRE(count >= 1, 5, "count must be >= 1");
+    return capture_plan(
+        h, MFH_OP_DTRSV, mfh_exact_ws_bytes(count), 0,
+        [&]() { return mfh_dtrsv_vbatched(h, uplo, trans, diag, count, n, a, lda, x, incx, info, max_n); }, plan);
+}
+
+int mfh_plan_create_dgesv(mfh_handle* h, int count, const int* n, const double* const* a, const int* lda,
+                          const double* const* b, double* const* x, double* berr, int* info, int max_n,
+                          mfh_plan** plan)
+{
+    MFH_REQUIRE(h != nullptr, 1, "handle must not be null");
+    MFH_REQUIRE(plan != nullptr, 11, "plan must not be null");
+    MFH_REQUIRE(count >= 1, 2, "count must be >= 1");
+    MFH_REQUIRE(max_n >= 1 && max_n <= MFH_BLOCKED_MAX_N, 10, "max_n must be in [1, MFH_BLOCKED_MAX_N]");
+    return capture_plan(
+        h, MFH_OP_DGESV, mfh_gesv_ws_bytes(count, max_n, false), 0,
+        [&]() { return mfh_dgesv_vbatched(h, count, n, a, lda, b, x, berr, info, max_n); }, plan);
+}
+
+int mfh_plan_create_dgesv_mixed(mfh_handle* h, int count, const int* n, const double* const* a, const int* lda,
+                                const double* const* b, double* const* x, int max_iter, double tol_per_n,
+                                int* iters, double* berr, int* info, int max_n, mfh_plan** plan)
+{
+    MFH_REQUIRE(h != nullptr, 1, "handle must not be null");
+    MFH_REQUIRE(plan != nullptr, 14, "plan must not be null");
+    MFH_REQUIRE(count >= 1, 2, "count must be >= 1");
+    MFH_REQUIRE(max_n >= 1 && max_n <= MFH_BLOCKED_MAX_N, 13, "max_n must be in [1, MFH_BLOCKED_MAX_N]");
+    return capture_plan(
+        h, MFH_OP_DGESV_MIXED, mfh_gesv_ws_bytes(count, max_n, true), 0,
+        [&]() {
+            return mfh_dgesv_mixed_vbatched(h, count, n, a, lda, b, x, max_iter, tol_per_n, iters, berr, info,
+                                            max_n);
+        },
         plan);
 }
 
@@ -736,7 +822,7 @@ static int plan_gemm(mfh_handle* h, int count, const int* transa, const int* tra
     MFH_REQUIRE(max_n >= 1 && max_n <= 4096, 18, "max_n must be in [1, 4096]");
     const size_t ws = mfh_gemm_ws_bytes(count, max_m, max_n, h->math_mode == MFH_MATH_FAST);
     return capture_plan(
-        h, ws,
+        h, cplx ? MFH_OP_ZGEMM : MFH_OP_DGEMM, ws, 0,
         [&]() {
             return cplx ? mfh_zgemm_vbatched(h, count, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc,
                                              info, max_m, max_n, max_k)
@@ -764,6 +850,14 @@ int mfh_plan_create_zgemm(mfh_handle* h, int count, const int* transa, const int
                      max_k, true, plan);
 }
 
+int mfh_plan_get_op(mfh_plan* plan, int* op)
+{
+    MFH_REQUIRE(plan != nullptr, 1, "plan must not be null");
+    MFH_REQUIRE(op != nullptr, 2, "op must not be null");
+    *op = plan->op;
+    return MFH_OK;
+}
+
 int mfh_plan_execute(mfh_plan* plan)
 {
     MFH_REQUIRE(plan != nullptr, 1, "plan must not be null");
@@ -776,10 +870,9 @@ int mfh_plan_destroy(mfh_plan* plan)
     MFH_REQUIRE(plan != nullptr, 1, "plan must not be null");
     if (plan->exec) hipGraphExecDestroy(plan->exec);
     if (plan->graph) hipGraphDestroy(plan->graph);
-    if (plan->ws) {
-        hipStreamSynchronize(plan->h->stream);
-        hipFree(plan->ws);
-    }
+    if (plan->ws || plan->aux) hipStreamSynchronize(plan->h->stream);
+    if (plan->aux) hipFree(plan->aux);
+    if (plan->ws) hipFree(plan->ws);
     delete plan;
     return MFH_OK;
 }

@@ -133,13 +133,17 @@ __device__ __forceinline__ void mfh_wave_argmax(double& key, int& idx)
 // Workgroup w takes items w, w + grid, w + 2*grid, ...: no shared counter,
 // so no single-address atomic traffic (which serialises in L2), and
 // neighbouring workgroups get neighbouring (similar-size) matrices.
+// A queue with items == nullptr is the whole batch in order: item idx is
+// matrix idx for idx < direct (a uniform batch needs no sort).
 struct MfhQueue {
     const int* start;  // device: first item of the bucket in items[]
     const int* count;  // device: number of items in the bucket
     const int* items;  // matrix ids sorted by size key, descending
+    int direct;        // items == nullptr: the batch size
 };
 
 __device__ __forceinline__ int mfh_item(const MfhQueue& q, int idx)
 {
+    if (q.items == nullptr) return idx < q.direct ? idx : -1;
     return (idx < *q.count) ? q.items[*q.start + idx] : -1;
 }

@@ -173,8 +173,8 @@ int device_sort(mfh_handle* h, const Op& op, int count, SortWs* out)
     return MFH_OK;
 }
 
-MfhQueue bucket_queue(const SortWs& ws, int b) { return MfhQueue{ws.bstart + b, ws.bcount + b, ws.items}; }
-MfhQueue fine_queue(const SortWs& ws, int f) { return MfhQueue{ws.fstart + f, ws.fcount + f, ws.items}; }
+MfhQueue bucket_queue(const SortWs& ws, int b) { return MfhQueue{ws.bstart + b, ws.bcount + b, ws.items, 0}; }
+MfhQueue fine_queue(const SortWs& ws, int f) { return MfhQueue{ws.fstart + f, ws.fcount + f, ws.items, 0}; }
 
 // Launches a persistent kernel for one bucket: as many workgroups as fit
 // on the device (by occupancy), capped by what the bucket could need.
@@ -894,10 +894,18 @@ int run_chol_bucket(mfh_handle* h, int b, int nb_rows, int count, const MfhCholB
 
 int mfh_launch_getrf(mfh_handle* h, int count, const MfhGetrfBatch& B, int max_m, int max_n)
 {
+    const int top = max_m > max_n ? max_m : max_n;
+    if (B.a == nullptr && top <= MFH_LDS_MAX_N) {
+        // one shape, checked on the host: no sort, one launch over the batch
+        const MfhQueue q{nullptr, nullptr, nullptr, count};
+        const size_t lds = square_lds_bytes(top);
+        if (threads_for(top) == 64)
+            return launch_bucket(h, MFH_KERNEL_GETRF_EXACT, top, k_getrf_exact<64>, 64, lds, count, B, q);
+        return launch_bucket(h, MFH_KERNEL_GETRF_EXACT, top, k_getrf_exact<128>, 128, lds, count, B, q);
+    }
     SortWs ws;
     int st = device_sort(h, GetrfKey{B, max_m, max_n}, count, &ws);
     if (st) return st;
-    const int top = max_m > max_n ? max_m : max_n;
     const int key = top > MFH_LDS_MAX_N ? MFH_LDS_MAX_N : top;
     for (int b = 0; b < nbuckets_for(key); ++b) {
         const int nb = bucket_bound(b, key);

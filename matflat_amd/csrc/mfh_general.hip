@@ -636,6 +636,19 @@ int launch_trsv(mfh_handle* h, int count, const GenTrsv<E>& B)
 
 }  // namespace
 
+// Bytes of the handle's second buffer a Cholesky launch of this shape uses
+// (its HBM scratch; 0 when every matrix is staged in LDS): plans reserve it
+// before capture.
+size_t mfh_potrf_gen_aux_bytes(const mfh_handle* h, int count, int max_n, bool cplx)
+{
+    if (!cplx && max_n <= MFH_LDS_MAX_N) return 0;
+    const size_t elem = cplx ? sizeof(zc) : sizeof(double);
+    if ((size_t)(max_n | 1) * max_n * elem <= kStageBytes) return 0;
+    long cap = 2L * h->num_cus;
+    if (cap > count) cap = count;
+    return (size_t)cap * max_n * max_n * elem;
+}
+
 // ---------------------------------------------------------------- double
 // Orders above the LDS-resident kernels' range (MFH_LDS_MAX_N); the size
 // sort of mfh_exact.hip has reported the invalid matrices already.

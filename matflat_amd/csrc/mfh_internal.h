@@ -209,6 +209,7 @@ int mfh_launch_gesv_mixed(mfh_handle* h, int count, const int* n, const double* 
                           bool mixed);
 // mfh_general.hip: double matrices above MFH_LDS_MAX_N rows (the size sort
 // has reported invalid ones), and the complex128 entry points.
+size_t mfh_potrf_gen_aux_bytes(const mfh_handle* h, int count, int max_n, bool cplx);
 int mfh_launch_getrf_gen_d(mfh_handle* h, int count, const MfhGetrfBatch& B, int max_m, int max_n);
 int mfh_launch_potrf_gen_d(mfh_handle* h, int count, const MfhCholBatch& B, int max_n, bool posv);
 int mfh_launch_trsv_gen_d(mfh_handle* h, int count, const MfhTrsvBatch& B, int max_n);

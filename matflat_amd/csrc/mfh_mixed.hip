@@ -78,6 +78,7 @@ struct GesvArgs {
     long long wstride;
     int ldw;           // row stride of W (elements)
     int* counter;      // work counter (zeroed before launch)
+    int* active;       // [max_iter + 2]: matrices still active entering step it (zeroed before launch)
     // state kept between the factorization and the refinement launches
     int* perm;         // [count][ldw] position -> original row of A
     double* resid;     // [count][ldw] last residual b - A x
@@ -1182,6 +1183,8 @@ __device__ __forceinline__ void gesv_step(const GesvArgs& G, const Lds<T>& L, in
             const bool bad = MIXED ? !(eta <= G.tol_per_n * n) : !(eta <= __builtin_huge_val());
             G.info[bi] = zero ? zero : (bad ? n + 1 : 0);
             G.state[bi] = 1;
+        } else {
+            atomicAdd(&G.active[it + 1], 1);
         }
     }
 }
@@ -1266,6 +1269,10 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(MFH_STEP_WAV
 {
     extern __shared__ __attribute__((aligned(16))) char lds_raw[];
     const Lds<T> L = carve_step<T>(lds_raw, G.ldw);
+    // a refinement step with nothing left to refine ends at once (the
+    // previous step counted the matrices it left active; every workgroup
+    // reads the same count)
+    if (it > 0 && __builtin_amdgcn_readfirstlane(*(volatile int*)&G.active[it]) == 0) return;
     for (int bi = blockIdx.x; bi < G.count; bi += gridDim.x) {
         const int active = __builtin_amdgcn_readfirstlane(G.state[bi]) == 0;
         if (active) {
@@ -1277,7 +1284,12 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(MFH_STEP_WAV
     }
 }
 
-__global__ void k_zero_counter(int* c) { *c = 0; }
+// Zeroes the work counter and the per-step active counts (a kernel, not a
+// memset: the launch sequence may be captured into a graph).
+__global__ void k_zero_counter(int* c, int n)
+{
+    for (int i = threadIdx.x; i < n; i += blockDim.x) c[i] = 0;
+}
 
 // Factor-only mode: writes the FP64 factors from the row-major workspace W
 // back to the caller's column-major matrix (through 32 x 33 LDS tiles, both
@@ -1329,7 +1341,7 @@ int launch_gesv(mfh_handle* h, GesvArgs G)
     if (per_cu < 1) per_cu = 1;
     long grid = (long)per_cu * h->num_cus;
     if (grid > G.count) grid = G.count;
-    hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, h->stream, G.counter);
+    hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(128), 0, h->stream, G.counter, 2 + G.max_iter + 1);
     int st = mfh_timing_begin(h, MIXED ? MFH_KERNEL_GESV_MIXED : MFH_KERNEL_GESV_F64, G.max_n);
     if (st) return st;
     hipLaunchKernelGGL((k_gesv<T, MIXED>), dim3((unsigned)grid), dim3(BT), lds, h->stream, G);
@@ -1381,7 +1393,7 @@ GesvLayout gesv_layout(int count, int max_n, bool mixed)
     const size_t ldw = (size_t)((max_n + 7) & ~7);
     GesvLayout g;
     g.per = up(ldw * max_n * (mixed ? 4 : 8));
-    g.w = 256;
+    g.w = 512;  // work counter, then the active counts of up to 101 steps
     g.perm = g.w + g.per * (size_t)count;
     g.resid = g.perm + up(sizeof(int) * ldw * count);
     g.norms = g.resid + up(sizeof(double) * ldw * count);
@@ -1418,6 +1430,7 @@ static int launch_gesv_common(mfh_handle* h, int count, const int* n, const doub
     G.tol_per_n = tol_per_n;
     char* base = (char*)h->ws;
     G.counter = (int*)base;
+    G.active = (int*)base + 1;
     G.ws = base + lay.w;
     G.wstride = (long long)lay.per;
     G.ldw = ldw;

@@ -279,59 +279,86 @@ def dposv_backward_vbatched(n, l_ptrs, lda, x_ptrs, xbar_ptrs, abar_ptrs, ldabar
                                              int(max_n)))
 
 
-class GemmPlan:
+class Plan:
+    """One batched call captured into a hipGraph (mfh_plan_create_*) and
+    replayed by execute() on the handle's stream.  The device arrays the call
+    was given must outlive the plan."""
+
+    def __init__(self, create, args, handle: Handle | None = None):
+        self.handle = handle or default_handle()
+        self._keep = args
+        p = ctypes.c_void_p()
+        _check(create(self.handle.h, *[_p(a) if not isinstance(a, (int, float)) else a for a in args],
+                      ctypes.byref(p)))
+        self.p = p
+
+    def op(self) -> int:
+        v = ctypes.c_int()
+        _check(self.handle.lib.mfh_plan_get_op(self.p, ctypes.byref(v)))
+        return v.value
+
+    def execute(self) -> None:
+        self.handle.bind_stream()
+        _check(self.handle.lib.mfh_plan_execute(self.p))
+
+    def close(self) -> None:
+        if getattr(self, "p", None) is not None and self.p.value:
+            self.handle.lib.mfh_plan_destroy(self.p)
+            self.p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class GemmPlan(Plan):
     """A batched GEMM (gemm_vbatched's arguments) captured once into a
     hipGraph and replayed."""
 
     def __init__(self, transa, transb, m, n, k, alpha, a_ptrs, lda, b_ptrs, ldb, beta, c_ptrs, ldc, info,
                  max_m, max_n, max_k, complex_: bool = False, handle: Handle | None = None):
-        self.handle = handle or default_handle()
-        self._keep = (transa, transb, m, n, k, alpha, a_ptrs, lda, b_ptrs, ldb, beta, c_ptrs, ldc, info)
-        fn = self.handle.lib.mfh_plan_create_zgemm if complex_ else self.handle.lib.mfh_plan_create_dgemm
-        p = ctypes.c_void_p()
-        _check(fn(self.handle.h, int(m.numel()), _p(transa), _p(transb), _p(m), _p(n), _p(k), _p(alpha), _p(a_ptrs),
-                  _p(lda), _p(b_ptrs), _p(ldb), _p(beta), _p(c_ptrs), _p(ldc), _p(info), int(max_m), int(max_n),
-                  int(max_k), ctypes.byref(p)))
-        self.p = p
-
-    def execute(self) -> None:
-        self.handle.bind_stream()
-        _check(self.handle.lib.mfh_plan_execute(self.p))
-
-    def close(self) -> None:
-        if self.p is not None and self.p.value:
-            self.handle.lib.mfh_plan_destroy(self.p)
-            self.p = None
-
-    def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        h = handle or default_handle()
+        fn = h.lib.mfh_plan_create_zgemm if complex_ else h.lib.mfh_plan_create_dgemm
+        super().__init__(fn, (int(m.numel()), transa, transb, m, n, k, alpha, a_ptrs, lda, b_ptrs, ldb, beta, c_ptrs,
+                              ldc, info, int(max_m), int(max_n), int(max_k)), h)
 
 
-class DposvPlan:
+class DposvPlan(Plan):
     """A batched SPD solve captured once into a hipGraph and replayed."""
 
     def __init__(self, n, a_ptrs, lda, x_ptrs, info, max_n, handle: Handle | None = None):
-        self.handle = handle or default_handle()
-        self._keep = (n, a_ptrs, lda, x_ptrs, info)
-        p = ctypes.c_void_p()
-        _check(self.handle.lib.mfh_plan_create_dposv(self.handle.h, int(n.numel()), _p(n), _p(a_ptrs), _p(lda),
-                                                     _p(x_ptrs), _p(info), int(max_n), ctypes.byref(p)))
-        self.p = p
+        h = handle or default_handle()
+        super().__init__(h.lib.mfh_plan_create_dposv, (int(n.numel()), n, a_ptrs, lda, x_ptrs, info, int(max_n)), h)
 
-    def execute(self) -> None:
-        self.handle.bind_stream()
-        _check(self.handle.lib.mfh_plan_execute(self.p))
 
-    def close(self) -> None:
-        if self.p is not None and self.p.value:
-            self.handle.lib.mfh_plan_destroy(self.p)
-            self.p = None
+def plan_dgetrf(m, n, a_ptrs, lda, piv_ptrs, sign, info, max_m, max_n, handle: Handle | None = None) -> Plan:
+    h = handle or default_handle()
+    return Plan(h.lib.mfh_plan_create_dgetrf,
+                (int(m.numel()), m, n, a_ptrs, lda, piv_ptrs, sign, info, int(max_m), int(max_n)), h)
 
-    def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+
+def plan_dpotrf(n, a_ptrs, lda, info, max_n, handle: Handle | None = None) -> Plan:
+    h = handle or default_handle()
+    return Plan(h.lib.mfh_plan_create_dpotrf, (int(n.numel()), n, a_ptrs, lda, info, int(max_n)), h)
+
+
+def plan_dtrsv(uplo, trans, diag, n, a_ptrs, lda, x_ptrs, incx, info, max_n, handle: Handle | None = None) -> Plan:
+    h = handle or default_handle()
+    return Plan(h.lib.mfh_plan_create_dtrsv,
+                (int(uplo), int(trans), int(diag), int(n.numel()), n, a_ptrs, lda, x_ptrs, incx, info, int(max_n)), h)
+
+
+def plan_dgesv(n, a_ptrs, lda, b_ptrs, x_ptrs, berr, info, max_n, handle: Handle | None = None) -> Plan:
+    h = handle or default_handle()
+    return Plan(h.lib.mfh_plan_create_dgesv, (int(n.numel()), n, a_ptrs, lda, b_ptrs, x_ptrs, berr, info, int(max_n)),
+                h)
+
+
+def plan_dgesv_mixed(n, a_ptrs, lda, b_ptrs, x_ptrs, max_iter, tol_per_n, iters, berr, info, max_n,
+                     handle: Handle | None = None) -> Plan:
+    h = handle or default_handle()
+    return Plan(h.lib.mfh_plan_create_dgesv_mixed,
+                (int(n.numel()), n, a_ptrs, lda, b_ptrs, x_ptrs, int(max_iter), float(tol_per_n), iters, berr, info,
+                 int(max_n)), h)

@@ -3,21 +3,22 @@
 ``Factorization.Lu``, ``Factorization.Cholesky``, ``Blas.SolveTriangular``
 and ``Blas.MulMatMat`` keep the reference's argument lists, argument checks
 (same order, same conditions) and exceptions, with device tensors in place
-of pointers: ``a`` is a 1-D float64 (complex128 for complex GEMM) CUDA
-tensor holding a column-major matrix with leading dimension ``lda``, exactly
-the buffer the reference's pointer addresses.  Each call runs the batched
+of pointers: ``a`` is a 1-D float64 or complex128 CUDA tensor holding a
+column-major matrix with leading dimension ``lda``, exactly the buffer the
+reference's pointer addresses; a complex128 tensor selects the reference's
+Complex overload.  Each call runs the batched
 HIP kernel with a batch of one and waits for it (the reference is
 synchronous).
 
 Reference signatures:
-* Factorization.Lu(int m, int n, double* a, int lda, int* piv) -> int
-  (MatFlat/Factorization.Lu.cs:122-161)
-* Factorization.Cholesky(int n, double* a, int lda)
-  (MatFlat/Factorization.Cholesky.cs:114-168)
+* Factorization.Lu(int m, int n, double* / Complex* a, int lda, int* piv) -> int
+  (MatFlat/Factorization.Lu.cs:122-161, :199-238)
+* Factorization.Cholesky(int n, double* / Complex* a, int lda)
+  (MatFlat/Factorization.Cholesky.cs:114-168, :195-249)
 * Factorization.LuInverse(int n, double* a, int lda, int* piv)
   (MatFlat/Factorization.LuInverse.cs:105-192)
-* Blas.SolveTriangular(Uplo, Transpose, int n, double* a, int lda, double* x, int incx)
-  (MatFlat/Blas.SolveTriangular.cs:162-242)
+* Blas.SolveTriangular(Uplo, Transpose, int n, double* / Complex* a, int lda, x, int incx)
+  (MatFlat/Blas.SolveTriangular.cs:162-242, :280-395)
 * Blas.MulMatMat(Transpose, Transpose, int m, int n, int k, double* a, int lda,
   double* b, int ldb, double* c, int ldc)  (MatFlat/Blas.MulMatMat.cs:226-368, :407-575)
 * Blas.MulMatVec(Transpose, int m, int n, double* a, int lda, double* x, int incx,
@@ -114,7 +115,11 @@ class Factorization:
         dev = _dev(a)
         sign = torch.zeros(1, dtype=torch.int32, device=dev)
         info = torch.zeros(1, dtype=torch.int32, device=dev)
-        hip.dgetrf_strided_batched(1, m, n, a, lda, lda * n, piv, m, sign, info, handle=_exact())
+        if a.is_complex():  # Factorization.Lu.cs:199-238
+            hip.zgetrf_vbatched(_i32(m, dev), _i32(n, dev), _ptrs(a), _i32(lda, dev), _ptrs(piv), sign, info, m, n,
+                                handle=_exact())
+        else:
+            hip.dgetrf_strided_batched(1, m, n, a, lda, lda * n, piv, m, sign, info, handle=_exact())
         torch.cuda.synchronize()
         return int(sign.item())
 
@@ -130,7 +135,10 @@ class Factorization:
         _need(a, lda * (n - 1) + n, "a")
         dev = _dev(a)
         info = torch.zeros(1, dtype=torch.int32, device=dev)
-        hip.dpotrf_vbatched(_i32(n, dev), _ptrs(a), _i32(lda, dev), info, n, handle=_exact())
+        if a.is_complex():  # Factorization.Cholesky.cs:195-249
+            hip.zpotrf_vbatched(_i32(n, dev), _ptrs(a), _i32(lda, dev), info, n, handle=_exact())
+        else:
+            hip.dpotrf_vbatched(_i32(n, dev), _ptrs(a), _i32(lda, dev), info, n, handle=_exact())
         if int(info.item()) != 0:
             raise MatrixFactorizationException(
                 "Cholesky decomposition failed. The matrix must be positive definite.")
@@ -172,16 +180,25 @@ class Blas:
             raise ArgumentNullException("x")
         if incx <= 0:
             raise ArgumentException("The value must be greater than or equal to one.", "incx")
+        cplx = a.is_complex()
+        # the reference checks transa inside the uplo branch, so an invalid
+        # uplo is reported first (Blas.SolveTriangular.cs:190-241, :306-394)
         if uplo not in (Uplo.Upper, Uplo.Lower):
             raise ArgumentException("Invalid enum value.", "uplo")
-        if transa not in (Transpose.NoTrans, Transpose.Trans):
+        valid = (Transpose.NoTrans, Transpose.Trans, Transpose.ConjTrans, Transpose.ConjNoTrans) if cplx else \
+            (Transpose.NoTrans, Transpose.Trans)
+        if transa not in valid:
             raise ArgumentException("Invalid enum value.", "transa")
         _need(a, lda * (n - 1) + n, "a")
         _need(x, incx * (n - 1) + 1, "x")
         dev = _dev(a)
         info = torch.zeros(1, dtype=torch.int32, device=dev)
-        hip.dtrsv_vbatched(int(uplo), int(transa), hip.NONUNIT, _i32(n, dev), _ptrs(a), _i32(lda, dev), _ptrs(x),
-                           _i32(incx, dev), info, n, handle=_exact())
+        if cplx:  # Blas.SolveTriangular.cs:280-395
+            hip.ztrsv_vbatched(int(uplo), int(transa), _i32(n, dev), _ptrs(a), _i32(lda, dev), _ptrs(x),
+                               _i32(incx, dev), info, n, handle=_exact())
+        else:
+            hip.dtrsv_vbatched(int(uplo), int(transa), hip.NONUNIT, _i32(n, dev), _ptrs(a), _i32(lda, dev), _ptrs(x),
+                               _i32(incx, dev), info, n, handle=_exact())
         torch.cuda.synchronize()
 
     @staticmethod

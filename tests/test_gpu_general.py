@@ -375,3 +375,49 @@ def test_limits_of_small_entry_points():
         hip.dposv_vbatched_noalloc(i32([2]), a.ptrs, i32([2]), a.ptrs, info, 129, handle=h)
     with pytest.raises(hip.ArgumentError):
         hip.dgetri_vbatched(i32([2]), a.ptrs, i32([2]), a.ptrs, info, 129, handle=h)
+
+
+def test_matflat_complex_overloads():
+    """The drop-in signatures with complex128 tensors select the Complex
+    overloads (Factorization.Lu.cs:199-238, Factorization.Cholesky.cs:195-249,
+    Blas.SolveTriangular.cs:280-395)."""
+    from matflat_amd.matflat import (ArgumentException, Blas, Factorization, MatrixFactorizationException,
+                                     Transpose, Uplo)
+    m, n, lda = 13, 9, 15
+    a = rand_cbuf(5, m, n, lda, dist="normal")
+    dev = torch.from_numpy(a.copy()).cuda()
+    piv = torch.zeros(m, dtype=torch.int32, device="cuda")
+    sign = Factorization.Lu(m, n, dev, lda, piv)
+    want = a.copy()
+    wpiv, wsign = oracle.zgetrf(m, n, want, lda)
+    assert_bits_equal(dev.cpu().numpy(), want, "complex Lu")
+    assert np.array_equal(piv.cpu().numpy(), wpiv) and sign == wsign
+    h = hpd_upper_cbuf(8, 7, 8)
+    dev = torch.from_numpy(h.copy()).cuda()
+    Factorization.Cholesky(7, dev, 8)
+    want = h.copy()
+    assert oracle.zpotrf(7, want, 8) == 0
+    assert_bits_equal(dev.cpu().numpy(), want, "complex Cholesky")
+    bad = h.copy()
+    bad[3 * 8 + 3] = -1.0
+    dev = torch.from_numpy(bad.copy()).cuda()
+    with pytest.raises(MatrixFactorizationException):
+        Factorization.Cholesky(7, dev, 8)
+    want = bad.copy()
+    assert oracle.zpotrf(7, want, 8) == 4
+    assert_bits_equal(dev.cpu().numpy(), want, "complex Cholesky partial writes")
+    t = rand_cbuf(9, 6, 6, 6) + 3.0 * np.eye(6).reshape(-1)
+    tri = torch.from_numpy(t.copy()).cuda()
+    for tr in (Transpose.NoTrans, Transpose.Trans, Transpose.ConjTrans, Transpose.ConjNoTrans):
+        x0 = rand_cbuf(10 + int(tr), 6, 1, 6)
+        x = torch.from_numpy(x0.copy()).cuda()
+        Blas.SolveTriangular(Uplo.Upper, tr, 6, tri, 6, x, 1)
+        want = x0.copy()
+        oracle.ztrsv(121, int(tr), 6, t, 6, want, 1)
+        assert_bits_equal(x.cpu().numpy(), want, f"complex SolveTriangular {tr}")
+    with pytest.raises(ArgumentException):
+        Blas.SolveTriangular(Uplo.Upper, 115, 6, tri, 6, x, 1)
+    with pytest.raises(ArgumentException):
+        Blas.SolveTriangular(Uplo.Lower, Transpose.ConjTrans, 6, torch.zeros(36, dtype=torch.float64,
+                                                                                  device="cuda"), 6,
+                             torch.zeros(6, dtype=torch.float64, device="cuda"), 1)
