@@ -94,9 +94,21 @@ int mfh_get_math_mode(mfh_handle* handle, int* mode);
 int mfh_set_fast_waves(mfh_handle* handle, int row_blocks, int waves);
 int mfh_get_fast_waves(mfh_handle* handle, int row_blocks, int* waves);
 
+/* Workgroup shape of the exact (MatFlat-order) Cholesky / SPD-solve kernel
+ * for size bucket `bucket` (matrices of 16*bucket+1 .. 16*(bucket+1) rows,
+ * n + 1 rows for the SPD solve): `lanes` lanes per matrix (8, 16, 32 or 64;
+ * 64 / lanes matrices per wavefront) and `rows_per_lane` rows each (2 or 3),
+ * among the built shapes (8,2) (16,2) (16,3) (32,2) (64,2) (64,3) with
+ * lanes * rows_per_lane at least the bucket's rows and the packed triangles
+ * of a wavefront's matrices within 160 KiB of LDS; (0, 0) restores the
+ * built-in choice.  Results do not depend on the shape (every element is
+ * the same MatFlat-order dot product); matflat_amd/autotune.py picks the
+ * fastest per bucket. */
+int mfh_set_chol_shape(mfh_handle* handle, int bucket, int lanes, int rows_per_lane);
+
 /* Concurrency of the FAST-mode Cholesky / SPD-solve size buckets (no MatFlat
  * counterpart): the per-bucket launches are forked over `streams` (1..4,
- * default 2) streams of default priority that the handle owns, largest
+ * default 3) streams of default priority that the handle owns, largest
  * bucket first, and joined back into the handle's stream, so a bucket's
  * last partial round overlaps the next bucket.  1 keeps them in order on
  * the handle's stream.  With per-launch timing on (mfh_set_timing) the

@@ -52,6 +52,7 @@ SIGNATURES = {
     "mfh_set_fast_waves": [_p, _i, _i],
     "mfh_get_fast_waves": [_p, _i, _p],
     "mfh_set_bucket_streams": [_p, _i],
+    "mfh_set_chol_shape": [_p, _i, _i, _i],
     "mfh_get_bucket_streams": [_p, _p],
     "mfh_set_timing": [_p, _i],
     "mfh_get_timing": [_p, _i, _p, _p, _p, _p],

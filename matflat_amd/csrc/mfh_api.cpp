@@ -221,6 +221,18 @@ int mfh_set_fast_waves(mfh_handle* h, int row_blocks, int waves)
     return MFH_OK;
 }
 
+int mfh_set_chol_shape(mfh_handle* h, int bucket, int lanes, int rows_per_lane)
+{
+    MFH_REQUIRE(h != nullptr, 1, "handle must not be null");
+    MFH_REQUIRE(bucket >= 0 && bucket < MFH_NBUCKETS, 2, "bucket must be in [0, 8]");
+    const bool reset = lanes == 0 && rows_per_lane == 0;
+    MFH_REQUIRE(reset || mfh_chol_shape_ok(bucket, lanes, rows_per_lane), 3,
+                "(lanes, rows_per_lane) must be a built shape covering the bucket's rows, or (0, 0)");
+    h->chol_shape[bucket][0] = lanes;
+    h->chol_shape[bucket][1] = rows_per_lane;
+    return MFH_OK;
+}
+
 int mfh_set_bucket_streams(mfh_handle* h, int streams)
 {
     MFH_REQUIRE(h != nullptr, 1, "handle must not be null");

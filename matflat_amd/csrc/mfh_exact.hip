@@ -855,12 +855,14 @@ struct CholShape {
     int G, S;
 };
 
+// Built-in shapes, as matflat_amd/autotune.py measured them on MI355X
+// (its table is matflat_amd/tuning/gfx950.json).
 CholShape chol_shape(int b)
 {
     switch (b) {
-    case 0: return {8, 2};
+    case 0: return {16, 2};
     case 1: return {16, 2};
-    case 2: return {16, 3};
+    case 2: return {32, 2};
     case 3: return {32, 2};
     case 8: return {64, 3};
     default: return {64, 2};
@@ -870,7 +872,9 @@ CholShape chol_shape(int b)
 template <bool POSV>
 int run_chol_bucket(mfh_handle* h, int b, int nb_rows, int count, const MfhCholBatch& B, const SortWs& ws)
 {
-    const CholShape sh = chol_shape(b);
+    CholShape sh = chol_shape(b);
+    if (h->chol_shape[b][0] > 0 && h->chol_shape[b][0] * h->chol_shape[b][1] >= nb_rows)
+        sh = CholShape{h->chol_shape[b][0], h->chol_shape[b][1]};
     int per_mat = nb_rows * (nb_rows - 1) / 2 + 5;
     per_mat = (per_mat + 1) & ~1;
     const int mw = MFH_WAVE / sh.G;
@@ -891,6 +895,18 @@ int run_chol_bucket(mfh_handle* h, int b, int nb_rows, int count, const MfhCholB
 }
 
 }  // namespace
+
+bool mfh_chol_shape_ok(int bucket, int g, int s)
+{
+    const bool built = (g == 8 && s == 2) || (g == 16 && s == 2) || (g == 16 && s == 3) || (g == 32 && s == 2) ||
+                       (g == 64 && s == 2) || (g == 64 && s == 3);
+    if (!built || bucket < 0 || bucket >= MFH_NBUCKETS) return false;
+    // every row of the bucket's largest matrix has a lane, and the packed
+    // triangles of the 64 / g matrices of a wavefront fit the LDS
+    const long rows = MFH_BUCKET_STEP * (bucket + 1);
+    const long per_mat = rows * (rows - 1) / 2 + 6;
+    return g * s >= rows && per_mat * 8 * (64 / g) <= 160 * 1024;
+}
 
 int mfh_launch_getrf(mfh_handle* h, int count, const MfhGetrfBatch& B, int max_m, int max_n)
 {

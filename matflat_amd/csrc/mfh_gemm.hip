@@ -43,6 +43,33 @@ constexpr int SCAN_T = 256;
 constexpr int SCAN_E = 16;
 constexpr int SCAN_CHUNK = SCAN_T * SCAN_E;
 
+// XCD-aware walk of the tile list.  Workgroups are dealt round-robin to the
+// eight XCDs of the device, each with its own L2: workgroup w runs on XCD
+// w % 8.  Walking the list with one global stride puts the tiles of a
+// product -- which read the same rows of op(A) and columns of op(B) -- on up
+// to four XCDs, and each of those L2s fetches the operands from HBM again.
+// Here the list is cut into eight contiguous ranges and the workgroups of
+// XCD x walk range x with their own stride, so a product's tiles share one
+// L2 (only the products that straddle a range boundary are split).
+constexpr int NXCD = 8;
+
+struct TileWalk {
+    int next, end, stride;
+};
+
+__device__ inline TileWalk tile_walk(int total, int per_group, int sub)
+{
+    const int groups = (int)gridDim.x;
+    const int nx = (groups % NXCD == 0) ? NXCD : 1;  // the host makes the grid a multiple of 8
+    const int x = (int)blockIdx.x % nx, r = (int)blockIdx.x / nx;
+    const int first = (int)((long)total * x / nx), end = (int)((long)total * (x + 1) / nx);
+    return TileWalk{first + r * per_group + sub, end, (groups / nx) * per_group};
+}
+
+// Grid size for the XCD-aware walk: a multiple of the XCD count once there
+// are at least that many workgroups.
+inline long xcd_grid(long grid) { return grid >= NXCD ? grid / NXCD * NXCD : grid; }
+
 __device__ inline bool is_trans(int t) { return t == MFH_TRANS || t == MFH_CONJTRANS; }
 __device__ inline bool is_conj(int t) { return t == MFH_CONJTRANS || t == MFH_CONJNOTRANS; }
 
@@ -176,8 +203,8 @@ __global__ __launch_bounds__(GT) void k_dgemm_exact(MfhGemmBatch B, int count, G
     __shared__ double Bs[KC][TN + 1];
     const int t = threadIdx.x;
     const int tr = t % 8, tc = t / 8;
-    const int total = ws.tiles[count];
-    for (int tile = blockIdx.x; tile < total; tile += gridDim.x) {
+    const TileWalk walk = tile_walk(ws.tiles[count], 1, 0);
+    for (int tile = walk.next; tile < walk.end; tile += walk.stride) {
         const int p = ws.map[tile];
         const int local = tile - ws.tiles[p];
         const int m = B.m[p], n = B.n[p], k = B.k[p];
@@ -324,8 +351,8 @@ __global__ __launch_bounds__(GT) void k_zgemm_exact(MfhGemmBatch B, int count, G
     __shared__ Cplx Bs[KC][TN + 1];
     const int t = threadIdx.x;
     const int tr = t % 8, tc = t / 8;
-    const int total = ws.tiles[count];
-    for (int tile = blockIdx.x; tile < total; tile += gridDim.x) {
+    const TileWalk walk = tile_walk(ws.tiles[count], 1, 0);
+    for (int tile = walk.next; tile < walk.end; tile += walk.stride) {
         const int p = ws.map[tile];
         const int local = tile - ws.tiles[p];
         const int m = B.m[p], n = B.n[p], k = B.k[p];
@@ -658,8 +685,8 @@ __global__ __launch_bounds__(64 * FW) void k_gemm_mfma(MfhGemmBatch B, int count
     constexpr int TS = FAST_TS;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int total = ws.tiles[count];
-    for (int tile = blockIdx.x * FW + wave; tile < total; tile += gridDim.x * FW) {
+    const TileWalk walk = tile_walk(ws.tiles[count], FW, wave);
+    for (int tile = walk.next; tile < walk.end; tile += walk.stride) {
         const int p = __builtin_amdgcn_readfirstlane(ws.map[tile]);
         const int local = tile - ws.tiles[p];
         const int m = B.m[p], n = B.n[p];
@@ -741,6 +768,7 @@ int mfh_launch_gemm(mfh_handle* h, int count, const MfhGemmBatch& B, int max_m, 
         long grid = (long)per_cu * h->num_cus;
         const long groups = (maxtiles + FW - 1) / FW;
         if (grid > groups) grid = groups;
+        grid = xcd_grid(grid);
         if (grid < 1) grid = 1;
         if (cplx) hipLaunchKernelGGL(k_gemm_mfma<true>, dim3((unsigned)grid), dim3(64 * FW), 0, h->stream, B, count, ws);
         else hipLaunchKernelGGL(k_gemm_mfma<false>, dim3((unsigned)grid), dim3(64 * FW), 0, h->stream, B, count, ws);
@@ -753,6 +781,7 @@ int mfh_launch_gemm(mfh_handle* h, int count, const MfhGemmBatch& B, int max_m, 
     if (per_cu < 1) per_cu = 1;
     long grid = (long)per_cu * h->num_cus;
     if (grid > maxtiles) grid = maxtiles;
+    grid = xcd_grid(grid);
     if (grid < 1) grid = 1;
     if (cplx) hipLaunchKernelGGL(k_zgemm_exact, dim3((unsigned)grid), dim3(GT), 0, h->stream, B, count, ws);
     else hipLaunchKernelGGL(k_dgemm_exact, dim3((unsigned)grid), dim3(GT), 0, h->stream, B, count, ws);

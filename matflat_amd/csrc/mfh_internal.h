@@ -45,8 +45,9 @@ struct mfh_handle {
     size_t scratch_bytes = 0;
     int math_mode = MFH_MATH_EXACT;
     int fast_waves[18] = {};              // FAST Cholesky waves per matrix by row blocks (0 = default)
+    int chol_shape[MFH_NBUCKETS][2] = {}; // exact Cholesky: lanes per matrix, rows per lane, by bucket (0 = default)
     bool timing = false;                  // bracket launches with events
-    int bucket_streams = 2;               // FAST bucket launches forked over this many streams (1 = in order)
+    int bucket_streams = 3;               // FAST bucket launches forked over this many streams (1 = in order)
     std::vector<hipStream_t> side;        // side streams of the fork (default priority), created on first use
     std::vector<hipEvent_t> side_ev;      // fork / join events, one per side stream plus one
     std::vector<MfhTimedLaunch> timed;    // recorded launches not yet read
@@ -179,6 +180,10 @@ size_t mfh_exact_ws_bytes(int count);
 
 // Workspace bytes of one GEMM call (mfh_gemm.hip); plans reserve this.
 size_t mfh_gemm_ws_bytes(int count, int max_m, int max_n, bool fast);
+
+// Shapes (lanes per matrix G, rows per lane S) the exact Cholesky kernel is
+// built for; a bucket of up to R rows can take any of them with G * S >= R.
+bool mfh_chol_shape_ok(int bucket, int g, int s);
 
 // Built-in choice of wavefronts per matrix (1 or 2) of the FAST Cholesky
 // kernel for matrices of up to 8 * rb rows.

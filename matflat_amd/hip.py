@@ -60,9 +60,13 @@ def _p(t) -> int:
 
 
 class Handle:
-    """Owns an mfh_handle bound to torch's current stream at call time."""
+    """Owns an mfh_handle bound to torch's current stream at call time.
 
-    def __init__(self, device: int | None = None):
+    With ``tuned`` (the default) the launch shapes of the committed tuning
+    table for this GPU's architecture (matflat_amd/tuning/<arch>.json,
+    written by ``python -m matflat_amd.autotune``) are applied."""
+
+    def __init__(self, device: int | None = None, tuned: bool = True):
         self.lib = _lib.load()
         if device is not None:
             torch.cuda.set_device(device)
@@ -70,6 +74,17 @@ class Handle:
         h = ctypes.c_void_p()
         _check(self.lib.mfh_create(ctypes.byref(h)))
         self.h = h
+        if tuned:
+            from . import autotune
+            arch = torch.cuda.get_device_properties(torch.cuda.current_device()).gcnArchName.split(":")[0]
+            table = autotune.load(arch)
+            if table is not None:
+                autotune.apply(self, table)
+
+    def set_chol_shape(self, bucket: int, lanes: int, rows_per_lane: int) -> None:
+        """Exact Cholesky: lanes per matrix and rows per lane for a 16-row
+        bucket ((0, 0) = built-in)."""
+        _check(self.lib.mfh_set_chol_shape(self.h, int(bucket), int(lanes), int(rows_per_lane)))
 
     def bind_stream(self, stream: torch.cuda.Stream | None = None) -> None:
         s = stream if stream is not None else torch.cuda.current_stream()
