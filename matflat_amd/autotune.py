@@ -8,7 +8,8 @@ bucket and keeps the fastest:
   4) for each 8-row bucket (``mfh_set_fast_waves``);
 * exact Cholesky / SPD solve (``k_chol_packed``): the workgroup shape --
   lanes per matrix and rows per lane, i.e. how many matrices share a
-  wavefront -- for each 16-row bucket (``mfh_set_chol_shape``);
+  wavefront -- for each 16-row bucket (``mfh_set_chol_shape``), on orders
+  uniform over the bucket and on the bucket's smallest order;
 * the number of streams the FAST size buckets are forked over
   (``mfh_set_bucket_streams``), on the 100,000-matrix SPD-solve batch with
   n ~ U{10..100}; a bucket whose two best wavefront counts are within 5 %
@@ -152,6 +153,11 @@ def tune_fast_waves(h, count: int, steps: int, max_rb: int, log) -> tuple[dict, 
 
 
 def tune_chol_shapes(h, count: int, steps: int, log) -> tuple[dict, dict]:
+    """Each shape is timed on two batches of the bucket: orders uniform over
+    it, and every matrix at its smallest order (where a shape with more
+    lanes per matrix wastes the most).  The shape with the smallest sum of
+    times relative to the best on each batch wins, so neither case is traded
+    away for a small gain on the other."""
     from . import _lib, hip
     h.set_math_mode(hip.MATH_EXACT)
     best, times = {}, {}
@@ -161,17 +167,24 @@ def tune_chol_shapes(h, count: int, steps: int, log) -> tuple[dict, dict]:
         nmin, nmax = max(1, 16 * b), min(16 * (b + 1) - 1, _lib.SMALL_MAX_N)
         if len(shapes) < 2 or nmin > nmax:
             continue
-        batch = spd_batch(count, nmin, nmax, seed=100 + b)
+        rows = min(16 * (b + 1), nmax + 1)
         res = {}
-        for g, s in shapes:
-            h.set_chol_shape(b, g, s)
-            res[(g, s)] = _timed(h, batch, nmax, steps, _lib.KERNEL_CHOL_EXACT, min(16 * (b + 1), nmax + 1))
+        for tag, lo, hi, seed in (("uniform", nmin, nmax, 100 + b), ("smallest", nmin, nmin, 200 + b)):
+            batch = spd_batch(count, lo, hi, seed=seed)
+            for g, s in shapes:
+                h.set_chol_shape(b, g, s)
+                # the launch's row bound is that of the call's largest matrix
+                res[(tag, g, s)] = _timed(h, batch, hi, steps, _lib.KERNEL_CHOL_EXACT, min(16 * (b + 1), hi + 1))
+            del batch
         h.set_chol_shape(b, 0, 0)
-        best[b] = list(min(res, key=res.get))
-        times[b] = {f"{g}x{s}": round(t, 5) for (g, s), t in res.items()}
-        log(f"exact bucket {b} n={nmin:3d}..{nmax:3d} " + " ".join(f"{g}x{s}={t:.4f}ms" for (g, s), t in res.items())
+        score = {}
+        for g, s in shapes:
+            score[(g, s)] = sum(res[(tag, g, s)] / min(res[(tag, gg, ss)] for gg, ss in shapes)
+                                for tag in ("uniform", "smallest"))
+        best[b] = list(min(score, key=score.get))
+        times[b] = {f"{tag}:{g}x{s}": round(t, 5) for (tag, g, s), t in res.items()}
+        log(f"exact bucket {b} rows {rows} " + " ".join(f"{tag[0]}:{g}x{s}={t:.4f}" for (tag, g, s), t in res.items())
             + f" -> {best[b]}")
-        del batch
     return best, times
 
 

@@ -860,9 +860,9 @@ struct CholShape {
 CholShape chol_shape(int b)
 {
     switch (b) {
-    case 0: return {16, 2};
+    case 0: return {8, 2};
     case 1: return {16, 2};
-    case 2: return {32, 2};
+    case 2: return {16, 3};
     case 3: return {32, 2};
     case 8: return {64, 3};
     default: return {64, 2};

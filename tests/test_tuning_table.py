@@ -30,3 +30,24 @@ def test_shape_candidates_cover_their_bucket():
         shapes = autotune.chol_shapes_for(b)
         assert shapes, b
         assert all(g * s >= rows for g, s in shapes)
+
+
+def test_builtin_defaults_match_the_table():
+    """The library's compiled-in choices (used by callers that never load
+    the table) are the table's: exact-Cholesky shapes, FAST wavefronts and
+    the bucket-stream count."""
+    import re
+    t = json.load(open(os.path.join(ROOT, "matflat_amd", "tuning", "gfx950.json")))
+    src = open(os.path.join(ROOT, "matflat_amd", "csrc", "mfh_exact.hip")).read()
+    body = src[src.index("CholShape chol_shape(int b)"):]
+    body = body[:body.index("}\n}")]
+    cases = {int(a): (int(g), int(s)) for a, g, s in re.findall(r"case (\d+): return \{(\d+), (\d+)\}", body)}
+    default = tuple(int(v) for v in re.search(r"default: return \{(\d+), (\d+)\}", body).groups())
+    for b, (g, s) in t["chol_shape"].items():
+        assert cases.get(int(b), default) == (g, s), b
+    fast = open(os.path.join(ROOT, "matflat_amd", "csrc", "mfh_fast.hip")).read()
+    lo, hi = (int(v) for v in re.search(r"rb >= (\d+) && rb <= (\d+)\) \? 2 : 1", fast).groups())
+    for rb, w in t["fast_waves"].items():
+        assert w == (2 if lo <= int(rb) <= hi else 1), rb
+    internal = open(os.path.join(ROOT, "matflat_amd", "csrc", "mfh_internal.h")).read()
+    assert int(re.search(r"int bucket_streams = (\d+);", internal).group(1)) == t["bucket_streams"]
