@@ -117,6 +117,50 @@ __device__ __forceinline__ void mfh_pivot_combine(double& key, int& idx, double 
     }
 }
 
+// Pivot key of a candidate as an unsigned 64-bit integer: the bits of |w|
+// plus one (the bits of a non-negative double order like the values), 0 for
+// NaN and for lanes without a candidate.  A NaN then ranks below every
+// number, as in MatFlat's strict '>' scan; the w_k-is-NaN case is handled by
+// the caller.
+__device__ __forceinline__ unsigned long long mfh_pivot_key_u64(bool act, double w)
+{
+    const unsigned long long bits = (unsigned long long)__double_as_longlong(__builtin_fabs(w));
+    return (act && !__builtin_isnan(w)) ? bits + 1 : 0ull;
+}
+
+// One DPP step of a wave maximum of unsigned 64-bit values: lanes the
+// control or the row mask leave out read 0, the identity of the maximum.
+#define MFH_DPP_MAX_U64(v, CTRL, RMASK)                                                              \
+    do {                                                                                           \
+        const unsigned lo_ = __builtin_amdgcn_update_dpp(0u, (unsigned)(v), CTRL, RMASK, 0xF, false);   \
+        const unsigned hi_ = __builtin_amdgcn_update_dpp(0u, (unsigned)((v) >> 32), CTRL, RMASK, 0xF, false); \
+        const unsigned long long o_ = ((unsigned long long)hi_ << 32) | lo_;                          \
+        (v) = o_ > (v) ? o_ : (v);                                                                 \
+    } while (0)
+
+// Wave-wide maximum through DPP row shifts and row broadcasts (no LDS
+// traffic, unlike a shuffle); every lane receives it.
+__device__ __forceinline__ unsigned long long mfh_wave_max_u64(unsigned long long v)
+{
+    MFH_DPP_MAX_U64(v, 0x111, 0xF);  // row_shr:1
+    MFH_DPP_MAX_U64(v, 0x112, 0xF);  // row_shr:2
+    MFH_DPP_MAX_U64(v, 0x114, 0xF);  // row_shr:4
+    MFH_DPP_MAX_U64(v, 0x118, 0xF);  // row_shr:8  -> lane 15 of each row: the row's maximum
+    MFH_DPP_MAX_U64(v, 0x142, 0xA);  // row_bcast:15 into rows 1 and 3
+    MFH_DPP_MAX_U64(v, 0x143, 0xC);  // row_bcast:31 into rows 2 and 3 -> lane 63: the maximum
+    const unsigned lo = __builtin_amdgcn_readlane((unsigned)v, 63);
+    const unsigned hi = __builtin_amdgcn_readlane((unsigned)(v >> 32), 63);
+    return ((unsigned long long)hi << 32) | lo;
+}
+
+// The double lane `lane` (wave-uniform) holds.
+__device__ __forceinline__ double mfh_readlane_d(double v, int lane)
+{
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+    return __hiloint2double(hi, lo);
+}
+
 // Wave-wide (key, index) max with lowest-index tie break.
 __device__ __forceinline__ void mfh_wave_argmax(double& key, int& idx)
 {

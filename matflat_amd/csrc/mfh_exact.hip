@@ -343,6 +343,123 @@ __global__ __launch_bounds__(T) void k_getrf_exact(MfhGetrfBatch B, MfhQueue q)
     }
 }
 
+// Factorization.Lu.cs:296-350 for m, n <= 64 with two wavefronts per matrix,
+// pipelined: at step k wavefront 1 finishes row k-1 of U (U[k-1][j], j >= k,
+// the dots the reference forms when it reaches column j) while wavefront 0
+// forms the candidates of column k (rows i >= k).  A candidate's dot of
+// length k needs U[k-1][k] only in its last pair, (p_{k-2} + p_{k-1}):
+// everything before that pair is Internals.Dot of length k-2 (same parity,
+// same pairing) and runs concurrently with the row; after the barrier the
+// last pair is added, the pivot is found (a DPP wave maximum of the key's
+// ordered bits, then a ballot for the lowest row), the column is swapped and
+// divided in registers and the other columns' rows k and p are exchanged by
+// the same wavefront: two barriers per step.
+// Every element is the same sequence of operations as in k_getrf_exact, so
+// the factors, pivots and sign are bit-identical; one dot-product latency
+// per step instead of two.
+__global__ __launch_bounds__(128) void k_getrf_exact_w2(MfhGetrfBatch B, MfhQueue q)
+{
+    extern __shared__ double lds[];
+    const int tid = threadIdx.x;
+    const int lane = tid & (MFH_WAVE - 1);
+    const bool w0 = __builtin_amdgcn_readfirstlane(tid >> 6) == 0;
+    for (int it = blockIdx.x;; it += gridDim.x) {
+        const int b = mfh_item(q, it);
+        if (b < 0) break;
+        const int m = B.m ? B.m[b] : B.m0;
+        const int n = B.n ? B.n[b] : B.n0;
+        const int lda = B.lda ? B.lda[b] : B.lda0;
+        double* A = B.a ? B.a[b] : B.a0 + (size_t)b * B.stride_a;
+        int* P = B.piv ? B.piv[b] : B.piv0 + (size_t)b * B.stride_piv;
+        const int ml = m | 1;  // odd leading dimension: row-wise reads in distinct banks
+        double* L = lds;
+        int* pv = (int*)(lds + ml * n);
+        stage_block<128>(L, ml, A, lda, m, n, tid);
+        for (int i = tid; i < m; i += 128) pv[i] = i;
+        __syncthreads();
+        int sign = 1;
+        const int kmax = m < n ? m : n;
+        for (int k = 0; k < kmax; ++k) {
+            const int i = k + lane;
+            const bool act = i < m;
+            double sp = 0.0, p2 = 0.0;
+            if (w0) {
+                // all of the candidate's dot but its last pair
+                if (act && k >= 2) {
+                    sp = mfh_dot_exact(k - 2, L + i, ml, L + k * ml, 1);
+                    p2 = L[i + (k - 2) * ml] * L[(k - 2) + k * ml];
+                }
+            } else {
+                const int j = k + lane;
+                if (k >= 1 && j < n)
+                    L[(k - 1) + j * ml] = L[(k - 1) + j * ml] - mfh_dot_exact(k - 1, L + (k - 1), ml, L + j * ml, 1);
+            }
+            __syncthreads();
+            if (w0) {
+                double w = 0.0;
+                if (act) {
+                    double s;
+                    if (k >= 2) {
+                        s = sp + (p2 + L[i + (k - 1) * ml] * L[(k - 1) + k * ml]);
+                    } else if (k == 1) {
+                        s = L[i] * L[ml];  // Internals.Dot of length 1: x0 * y0
+                    } else {
+                        s = 0.0;
+                    }
+                    w = L[i + k * ml] - s;
+                }
+                // the largest key, then the lowest lane holding it (the
+                // lowest row: the strict '>' scan's choice)
+                const unsigned long long key = mfh_pivot_key_u64(act, w);
+                const unsigned long long mk = mfh_wave_max_u64(key);
+                const unsigned long long hit = __ballot(key == mk);
+                const double wk = mfh_readlane_d(w, 0);  // lane 0 holds row k
+                const int p = __builtin_isnan(wk) ? k : k + __ffsll((long long)hit) - 1;
+                const double wp = mfh_readlane_d(w, p - k);
+                // column k after the exchange of rows k and p, then the
+                // multipliers (true division, skipped for a zero pivot)
+                double v = (i == p) ? wk : ((i == k) ? wp : w);
+                if (act && i > k && wp != 0.0) v = v / wp;
+                if (act) L[i + k * ml] = v;
+                // the exchange of rows k and p in the other columns, one
+                // column per lane, while wavefront 1 waits at the barrier
+                if (p != k) {
+                    for (int j = lane; j < n; j += MFH_WAVE) {
+                        if (j == k) continue;
+                        const double t = L[k + j * ml];
+                        L[k + j * ml] = L[p + j * ml];
+                        L[p + j * ml] = t;
+                    }
+                    if (lane == 0) {
+                        const int t = pv[k];
+                        pv[k] = pv[p];
+                        pv[p] = t;
+                    }
+                    sign = -sign;  // only thread 0's copy is written out
+                }
+            }
+            __syncthreads();
+        }
+        // the last row of U (columns beyond the last step, n > m)
+        if (!w0) {
+            const int j = kmax + lane;
+            if (j < n) L[(kmax - 1) + j * ml] = L[(kmax - 1) + j * ml] - mfh_dot_exact(kmax - 1, L + (kmax - 1), ml, L + j * ml, 1);
+        }
+        __syncthreads();
+        for (int e = tid; e < m * n; e += 128) {
+            const int j = e / m;
+            const int ii = e - j * m;
+            A[ii + (size_t)j * lda] = L[ii + j * ml];
+        }
+        for (int ii = tid; ii < m; ii += 128) P[ii] = pv[ii];
+        if (tid == 0) {
+            B.sign[b] = sign;
+            B.info[b] = 0;
+        }
+        __syncthreads();
+    }
+}
+
 // ------------------------------------------------------------ Cholesky / posv
 struct CholKey {
     MfhCholBatch B;
@@ -915,8 +1032,8 @@ int mfh_launch_getrf(mfh_handle* h, int count, const MfhGetrfBatch& B, int max_m
         // one shape, checked on the host: no sort, one launch over the batch
         const MfhQueue q{nullptr, nullptr, nullptr, count};
         const size_t lds = square_lds_bytes(top);
-        if (threads_for(top) == 64)
-            return launch_bucket(h, MFH_KERNEL_GETRF_EXACT, top, k_getrf_exact<64>, 64, lds, count, B, q);
+        if (top <= MFH_WAVE)
+            return launch_bucket(h, MFH_KERNEL_GETRF_EXACT, top, k_getrf_exact_w2, 128, lds, count, B, q);
         return launch_bucket(h, MFH_KERNEL_GETRF_EXACT, top, k_getrf_exact<128>, 128, lds, count, B, q);
     }
     SortWs ws;
@@ -928,7 +1045,7 @@ int mfh_launch_getrf(mfh_handle* h, int count, const MfhGetrfBatch& B, int max_m
         const int t = threads_for(nb);
         const size_t lds = square_lds_bytes(nb);
         const MfhQueue q = bucket_queue(ws, b);
-        if (t == 64) st = launch_bucket(h, MFH_KERNEL_GETRF_EXACT, nb, k_getrf_exact<64>, 64, lds, count, B, q);
+        if (t == 64) st = launch_bucket(h, MFH_KERNEL_GETRF_EXACT, nb, k_getrf_exact_w2, 128, lds, count, B, q);
         else st = launch_bucket(h, MFH_KERNEL_GETRF_EXACT, nb, k_getrf_exact<128>, 128, lds, count, B, q);
         if (st) return st;
     }
