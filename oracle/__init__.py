@@ -19,6 +19,9 @@ CPU fallback.
                               Factorization.Cholesky.cs:195-249,
                               Blas.SolveTriangular.cs:280-445, with BCL
                               Complex division (Smith's method)
+* ``Factorization.Gevd``      MatFlat/Factorization.Gevd.cs:330-376 with
+                              SvdCore (Factorization.Svd.cs:786-1256),
+                              Internals.Norm and Internals.Drotg
 
 compiled with ``-O2 -ffp-contract=off -fno-fast-math``.  Pinning (what makes
 these functions trustworthy, beyond being written from the reference) lives in
@@ -107,6 +110,16 @@ def lib() -> ctypes.CDLL:
         _lib.mfo_ztrsv.argtypes = [i, i, i, p, i, p, i]
         _lib.mfo_dposv_backward.restype = None
         _lib.mfo_dposv_backward.argtypes = [i, p, i, p, p, p, i, p]
+        _lib.mfo_dnorm.restype = d
+        _lib.mfo_dnorm.argtypes = [i, p]
+        _lib.mfo_drotg.restype = None
+        _lib.mfo_drotg.argtypes = [p, p, p, p]
+        _lib.mfo_dsvd_u.restype = i
+        _lib.mfo_dsvd_u.argtypes = [i, p, i, p, p, i, p, p, p]
+        _lib.mfo_dgevd.restype = i
+        _lib.mfo_dgevd.argtypes = [i, p, i, p, i, p, p, p]
+        _lib.mfo_dgevd_batch.restype = None
+        _lib.mfo_dgevd_batch.argtypes = [i, p, p, p, p, p, p, p, p, p, p, i]
     return _lib
 
 
@@ -282,3 +295,43 @@ def dposv_backward(n, l, lda, x, w, ldab=None):
     w = np.ascontiguousarray(w, dtype=np.float64)
     lib().mfo_dposv_backward(n, _ptr(l), lda, _ptr(x), _ptr(w), _ptr(abar), ldab, _ptr(bbar))
     return abar, bbar
+
+
+def dnorm(n: int, x) -> float:
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    return float(lib().mfo_dnorm(n, _ptr(x)))
+
+
+def drotg(a: float, b: float):
+    """Internals.Drotg: returns (r, z, c, s)."""
+    v = np.array([a, b, 0.0, 0.0])
+    base = v.ctypes.data
+    lib().mfo_drotg(base, base + 8, base + 16, base + 24)
+    return float(v[0]), float(v[1]), float(v[2]), float(v[3])
+
+
+def dsvd_u(n: int, a: np.ndarray, lda: int, u: np.ndarray, ldu: int):
+    """SvdCore of the square matrix in ``a`` (destroyed) with left vectors
+    into ``u``; returns (info, s)."""
+    s = np.zeros(n)
+    work = np.zeros(3 * n)
+    base = work.ctypes.data
+    info = lib().mfo_dsvd_u(n, _ptr(a), lda, _ptr(s), _ptr(u), ldu, base, base + 8 * n, base + 16 * n)
+    return info, s
+
+
+def dgevd(n: int, a: np.ndarray, lda: int, b: np.ndarray, ldb: int, w: np.ndarray) -> int:
+    """Factorization.Gevd in place: a -> eigenvectors, b -> L, w -> eigenvalues.
+    Returns 0, j+1 (B not positive definite at row j) or n+1 (no convergence)."""
+    assert a.dtype == np.float64 and b.dtype == np.float64 and w.dtype == np.float64
+    c = np.zeros(n * n + 3 * n)
+    return lib().mfo_dgevd(n, _ptr(a), lda, _ptr(b), ldb, _ptr(w), _ptr(c), c.ctypes.data + 8 * n * n)
+
+
+def dgevd_batch(n, lda, ldb, aoff, boff, woff, a, b, w, threads=1) -> np.ndarray:
+    n, lda, ldb = (np.ascontiguousarray(v, dtype=np.int32) for v in (n, lda, ldb))
+    aoff, boff, woff = (np.ascontiguousarray(v, dtype=np.int64) for v in (aoff, boff, woff))
+    info = np.zeros(n.size, dtype=np.int32)
+    lib().mfo_dgevd_batch(n.size, _ptr(n), _ptr(lda), _ptr(ldb), _ptr(aoff), _ptr(boff), _ptr(woff), _ptr(a),
+                          _ptr(b), _ptr(w), _ptr(info), threads)
+    return info

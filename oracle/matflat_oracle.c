@@ -661,3 +661,343 @@ int mfo_ztrsv(int uplo, int trans, int n, const double* a, int lda, double* x, i
     }
     return 0;
 }
+
+/* ---- generalized symmetric-definite eigenproblem ------------------------ */
+
+/* Internals.Norm(int n, double* x), Internals.cs:435-460: odd n starts with
+ * x0*x0, even n with 0.0, then sum + (x_k*x_k + x_{k+1}*x_{k+1}); sqrt. */
+double mfo_dnorm(int n, const double* x)
+{
+    double s;
+    int c;
+    if (n & 1) {
+        s = x[0] * x[0];
+        c = 1;
+    } else {
+        s = 0.0;
+        c = 0;
+    }
+    for (; c < n; c += 2) s = s + (x[c] * x[c] + x[c + 1] * x[c + 1]);
+    return sqrt(s);
+}
+
+/* Internals.MulAdd (Internals.cs:49-73): dst[i] += x[i]*y, element-wise. */
+static void muladd(int n, const double* x, double y, double* dst)
+{
+    for (int i = 0; i < n; i++) dst[i] = dst[i] + x[i] * y;
+}
+
+/* Math.Max(double, double) of .NET: NaN if either argument is NaN. */
+static double net_max(double a, double b)
+{
+    if (a != b) {
+        if (!isnan(a)) return b < a ? a : b;
+        return a;
+    }
+    return signbit(b) ? a : b;
+}
+
+/* Internals.Drotg (Internals.cs:626-668), the reference BLAS drotg. */
+void mfo_drotg(double* da, double* db, double* c, double* s)
+{
+    double r, z;
+    double roe = *db;
+    double ada = fabs(*da);
+    double adb = fabs(*db);
+    if (ada > adb) roe = *da;
+    double scale = ada + adb;
+    if (scale == 0.0) {
+        *c = 1.0;
+        *s = 0.0;
+        r = 0.0;
+        z = 0.0;
+    } else {
+        double sda = *da / scale;
+        double sdb = *db / scale;
+        r = scale * sqrt(sda * sda + sdb * sdb);
+        if (roe < 0.0) r = -r;
+        *c = *da / r;
+        *s = *db / r;
+        z = 1.0;
+        if (ada > adb) z = *s;
+        if (adb >= ada && *c != 0.0) z = 1.0 / *c;
+    }
+    *da = r;
+    *db = z;
+}
+
+/* Passes of the singular-value loop after which a block is reported as not
+ * converged: 1000 QR steps (the reference's limit) plus, around each of
+ * them, the at most n+1 deflation passes that can separate two steps.  A
+ * finite matrix never reaches it; the reference loops without end on some
+ * NaN patterns, which end here instead. */
+static long svd_pass_cap(int n) { return 1001L * (n + 2) + 1; }
+
+/* Factorization.Svd.cs:786-1256 (SvdCore double, LINPACK dsvdc order) for a
+ * square n x n matrix with the left vectors only (vt == null, as
+ * Factorization.Gevd.cs:366 calls it): a is destroyed, s gets the singular
+ * values in descending order, u (ldu) the left singular vectors.  work, e
+ * and stmp hold n doubles each.  Returns 0, or 1 when the iteration did not
+ * converge (the reference throws MatrixFactorizationException; u then holds
+ * the vectors reached so far and s is not written). */
+int mfo_dsvd_u(int n, double* a, int lda, double* s, double* u, int ldu, double* work, double* e,
+               double* stmp)
+{
+    const int m = n;
+    const int nct = m - 1 < n ? m - 1 : n;
+    int nrt = n - 2 < m ? n - 2 : m;
+    if (nrt < 0) nrt = 0;
+    const int kmax = nct > nrt ? nct : nrt;
+    for (int k = 0; k < kmax; k++) {
+        const int kp1 = k + 1;
+        double* ak = a + (size_t)lda * k;
+        if (k < nct) {
+            stmp[k] = mfo_dnorm(m - k, ak + k);
+            if (stmp[k] != 0.0) {
+                if (ak[k] < 0.0) stmp[k] = -stmp[k];
+                for (int i = k; i < m; i++) ak[i] = ak[i] / stmp[k];
+                ak[k] = ak[k] + 1.0;
+            }
+            stmp[k] = -stmp[k];
+        }
+        for (int j = kp1; j < n; j++) {
+            double* aj = a + (size_t)lda * j;
+            if (k < nct && stmp[k] != 0.0) {
+                double t = -mfo_ddot(m - k, ak + k, 1, aj + k, 1) / ak[k];
+                muladd(m - k, ak + k, t, aj + k);
+            }
+            e[j] = aj[k];
+        }
+        if (k < nct) memcpy(u + (size_t)ldu * k + k, ak + k, sizeof(double) * (m - k));
+        if (k < nrt) {
+            e[k] = mfo_dnorm(n - kp1, e + kp1);
+            if (e[k] != 0.0) {
+                if (e[kp1] < 0.0) e[k] = -e[k];
+                for (int i = kp1; i < n; i++) e[i] = e[i] / e[k];
+                e[kp1] = e[kp1] + 1.0;
+            }
+            e[k] = -e[k];
+            if (kp1 < m && e[k] != 0.0) {
+                for (int i = kp1; i < m; i++) work[i] = 0.0;
+                for (int j = kp1; j < n; j++) muladd(m - kp1, a + (size_t)lda * j + kp1, e[j], work + kp1);
+                for (int j = kp1; j < n; j++) muladd(m - kp1, work + kp1, -e[j] / e[kp1], a + (size_t)lda * j + kp1);
+            }
+        }
+    }
+
+    int p = n < m + 1 ? n : m + 1;
+    if (nct < n) stmp[nct] = a[(size_t)nct * lda + nct];
+    if (m < p) stmp[p - 1] = 0.0;
+    if (nrt + 1 < p) e[nrt] = a[(size_t)(p - 1) * lda + nrt];
+    e[p - 1] = 0.0;
+
+    for (int j = nct; j < m; j++) {
+        double* uj = u + (size_t)ldu * j;
+        for (int i = 0; i < m; i++) uj[i] = 0.0;
+        uj[j] = 1.0;
+    }
+    for (int k = nct - 1; k >= 0; k--) {
+        double* uk = u + (size_t)ldu * k;
+        if (stmp[k] != 0.0) {
+            for (int j = k + 1; j < m; j++) {
+                double* uj = u + (size_t)ldu * j;
+                double t = -mfo_ddot(m - k, uk + k, 1, uj + k, 1) / uk[k];
+                muladd(m - k, uk + k, t, uj + k);
+            }
+            for (int i = k; i < m; i++) uk[i] = -uk[i];
+            uk[k] = uk[k] + 1.0;
+            for (int i = 0; i < k; i++) uk[i] = 0.0;
+        } else {
+            for (int i = 0; i < m; i++) uk[i] = 0.0;
+            uk[k] = 1.0;
+        }
+    }
+
+    const int pp = p - 1;
+    int iter = 0;
+    long passes = 0;
+    const double eps = 0x1p-52;
+    while (p > 0) {
+        if (iter >= 1000 || ++passes > svd_pass_cap(n)) return 1;
+        int k;
+        for (k = p - 2; k >= 0; k--) {
+            if (fabs(e[k]) <= eps * (fabs(stmp[k]) + fabs(stmp[k + 1]))) {
+                e[k] = 0.0;
+                break;
+            }
+        }
+        int kase;
+        if (k == p - 2) {
+            kase = 4;
+        } else {
+            int ks;
+            for (ks = p - 1; ks > k; ks--) {
+                double t = 0.0;
+                if (ks != p - 1) t = t + fabs(e[ks]);
+                if (ks != k + 1) t = t + fabs(e[ks - 1]);
+                if (fabs(stmp[ks]) <= eps * t) {
+                    stmp[ks] = 0.0;
+                    break;
+                }
+            }
+            if (ks == k) {
+                kase = 3;
+            } else if (ks == p - 1) {
+                kase = 1;
+            } else {
+                kase = 2;
+                k = ks;
+            }
+        }
+        k++;
+
+        double f, cs, sn;
+        if (kase == 1) {
+            f = e[p - 2];
+            e[p - 2] = 0.0;
+            for (int j = k; j < p - 1; j++) {
+                int l = p - 2 - j + k;
+                double t = stmp[l];
+                mfo_drotg(&t, &f, &cs, &sn);
+                stmp[l] = t;
+                if (l != k) {
+                    f = -sn * e[l - 1];
+                    e[l - 1] = e[l - 1] * cs;
+                }
+            }
+        } else if (kase == 2) {
+            f = e[k - 1];
+            e[k - 1] = 0.0;
+            for (int j = k; j < p; j++) {
+                double t = stmp[j];
+                mfo_drotg(&t, &f, &cs, &sn);
+                stmp[j] = t;
+                f = -sn * e[j];
+                e[j] = cs * e[j];
+                double* uj = u + (size_t)ldu * j;
+                double* ukm1 = u + (size_t)ldu * (k - 1);
+                for (int i = 0; i < m; i++) {
+                    double z = cs * uj[i] + sn * ukm1[i];
+                    ukm1[i] = cs * ukm1[i] - sn * uj[i];
+                    uj[i] = z;
+                }
+            }
+        } else if (kase == 3) {
+            double scale = 0.0;
+            scale = net_max(scale, fabs(stmp[p - 1]));
+            scale = net_max(scale, fabs(stmp[p - 2]));
+            scale = net_max(scale, fabs(e[p - 2]));
+            scale = net_max(scale, fabs(stmp[k]));
+            scale = net_max(scale, fabs(e[k]));
+            double sp = stmp[p - 1] / scale;
+            double spm1 = stmp[p - 2] / scale;
+            double epm1 = e[p - 2] / scale;
+            double sk = stmp[k] / scale;
+            double ek = e[k] / scale;
+            double b = ((spm1 + sp) * (spm1 - sp) + epm1 * epm1) / 2.0;
+            double c = (sp * epm1) * (sp * epm1);
+            double shift = 0.0;
+            if (b != 0.0 || c != 0.0) {
+                shift = sqrt(b * b + c);
+                if (b < 0.0) shift = -shift;
+                shift = c / (b + shift);
+            }
+            f = (sk + sp) * (sk - sp) + shift;
+            double g = sk * ek;
+            for (int j = k; j < p - 1; j++) {
+                mfo_drotg(&f, &g, &cs, &sn);
+                if (j != k) e[j - 1] = f;
+                f = cs * stmp[j] + sn * e[j];
+                e[j] = cs * e[j] - sn * stmp[j];
+                g = sn * stmp[j + 1];
+                stmp[j + 1] = stmp[j + 1] * cs;
+                mfo_drotg(&f, &g, &cs, &sn);
+                stmp[j] = f;
+                f = cs * e[j] + sn * stmp[j + 1];
+                stmp[j + 1] = -(sn * e[j]) + cs * stmp[j + 1];
+                g = sn * e[j + 1];
+                e[j + 1] = e[j + 1] * cs;
+                if (j < m) {
+                    double* uj = u + (size_t)ldu * j;
+                    double* uj1 = u + (size_t)ldu * (j + 1);
+                    for (int i = 0; i < m; i++) {
+                        double z = cs * uj[i] + sn * uj1[i];
+                        uj1[i] = cs * uj1[i] - sn * uj[i];
+                        uj[i] = z;
+                    }
+                }
+            }
+            e[p - 2] = f;
+            iter++;
+        } else {
+            if (stmp[k] < 0.0) stmp[k] = -stmp[k];
+            while (k < pp) {
+                if (stmp[k] >= stmp[k + 1]) break;
+                double t = stmp[k];
+                stmp[k] = stmp[k + 1];
+                stmp[k + 1] = t;
+                if (k < m) {
+                    double* uk = u + (size_t)ldu * k;
+                    double* uk1 = u + (size_t)ldu * (k + 1);
+                    for (int i = 0; i < m; i++) {
+                        double z = uk[i];
+                        uk[i] = uk1[i];
+                        uk1[i] = z;
+                    }
+                }
+                k++;
+            }
+            iter = 0;
+            passes = 0;
+            p--;
+        }
+    }
+    memcpy(s, stmp, sizeof(double) * n);
+    return 0;
+}
+
+/* Factorization.Gevd double (Factorization.Gevd.cs:145-186, GevdCore
+ * :330-376): B = L L^T in place (b), C = L^-1 A L^-T from A's upper
+ * triangle by Lower/NoTrans solves of every row, then of every column,
+ * C = U S V^T, w = S, and the eigenvectors L^-T U overwrite a.  c holds
+ * n*n doubles and work 3n.  Returns 0; j+1 when B's Cholesky fails at row j
+ * (the reference throws MatFlatException; b then holds the partial writes
+ * and a is untouched); n+1 when the SVD did not converge (a holds U so far,
+ * w is not written). */
+int mfo_dgevd(int n, double* a, int lda, double* b, int ldb, double* w, double* c, double* work)
+{
+    int info = mfo_dpotrf(n, b, ldb);
+    if (info != 0) return info;
+    for (int j = 0; j < n; j++) {
+        const double* aj = a + (size_t)lda * j;
+        for (int i = 0; i < j; i++) {
+            c[(size_t)n * j + i] = aj[i];
+            c[(size_t)n * i + j] = aj[i];
+        }
+        c[(size_t)n * j + j] = aj[j];
+    }
+    for (int i = 0; i < n; i++) mfo_dtrsv(MFO_LOWER, MFO_NOTRANS, n, b, ldb, c + i, n);
+    for (int i = 0; i < n; i++) mfo_dtrsv(MFO_LOWER, MFO_NOTRANS, n, b, ldb, c + (size_t)n * i, 1);
+    if (mfo_dsvd_u(n, c, n, w, a, lda, work, work + n, work + 2 * n) != 0) return n + 1;
+    for (int i = 0; i < n; i++) mfo_dtrsv(MFO_LOWER, MFO_TRANS, n, b, ldb, a + (size_t)lda * i, 1);
+    return 0;
+}
+
+/* mfo_dgevd over a batch stored back-to-back (offsets in doubles). */
+void mfo_dgevd_batch(int count, const int* n, const int* lda, const int* ldb, const int64_t* aoff,
+                     const int64_t* boff, const int64_t* woff, double* a, double* b, double* w,
+                     int* info, int threads)
+{
+#ifdef _OPENMP
+    if (threads < 1) threads = 1;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads)
+#endif
+    for (int i = 0; i < count; i++) {
+        double* c = (double*)malloc(sizeof(double) * ((size_t)n[i] * n[i] + 3 * (size_t)n[i]));
+        info[i] = c ? mfo_dgevd(n[i], a + aoff[i], lda[i], b + boff[i], ldb[i], w + woff[i], c,
+                                c + (size_t)n[i] * n[i])
+                    : -1;
+        free(c);
+    }
+    (void)threads;
+}

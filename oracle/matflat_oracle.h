@@ -99,6 +99,28 @@ int mfo_zpotrf(int n, double* a, int lda);
  * uplo / transpose (NoTrans, Trans, ConjTrans, ConjNoTrans). */
 int mfo_ztrsv(int uplo, int trans, int n, const double* a, int lda, double* x, int incx);
 
+/* ---- generalized symmetric-definite eigenproblem -------------------------- */
+
+/* Internals.Norm(int n, double* x), Internals.cs:435-460. */
+double mfo_dnorm(int n, const double* x);
+
+/* Internals.Drotg, Internals.cs:626-668 (reference BLAS drotg). */
+void mfo_drotg(double* da, double* db, double* c, double* s);
+
+/* SvdCore double (Factorization.Svd.cs:786-1256) for a square matrix with
+ * left vectors only: s descending, u (ldu) the left singular vectors; a is
+ * destroyed; work, e, stmp hold n doubles each.  Returns 0, or 1 when the
+ * iteration did not converge (the reference throws). */
+int mfo_dsvd_u(int n, double* a, int lda, double* s, double* u, int ldu, double* work, double* e,
+               double* stmp);
+
+/* Factorization.Gevd double, Factorization.Gevd.cs:145-186 / :330-376: A v = w B v
+ * for symmetric A (upper triangle read) and SPD B (upper triangle read).  b
+ * gets L (B = L L^T), w the eigenvalues (descending), a the eigenvectors
+ * (L^-T U).  c holds n*n doubles, work 3n.  Returns 0, j+1 if B's Cholesky
+ * fails at row j, n+1 if the SVD did not converge. */
+int mfo_dgevd(int n, double* a, int lda, double* b, int ldb, double* w, double* c, double* work);
+
 /* ---- batched drivers (CPU baseline; OpenMP over matrices) ---------------- */
 
 /* Cholesky + two solves over a batch stored back-to-back: matrix i at
@@ -124,6 +146,12 @@ void mfo_gemm_vbatch(int cplx, int count, const int* ta, const int* tb, const in
                      const int* lda, const double* b, const int64_t* offb, const int* ldb,
                      const double* beta, double* c, const int64_t* offc, const int* ldc, int* info,
                      int threads);
+
+/* mfo_dgevd over a batch stored back-to-back: A, B and w of matrix i at
+ * a + aoff[i], b + boff[i], w + woff[i]; info[i] = its return value. */
+void mfo_dgevd_batch(int count, const int* n, const int* lda, const int* ldb, const int64_t* aoff,
+                     const int64_t* boff, const int64_t* woff, double* a, double* b, double* w,
+                     int* info, int threads);
 
 /* mfo_dposv_backward over a fixed-size batch (L and Abar stride elements
  * apart with leading dimension lda; x, w and bbar n apart). */
