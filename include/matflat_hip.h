@@ -141,6 +141,7 @@ int mfh_get_bucket_streams(mfh_handle* handle, int* streams);
 #define MFH_KERNEL_POTRF_GEN 14
 #define MFH_KERNEL_TRSV_GEN 15
 #define MFH_KERNEL_GETRS_PERMUTE 16
+#define MFH_KERNEL_GEVD_EXACT 17
 int mfh_set_timing(mfh_handle* handle, int enable);
 int mfh_get_timing(mfh_handle* handle, int capacity, int* n_records, int* kind, int* rows,
                    float* ms);
@@ -342,6 +343,22 @@ int mfh_dposv_backward_vbatched(mfh_handle* h, int count, const int* n, const do
                                 const int* lda, const double* const* x, const double* const* xbar,
                                 double* const* abar, const int* ldabar, double* const* bbar,
                                 int* info, int max_n);
+
+/* ---- generalized symmetric-definite eigenproblem ---------------------------
+ * Factorization.Gevd (double), Factorization.Gevd.cs:145-186 with GevdCore
+ * :330-376: A v = w B v for symmetric A and symmetric positive definite B,
+ * both read from their upper triangles only.  In MatFlat's operation order
+ * (bit-identical): b is overwritten by the Cholesky factor L (B = L L^T,
+ * strict upper triangle zeroed), w by the n eigenvalues in descending order
+ * and a by the eigenvectors (columns, B-orthonormal).  info[b] = 0; j+1 when
+ * B is not positive definite at row j (MatFlat throws MatFlatException; b
+ * holds the Cholesky's partial writes, a and w are untouched); n+1 when the
+ * singular-value iteration does not converge (MatFlat throws
+ * MatrixFactorizationException; a holds the vectors reached, w untouched);
+ * -(position) for invalid per-matrix arguments (nothing written).  Orders up
+ * to MFH_SMALL_MAX_N; the math mode does not apply (always MatFlat's order). */
+int mfh_dgevd_vbatched(mfh_handle* h, int count, const int* n, double* const* a, const int* lda,
+                       double* const* b, const int* ldb, double* const* w, int* info, int max_n);
 
 /* ---- plans: one batch captured into a hipGraph ------------------------------
  * A plan records one call of an entry point with the given arguments into a

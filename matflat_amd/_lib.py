@@ -32,10 +32,11 @@ KERNEL_SORT, KERNEL_CHOL_EXACT, KERNEL_CHOL_FAST, KERNEL_GETRF_EXACT, KERNEL_TRS
     1, 2, 3, 4, 5, 6
 KERNEL_GESV_MIXED, KERNEL_GESV_F64, KERNEL_GESV_STEP, KERNEL_GEMV_EXACT, KERNEL_GETRI_SETUP = 7, 8, 9, 10, 11
 KERNEL_GETRI_EXACT, KERNEL_GETRF_GEN, KERNEL_POTRF_GEN, KERNEL_TRSV_GEN, KERNEL_GETRS_PERMUTE = 12, 13, 14, 15, 16
+KERNEL_GEVD_EXACT = 17
 KERNEL_NAMES = {1: "size sort", 2: "k_chol_packed", 3: "k_chol_fast", 4: "k_getrf_exact", 5: "k_trsv_exact",
                 6: "k_getrs_exact", 7: "k_gesv<float, mixed>", 8: "k_gesv<double>", 9: "k_gesv_step",
                 10: "k_dgemv_exact", 11: "k_getri_setup", 12: "k_getri_exact", 13: "k_getrf_gen",
-                14: "k_potrf_gen", 15: "k_trsv_gen", 16: "k_getrs_permute"}
+                14: "k_potrf_gen", 15: "k_trsv_gen", 16: "k_getrs_permute", 17: "k_gevd_exact"}
 
 _i, _d, _p, _i64 = ctypes.c_int, ctypes.c_double, ctypes.c_void_p, ctypes.c_int64
 
@@ -74,6 +75,7 @@ SIGNATURES = {
     "mfh_dgesv_vbatched": [_p, _i, _p, _p, _p, _p, _p, _p, _p, _i],
     "mfh_dgetrf_blocked_vbatched": [_p, _i, _p, _p, _p, _p, _p, _i],
     "mfh_dposv_backward_vbatched": [_p, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i],
+    "mfh_dgevd_vbatched": [_p, _i, _p, _p, _p, _p, _p, _p, _p, _i],
     "mfh_plan_create_dposv": [_p, _i, _p, _p, _p, _p, _p, _i, _p],
     "mfh_plan_create_dgemm": [_p, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i, _i, _i, _p],
     "mfh_plan_create_zgemm": [_p, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i, _i, _i, _p],

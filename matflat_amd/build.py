@@ -22,7 +22,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmatflat_hip.so")
 OBJDIR = os.path.join(HERE, "_build")
-SOURCES = ["mfh_api.cpp", "mfh_exact.hip", "mfh_fast.hip", "mfh_fast_posv.hip", "mfh_gemm.hip", "mfh_gemv.hip", "mfh_adjoint.hip", "mfh_mixed.hip", "mfh_general.hip"]
+SOURCES = ["mfh_api.cpp", "mfh_exact.hip", "mfh_fast.hip", "mfh_fast_posv.hip", "mfh_gemm.hip", "mfh_gemv.hip", "mfh_adjoint.hip", "mfh_mixed.hip", "mfh_general.hip", "mfh_gevd.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [

@@ -659,6 +659,23 @@ int mfh_dposv_backward_vbatched(mfh_handle* h, int count, const int* n, const do
     return mfh_launch_posv_backward(h, count, B, max_n);
 }
 
+int mfh_dgevd_vbatched(mfh_handle* h, int count, const int* n, double* const* a, const int* lda, double* const* b,
+                       const int* ldb, double* const* w, int* info, int max_n)
+{
+    MFH_REQUIRE(h != nullptr, 1, "handle must not be null");
+    MFH_REQUIRE(count >= 0, 2, "count must be >= 0");
+    if (count == 0) return MFH_OK;
+    MFH_REQUIRE(n != nullptr, 3, "n must not be null");
+    MFH_REQUIRE(a != nullptr, 4, "a must not be null");
+    MFH_REQUIRE(lda != nullptr, 5, "lda must not be null");
+    MFH_REQUIRE(b != nullptr, 6, "b must not be null");
+    MFH_REQUIRE(ldb != nullptr, 7, "ldb must not be null");
+    MFH_REQUIRE(w != nullptr, 8, "w must not be null");
+    MFH_REQUIRE(info != nullptr, 9, "info must not be null");
+    MFH_REQUIRE(max_n >= 1 && max_n <= MFH_SMALL_MAX_N, 10, "max_n must be in [1, MFH_SMALL_MAX_N]");
+    return mfh_launch_gevd(h, count, n, a, lda, b, ldb, w, info, max_n);
+}
+
 // ---------------------------------------------------------------- plans
 }  // extern "C"
 

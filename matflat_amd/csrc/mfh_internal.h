@@ -208,6 +208,10 @@ int mfh_launch_gemm(mfh_handle* h, int count, const MfhGemmBatch& B, int max_m, 
 int mfh_launch_getri(mfh_handle* h, int count, const MfhGetriBatch& B, int max_n);
 int mfh_launch_gemv(mfh_handle* h, int count, const MfhGemvBatch& B, int max_m, int max_n);
 int mfh_launch_posv_backward(mfh_handle* h, int count, const MfhAdjointBatch& B, int max_n);
+// Factorization.Gevd over a batch (mfh_gevd.hip): argument check, the exact
+// Cholesky of B, then one wavefront per matrix.  Uses h->aux (2 count ints).
+int mfh_launch_gevd(mfh_handle* h, int count, const int* n, double* const* a, const int* lda, double* const* b,
+                    const int* ldb, double* const* w, int* info, int max_n);
 int mfh_launch_gesv_mixed(mfh_handle* h, int count, const int* n, const double* const* a,
                           const int* lda, const double* const* b, double* const* x, int max_iter,
                           double tol_per_n, int* iters, double* berr, int* info, int max_n,

@@ -294,6 +294,14 @@ def dposv_backward_vbatched(n, l_ptrs, lda, x_ptrs, xbar_ptrs, abar_ptrs, ldabar
                                              int(max_n)))
 
 
+def dgevd_vbatched(n, a_ptrs, lda, b_ptrs, ldb, w_ptrs, info, max_n, handle: Handle | None = None):
+    """Factorization.Gevd (double) over a batch: a -> eigenvectors, b -> L,
+    w -> eigenvalues (descending), in MatFlat's operation order."""
+    h = handle or default_handle()
+    _check(h.lib.mfh_dgevd_vbatched(h.h, int(n.numel()), _p(n), _p(a_ptrs), _p(lda), _p(b_ptrs), _p(ldb),
+                                    _p(w_ptrs), _p(info), int(max_n)))
+
+
 class Plan:
     """One batched call captured into a hipGraph (mfh_plan_create_*) and
     replayed by execute() on the handle's stream.  The device arrays the call

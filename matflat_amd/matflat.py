@@ -17,6 +17,8 @@ Reference signatures:
   (MatFlat/Factorization.Cholesky.cs:114-168, :195-249)
 * Factorization.LuInverse(int n, double* a, int lda, int* piv)
   (MatFlat/Factorization.LuInverse.cs:105-192)
+* Factorization.Gevd(int n, double* a, int lda, double* b, int ldb, double* w)
+  (MatFlat/Factorization.Gevd.cs:145-186, GevdCore :330-376)
 * Blas.SolveTriangular(Uplo, Transpose, int n, double* / Complex* a, int lda, x, int incx)
   (MatFlat/Blas.SolveTriangular.cs:162-242, :280-395)
 * Blas.MulMatMat(Transpose, Transpose, int m, int n, int k, double* a, int lda,
@@ -164,6 +166,39 @@ class Factorization:
         if int(info.item()) != 0:
             raise MatrixFactorizationException("The matrix is not invertible.")
 
+
+    @staticmethod
+    def Gevd(n: int, a: torch.Tensor | None, lda: int, b: torch.Tensor | None, ldb: int,
+             w: torch.Tensor | None) -> None:
+        """A v = w B v for symmetric A and symmetric positive definite B (upper
+        triangles read): a -> eigenvectors, b -> L of B = L L^T, w -> the
+        eigenvalues, descending (Factorization.Gevd.cs:145-186, double)."""
+        if n <= 0:
+            raise ArgumentException("The order of the matrix must be greater than or equal to one.", "n")
+        if a is None:
+            raise ArgumentNullException("a")
+        if lda < n:
+            raise ArgumentException(
+                "The leading dimension must be greater than or equal to the order of the matrix.", "lda")
+        if b is None:
+            raise ArgumentNullException("b")
+        if ldb < n:
+            raise ArgumentException(
+                "The leading dimension must be greater than or equal to the order of the matrix.", "ldb")
+        if w is None:
+            raise ArgumentNullException("w")
+        _need(a, lda * (n - 1) + n, "a")
+        _need(b, ldb * (n - 1) + n, "b")
+        _need(w, n, "w")
+        dev = _dev(a)
+        info = torch.zeros(1, dtype=torch.int32, device=dev)
+        hip.dgevd_vbatched(_i32(n, dev), _ptrs(a), _i32(lda, dev), _ptrs(b), _i32(ldb, dev), _ptrs(w), info, n,
+                           handle=_exact())
+        code = int(info.item())
+        if code == n + 1:  # Factorization.Svd.cs:990-993
+            raise MatrixFactorizationException("SVD failed. The solution did not converge.")
+        if code != 0:  # Factorization.Gevd.cs:336-343
+            raise MatFlatException("The right-hand side matrix is not positive definite.")
 
 class Blas:
     @staticmethod

@@ -5,8 +5,9 @@ MatFlat's own tests hold no golden vectors for this path (they compare with
 OpenBLAS at 1e-12), so these fixtures are the build's own: seeded inputs and
 the oracle's bit-exact outputs for the shapes the reference's tests use
 (LuTests.cs:103-122, CholeskyTests.cs, BlasTests.SolveTriangular*.cs,
-BlasTests.MulMatMat*.cs, BlasTests.MulMatVecDouble.cs, LuInverseTests.cs; the
-complex LU, Cholesky and triangular solve on the same shapes), stored with
+BlasTests.MulMatMat*.cs, BlasTests.MulMatVecDouble.cs, LuInverseTests.cs,
+GevdTests.cs; the complex LU, Cholesky and triangular solve on the same
+shapes), stored with
 allow_pickle=False arrays only.  The
 GPU tests compare the HIP path against them without running the oracle.
 
@@ -40,6 +41,8 @@ GEMV_TRANS = [(1, 1, 1, 1, 1), (1, 1, 3, 2, 4), (2, 2, 2, 1, 1), (2, 2, 3, 2, 3)
 GEMV_LARGE = [(64, 100, 67, 3, 2), (130, 40, 133, 1, 1), (70, 129, 71, 2, 1)]
 GETRI_SHAPES = [(1, 1), (1, 3), (2, 2), (2, 3), (3, 3), (3, 5), (5, 5), (5, 8), (10, 10), (10, 17), (64, 64),
                 (100, 103)]  # (n, lda): LuInverseTests.cs:75-84, plus n = 64 and 100
+GEVD_SHAPES = [(1, 1, 1), (1, 3, 2), (2, 2, 2), (2, 4, 3), (3, 3, 3), (3, 4, 5), (5, 5, 5), (5, 7, 8), (10, 10, 10),
+               (10, 17, 11), (16, 19, 16), (64, 64, 64), (100, 101, 100), (128, 128, 130)]  # GevdTests.cs:94-105, larger
 GEMM_SHAPES = [(1, 1, 1), (3, 4, 5), (8, 16, 4), (17, 9, 13), (19, 21, 24)]
 
 
@@ -281,6 +284,45 @@ def complex_fixture(rng):
     return out
 
 
+def gevd_fixture(rng):
+    """Factorization.Gevd (double): A = X X^T, B = Y Y^T + I/2 with X, Y ~
+    U(-1, 1) (GevdTests.cs:295-330 builds the same kind of pair), strict lower
+    triangles NaN (never read), padding rows random; the last case has a B
+    that is not positive definite (info = 4, partial writes in b)."""
+    meta, ains, bins, aouts, bouts, ws = [], [], [], [], [], []
+    cases = [(n, lda, ldb, False) for n, lda, ldb in GEVD_SHAPES] + [(6, 7, 6, True)]
+    for n, lda, ldb, bad in cases:
+        bufs = []
+        for ld, shift in ((lda, 0.0), (ldb, 0.5)):
+            buf = 2.0 * rng.random(ld * n) - 1.0
+            x = buf[: ld * n].reshape(n, ld).T[:n, :n].copy()
+            m = x @ x.T + shift * np.eye(n)
+            v = buf[: ld * n].reshape(n, ld).T
+            v[:n, :n] = m
+            for j in range(n):
+                v[j + 1:n, j] = np.nan
+            bufs.append(buf)
+        a, b = bufs
+        if bad:
+            b[3 + 3 * ldb] = -50.0
+        w = np.zeros(n)
+        ao, bo = a.copy(), b.copy()
+        info = oracle.dgevd(n, ao, lda, bo, ldb, w)
+        assert info == (4 if bad else 0)
+        meta.append((n, lda, ldb, info))
+        ains.append(a)
+        bins.append(b)
+        aouts.append(ao)
+        bouts.append(bo)
+        ws.append(w)
+    a_in, aoff = pack(ains)
+    b_in, boff = pack(bins)
+    w, woff = pack(ws)
+    return {"gevd_meta": np.array(meta, dtype=np.int64), "gevd_a_in": a_in, "gevd_a_off": aoff,
+            "gevd_b_in": b_in, "gevd_b_off": boff, "gevd_a_out": pack(aouts)[0], "gevd_b_out": pack(bouts)[0],
+            "gevd_w": w, "gevd_w_off": woff}
+
+
 def main():
     oracle.build()
     rng = np.random.default_rng(20240610)
@@ -291,6 +333,7 @@ def main():
     np.savez_compressed(os.path.join(HERE, "gemv.npz"), **gemv_fixture(rng))
     np.savez_compressed(os.path.join(HERE, "getri.npz"), **getri_fixture(rng))
     np.savez_compressed(os.path.join(HERE, "complex.npz"), **complex_fixture(np.random.default_rng(20261204)))
+    np.savez_compressed(os.path.join(HERE, "gevd.npz"), **gevd_fixture(np.random.default_rng(20261205)))
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)), "bytes")

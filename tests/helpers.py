@@ -100,3 +100,22 @@ def hpd_upper_cbuf(seed: int, n: int, lda: int, shift: float | None = None, nan_
     for j in range(n):
         buf[j * lda:j * lda + j + 1] = a[: j + 1, j]
     return buf
+
+
+def gevd_pair_bufs(seed: int, n: int, lda: int, ldb: int, shift: float = 0.0):
+    """The pair GevdTests.cs:106-186 builds: A = X X^T and B = Y Y^T (+ shift I)
+    from U(-1, 1) matrices, stored in buffers whose padding rows keep their
+    random values, with the strict lower triangles set to NaN
+    (GevdTests.cs:114-115).  Returns (a_buf, b_buf, A, B)."""
+    mats, bufs = [], []
+    for s, ld in ((seed, lda), (seed + 1, ldb)):
+        buf = rand_buf(s, n, n, ld)
+        x = view(buf, n, n, ld)
+        m = x @ x.T + (shift * np.eye(n) if s != seed else 0.0)
+        store(buf, m, ld)
+        v = buf[: ld * n].reshape(n, ld).T
+        for j in range(n):
+            v[j + 1 : n, j] = np.nan
+        mats.append(m)
+        bufs.append(buf)
+    return bufs[0], bufs[1], mats[0], mats[1]

@@ -113,3 +113,16 @@ def test_complex_fixtures():
         x = g["ztrsv_x_in"][xoff:xoff + n * inc].copy()
         oracle.ztrsv(int(uplo), int(trans), int(n), a, int(lda), x, int(inc))
         assert_bits_equal(x, g["ztrsv_x_out"][xoff:xoff + n * inc], f"ztrsv {i}")
+
+
+def test_gevd_fixtures():
+    g = load("gevd.npz")
+    for i, (n, lda, ldb, info) in enumerate(g["gevd_meta"]):
+        ao, bo, wo = g["gevd_a_off"][i], g["gevd_b_off"][i], g["gevd_w_off"][i]
+        a = g["gevd_a_in"][ao:ao + lda * n].copy()
+        b = g["gevd_b_in"][bo:bo + ldb * n].copy()
+        w = np.zeros(n)
+        assert oracle.dgevd(int(n), a, int(lda), b, int(ldb), w) == info
+        assert_bits_equal(a, g["gevd_a_out"][ao:ao + lda * n], f"gevd a {i}")
+        assert_bits_equal(b, g["gevd_b_out"][bo:bo + ldb * n], f"gevd b {i}")
+        assert_bits_equal(w, g["gevd_w"][wo:wo + n], f"gevd w {i}")
