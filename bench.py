@@ -927,7 +927,110 @@ def gemv_leg(steps: int, warmup: int, count: int = 200_000, seed: int = 6):
     return res
 
 
-LEGS = ("lu", "adjoint", "gemm", "zgemm", "gemv", "trsv")
+def gevd_leg(steps: int, warmup: int, count: int = 10_000, nmin: int = 8, nmax: int = 64, seed: int = 8,
+             cpu: bool = True):
+    """Factorization.Gevd (double, MatFlat's order) over `count` pairs, n ~
+    U{nmin..nmax}: A = X X^T, B = Y Y^T + n I / 4 with X, Y ~ N(0, 1), upper
+    triangles stored (lower triangles NaN, never read), lda = ldb = n.  The call
+    overwrites a and b, so both are restored (untimed) before every call."""
+    import torch
+
+    from matflat_amd import hip
+
+    rng = np.random.default_rng(seed)
+    ns = rng.integers(nmin, nmax + 1, size=count).astype(np.int32)
+    sizes = ns.astype(np.int64) ** 2
+    offs = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+    a_host = np.empty(int(sizes.sum()))
+    b_host = np.empty(int(sizes.sum()))
+    for i, n in enumerate(ns):
+        x = rng.standard_normal((n, n))
+        y = rng.standard_normal((n, n))
+        am = x @ x.T
+        bm = y @ y.T + (n / 4.0) * np.eye(n)
+        low = np.tril_indices(n, -1)
+        am[low] = np.nan
+        bm[low] = np.nan
+        a_host[offs[i]:offs[i] + n * n] = am.T.reshape(-1)  # column-major
+        b_host[offs[i]:offs[i] + n * n] = bm.T.reshape(-1)
+    woffs = np.concatenate([[0], np.cumsum(ns.astype(np.int64))[:-1]]).astype(np.int64)
+    dev = torch.device("cuda")
+    a_src = torch.from_numpy(a_host).to(dev)
+    b_src = torch.from_numpy(b_host).to(dev)
+    a, b = a_src.clone(), b_src.clone()
+    w = torch.zeros(int(ns.sum()), dtype=torch.float64, device=dev)
+    o = torch.from_numpy(offs).to(dev)
+    pa = (a.data_ptr() + 8 * o).contiguous()
+    pb = (b.data_ptr() + 8 * o).contiguous()
+    pw = (w.data_ptr() + 8 * torch.from_numpy(woffs).to(dev)).contiguous()
+    n_t = torch.from_numpy(ns).to(dev)
+    info = torch.zeros(count, dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream()
+    handle = hip.Handle()
+    handle.bind_stream(stream)
+
+    def restore():
+        a.copy_(a_src)
+        b.copy_(b_src)
+
+    ms = timed_calls(lambda: hip.dgevd_vbatched(n_t, pa, n_t, pb, n_t, pw, info, nmax, handle=handle), restore,
+                     steps, warmup, stream)
+    torch.cuda.synchronize()
+    ok = bool(torch.all(info == 0).item())
+    # one sampled matrix per size range checked against the oracle, bit for bit
+    check = {}
+    if cpu:
+        import oracle
+
+        ga, gb, gw = a.cpu().numpy(), b.cpu().numpy(), w.cpu().numpy()
+        same = True
+        for i in list(range(0, count, max(1, count // 16))):
+            n = int(ns[i])
+            aa = a_host[offs[i]:offs[i] + n * n].copy()
+            bb = b_host[offs[i]:offs[i] + n * n].copy()
+            ww = np.zeros(n)
+            oracle.dgevd(n, aa, n, bb, n, ww)
+            same &= bool(np.array_equal(aa.view(np.uint64), ga[offs[i]:offs[i] + n * n].view(np.uint64))
+                         and np.array_equal(bb.view(np.uint64), gb[offs[i]:offs[i] + n * n].view(np.uint64))
+                         and np.array_equal(ww.view(np.uint64), gw[woffs[i]:woffs[i] + n].view(np.uint64)))
+        check = {"sampled": len(range(0, count, max(1, count // 16))), "bit_exact": same}
+    handle.close()
+    n64 = ns.astype(np.float64)
+    alg = float(np.sum(8.0 * (n64 * (n64 + 1) + 2 * n64 * n64 + n64) + 4))
+    res = {"workload": f"{count} generalized eigenproblems A v = w B v (Factorization.Gevd, MatFlat's order), "
+                       f"n ~ U{{{nmin}..{nmax}}}, A = X X^T, B = Y Y^T + n I / 4, X, Y ~ N(0, 1)",
+           "ms_per_call": round(ms, 4), "problems_per_s": round(count / (ms / 1e3), 1), "all_info_zero": ok,
+           "roofline": hbm_entry(alg, ms), "output_check": check,
+           "note": "one wavefront per problem; the Householder, QR-sweep and rotation chains are sequential, so "
+                   "the time is latency, not bytes"}
+    if cpu:
+        import oracle
+
+        threads = host_cpu()["threads"]
+
+        def run():
+            aa, bb = a_host.copy(), b_host.copy()
+            ww = np.zeros(int(ns.sum()))
+            t0 = time.perf_counter()
+            info_c = oracle.dgevd_batch(ns, ns, ns, offs, offs, woffs, aa, bb, ww, threads=threads)
+            dt = time.perf_counter() - t0
+            assert np.all(info_c == 0)
+            return dt
+
+        med, reps = timed_cpu(run, 10.0, min_reps=3, max_reps=5)
+        res["cpu_baseline"] = cpu_entry(count / med, "problems/s",
+                                        f"the full {count}-pair batch, Factorization.Gevd in MatFlat's order "
+                                        f"(oracle/matflat_oracle.c), OpenMP dynamic over {threads} threads, "
+                                        f"median of {reps} runs", reps)
+    log(f"  gevd x{count} n~U{{{nmin}..{nmax}}}: {ms:.3f} ms/call, {res['problems_per_s']:.0f}/s, "
+        f"{res['roofline']['achieved']} GB/s; check {check}"
+        + (f"; cpu {res['cpu_baseline']['value']:.0f}/s" if cpu else ""))
+    del a, b, a_src, b_src
+    torch.cuda.empty_cache()
+    return res
+
+
+LEGS = ("lu", "adjoint", "gemm", "zgemm", "gemv", "trsv", "gevd")
 
 
 def other_legs(steps: int, cpu: bool = True, which=LEGS) -> dict:
@@ -937,7 +1040,8 @@ def other_legs(steps: int, cpu: bool = True, which=LEGS) -> dict:
            "gemm": ("gemm_fp64", lambda: gemm_leg(max(3, steps // 4), 1, cpu=cpu)),
            "zgemm": ("gemm_complex128", lambda: gemm_leg(max(3, steps // 4), 1, complex_=True, cpu=cpu)),
            "gemv": ("gemv_fp64", lambda: gemv_leg(steps, 2)),
-           "trsv": ("trsv", lambda: trsv_leg(steps, 2))}
+           "trsv": ("trsv", lambda: trsv_leg(steps, 2)),
+           "gevd": ("gevd", lambda: gevd_leg(max(3, steps // 4), 1, cpu=cpu))}
     out = {}
     for w in which:
         key, fn = run[w]
