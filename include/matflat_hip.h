@@ -344,6 +344,21 @@ int mfh_dposv_backward_vbatched(mfh_handle* h, int count, const int* n, const do
                                 double* const* abar, const int* ldabar, double* const* bbar,
                                 int* info, int max_n);
 
+/* ---- adjoint of the Cholesky factorization ---------------------------------
+ * No MatFlat counterpart.  For A = L L^T (l from mfh_dpotrf_vbatched, lower
+ * triangle read) and lbar = dloss/dL (lower triangle read, the strict upper
+ * ignored): abar = L^-T M L^-1, M = (Phi(L^T Lbar) + Phi(L^T Lbar)^T) / 2 with
+ * Phi = the lower triangle with its diagonal halved -- the gradient with
+ * respect to the symmetric matrix the factor came from, written as a full
+ * symmetric n x n matrix with leading dimension ldabar (which must not
+ * overlap l or lbar).  Both triangular solves are MatFlat's Lower/Trans solve
+ * (Blas.SolveTriangular.cs:362-371).  info[b] = 0, or -(position) for invalid
+ * per-matrix arguments.  No check of L's diagonal: a zero gives inf/NaN, as
+ * the triangular solve does.  n <= MFH_SMALL_MAX_N. */
+int mfh_dpotrf_backward_vbatched(mfh_handle* h, int count, const int* n, const double* const* l,
+                                 const int* ldl, const double* const* lbar, const int* ldlbar,
+                                 double* const* abar, const int* ldabar, int* info, int max_n);
+
 /* ---- generalized symmetric-definite eigenproblem ---------------------------
  * Factorization.Gevd (double), Factorization.Gevd.cs:145-186 with GevdCore
  * :330-376: A v = w B v for symmetric A and symmetric positive definite B,

@@ -76,6 +76,7 @@ SIGNATURES = {
     "mfh_dgetrf_blocked_vbatched": [_p, _i, _p, _p, _p, _p, _p, _i],
     "mfh_dposv_backward_vbatched": [_p, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i],
     "mfh_dgevd_vbatched": [_p, _i, _p, _p, _p, _p, _p, _p, _p, _i],
+    "mfh_dpotrf_backward_vbatched": [_p, _i, _p, _p, _p, _p, _p, _p, _p, _p, _i],
     "mfh_plan_create_dposv": [_p, _i, _p, _p, _p, _p, _p, _i, _p],
     "mfh_plan_create_dgemm": [_p, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i, _i, _i, _p],
     "mfh_plan_create_zgemm": [_p, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i, _i, _i, _p],

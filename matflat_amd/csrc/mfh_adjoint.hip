@@ -1,4 +1,5 @@
-// mfh_adjoint.hip -- adjoint of the batched SPD solve x = A^-1 b, A = L L^T.
+// mfh_adjoint.hip -- adjoints of the batched SPD solve x = A^-1 b and of the
+// Cholesky factorization A = L L^T.
 //
 // MatFlat has no derivatives; this is the differentiation addition beside
 // the plain path.  For loss(x) with xbar = dloss/dx:
@@ -123,7 +124,141 @@ int launch_t(mfh_handle* h, int count, const MfhAdjointBatch& B, int max_n)
     return MFH_OK;
 }
 
+// Adjoint of the Cholesky factorization: Abar = L^-T M L^-1 with M the
+// symmetrized Phi(L^T Lbar) / 2 (Phi: lower triangle, diagonal halved; I.
+// Murray, arXiv:1602.07527, eq. 10, taken through the symmetric part).
+// One workgroup per matrix.  LDS holds L in its lower triangle and Lbar's
+// strict lower triangle transposed into the strict upper one (Lbar's
+// diagonal beside it), so both inputs cost one matrix of LDS; the work
+// matrix W is a second LDS matrix when it fits (WLDS), else the output abar
+// itself.  Every value follows the oracle's order exactly:
+//   1. one thread per (i >= j): s = sum_{k >= i} L[k][i] Lbar[k][j] from 0.0
+//      upward, W[i][j] = W[j][i] = 0.5 s;
+//   2. one lane per column of W: MatFlat's Lower/Trans solve (axpy form,
+//      Blas.SolveTriangular.cs:362-371) -> L^-T M;
+//   3. one lane per row: the same solve along the row -> L^-T M L^-1;
+//   4. abar[i][j] = abar[j][i] = 0.5 (W[i][j] + W[j][i]) for i > j.
+template <int T, bool WLDS>
+__global__ __launch_bounds__(T) void k_potrf_backward(MfhCholAdjointBatch B, int count, int max_n, int ld)
+{
+    extern __shared__ double lds[];
+    const int tid = threadIdx.x;
+    double* A1 = lds;
+    double* dLb = lds + (size_t)max_n * ld;
+    double* Wl = dLb + max_n + 1;  // odd offset keeps the two matrices on different banks
+    for (int b = blockIdx.x; b < count; b += gridDim.x) {
+        const int n = __builtin_amdgcn_readfirstlane(B.n[b]);
+        const int ldl = __builtin_amdgcn_readfirstlane(B.ldl[b]);
+        const int ldlb = __builtin_amdgcn_readfirstlane(B.ldlbar[b]);
+        const int ldab = __builtin_amdgcn_readfirstlane(B.ldabar[b]);
+        int bad = 0;
+        if (n < 1 || n > max_n || n > T) bad = -3;
+        else if (B.l[b] == nullptr) bad = -4;
+        else if (ldl < n) bad = -5;
+        else if (B.lbar[b] == nullptr) bad = -6;
+        else if (ldlb < n) bad = -7;
+        else if (B.abar[b] == nullptr) bad = -8;
+        else if (ldab < n) bad = -9;
+        bad = __builtin_amdgcn_readfirstlane(bad);
+        if (bad) {
+            if (tid == 0) B.info[b] = bad;
+            continue;
+        }
+        const double* L = B.l[b];
+        const double* Lb = B.lbar[b];
+        double* Ab = B.abar[b];
+        double* W = WLDS ? Wl : Ab;
+        const int ldw = WLDS ? ld : ldab;
+        __syncthreads();  // the previous matrix's reads of LDS are done
+        for (int e = tid; e < n * n; e += T) {
+            const int j = e / n;
+            const int i = e - j * n;
+            if (i >= j) {
+                A1[i + j * ld] = L[i + (size_t)j * ldl];
+                const double v = Lb[i + (size_t)j * ldlb];
+                if (i > j) A1[j + i * ld] = v;
+                else dLb[i] = v;
+            }
+        }
+        __syncthreads();
+        for (int e = tid; e < n * n; e += T) {
+            const int j = e / n;
+            const int i = e - j * n;
+            if (i >= j) {
+                double s = 0.0;
+                for (int k = i; k < n; ++k) s = s + A1[k + i * ld] * (k == j ? dLb[j] : A1[j + k * ld]);
+                const double m = 0.5 * s;
+                W[i + (size_t)j * ldw] = m;
+                W[j + (size_t)i * ldw] = m;
+            }
+        }
+        __syncthreads();
+        for (int c = tid; c < n; c += T) {
+            double* x = W + (size_t)c * ldw;
+            for (int jj = n - 1; jj >= 0; --jj) {
+                const double y = x[jj] / A1[jj + jj * ld];
+                x[jj] = y;
+                for (int r = 0; r < jj; ++r) x[r] = x[r] - A1[jj + r * ld] * y;
+            }
+        }
+        __syncthreads();
+        for (int r0 = tid; r0 < n; r0 += T) {
+            double* x = W + r0;
+            for (int jj = n - 1; jj >= 0; --jj) {
+                const double y = x[(size_t)jj * ldw] / A1[jj + jj * ld];
+                x[(size_t)jj * ldw] = y;
+                for (int r = 0; r < jj; ++r) x[(size_t)r * ldw] = x[(size_t)r * ldw] - A1[jj + r * ld] * y;
+            }
+        }
+        __syncthreads();
+        for (int e = tid; e < n * n; e += T) {
+            const int j = e / n;
+            const int i = e - j * n;
+            if (i > j) {
+                const double v = 0.5 * (W[i + (size_t)j * ldw] + W[j + (size_t)i * ldw]);
+                Ab[i + (size_t)j * ldab] = v;
+                Ab[j + (size_t)i * ldab] = v;
+            } else if (i == j && WLDS) {
+                Ab[i + (size_t)i * ldab] = W[i + (size_t)i * ldw];
+            }
+        }
+        if (tid == 0) B.info[b] = 0;
+    }
+}
+
+size_t potrf_backward_lds(int max_n, int ld, bool wlds)
+{
+    return ((size_t)(wlds ? 2 : 1) * max_n * ld + max_n + 1) * sizeof(double);
+}
+
+template <int T, bool WLDS>
+int launch_potrf_backward_t(mfh_handle* h, int count, const MfhCholAdjointBatch& B, int max_n, int ld)
+{
+    const size_t lds = potrf_backward_lds(max_n, ld, WLDS);
+    MFH_HIP_CHECK(hipFuncSetAttribute((const void*)k_potrf_backward<T, WLDS>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    int per_cu = 0;
+    MFH_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_potrf_backward<T, WLDS>, T, lds));
+    if (per_cu < 1) per_cu = 1;
+    long grid = (long)per_cu * h->num_cus;
+    if (grid > count) grid = count;
+    hipLaunchKernelGGL((k_potrf_backward<T, WLDS>), dim3((unsigned)grid), dim3(T), lds, h->stream, B, count, max_n,
+                       ld);
+    MFH_HIP_CHECK(hipGetLastError());
+    return MFH_OK;
+}
+
 }  // namespace
+
+int mfh_launch_potrf_backward(mfh_handle* h, int count, const MfhCholAdjointBatch& B, int max_n)
+{
+    const int ld = max_n | 1;  // odd: lanes walking rows hit distinct LDS banks
+    const bool wlds = potrf_backward_lds(max_n, ld, true) <= 160 * 1024;
+    if (max_n <= 64) return wlds ? launch_potrf_backward_t<64, true>(h, count, B, max_n, ld)
+                                 : launch_potrf_backward_t<64, false>(h, count, B, max_n, ld);
+    return wlds ? launch_potrf_backward_t<128, true>(h, count, B, max_n, ld)
+                : launch_potrf_backward_t<128, false>(h, count, B, max_n, ld);
+}
 
 int mfh_launch_posv_backward(mfh_handle* h, int count, const MfhAdjointBatch& B, int max_n)
 {

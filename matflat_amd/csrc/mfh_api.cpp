@@ -659,6 +659,26 @@ int mfh_dposv_backward_vbatched(mfh_handle* h, int count, const int* n, const do
     return mfh_launch_posv_backward(h, count, B, max_n);
 }
 
+int mfh_dpotrf_backward_vbatched(mfh_handle* h, int count, const int* n, const double* const* l, const int* ldl,
+                                 const double* const* lbar, const int* ldlbar, double* const* abar,
+                                 const int* ldabar, int* info, int max_n)
+{
+    MFH_REQUIRE(h != nullptr, 1, "handle must not be null");
+    MFH_REQUIRE(count >= 0, 2, "count must be >= 0");
+    if (count == 0) return MFH_OK;
+    MFH_REQUIRE(n != nullptr, 3, "n must not be null");
+    MFH_REQUIRE(l != nullptr, 4, "l must not be null");
+    MFH_REQUIRE(ldl != nullptr, 5, "ldl must not be null");
+    MFH_REQUIRE(lbar != nullptr, 6, "lbar must not be null");
+    MFH_REQUIRE(ldlbar != nullptr, 7, "ldlbar must not be null");
+    MFH_REQUIRE(abar != nullptr, 8, "abar must not be null");
+    MFH_REQUIRE(ldabar != nullptr, 9, "ldabar must not be null");
+    MFH_REQUIRE(info != nullptr, 10, "info must not be null");
+    MFH_REQUIRE(max_n >= 1 && max_n <= MFH_SMALL_MAX_N, 11, "max_n must be in [1, MFH_SMALL_MAX_N]");
+    MfhCholAdjointBatch B{n, l, ldl, lbar, ldlbar, abar, ldabar, info};
+    return mfh_launch_potrf_backward(h, count, B, max_n);
+}
+
 int mfh_dgevd_vbatched(mfh_handle* h, int count, const int* n, double* const* a, const int* lda, double* const* b,
                        const int* ldb, double* const* w, int* info, int max_n)
 {

@@ -175,6 +175,17 @@ struct MfhAdjointBatch {
     int* info;
 };
 
+struct MfhCholAdjointBatch {
+    const int* n;
+    const double* const* l;
+    const int* ldl;
+    const double* const* lbar;
+    const int* ldlbar;
+    double* const* abar;
+    const int* ldabar;
+    int* info;
+};
+
 // Device workspace bytes the exact kernels need for a batch of count.
 size_t mfh_exact_ws_bytes(int count);
 
@@ -208,6 +219,7 @@ int mfh_launch_gemm(mfh_handle* h, int count, const MfhGemmBatch& B, int max_m, 
 int mfh_launch_getri(mfh_handle* h, int count, const MfhGetriBatch& B, int max_n);
 int mfh_launch_gemv(mfh_handle* h, int count, const MfhGemvBatch& B, int max_m, int max_n);
 int mfh_launch_posv_backward(mfh_handle* h, int count, const MfhAdjointBatch& B, int max_n);
+int mfh_launch_potrf_backward(mfh_handle* h, int count, const MfhCholAdjointBatch& B, int max_n);
 // Factorization.Gevd over a batch (mfh_gevd.hip): argument check, the exact
 // Cholesky of B, then one wavefront per matrix.  Uses h->aux (2 count ints).
 int mfh_launch_gevd(mfh_handle* h, int count, const int* n, double* const* a, const int* lda, double* const* b,

@@ -294,6 +294,15 @@ def dposv_backward_vbatched(n, l_ptrs, lda, x_ptrs, xbar_ptrs, abar_ptrs, ldabar
                                              int(max_n)))
 
 
+def dpotrf_backward_vbatched(n, l_ptrs, ldl, lbar_ptrs, ldlbar, abar_ptrs, ldabar, info, max_n,
+                             handle: Handle | None = None):
+    """Adjoint of the batched Cholesky: abar = dloss/dA (full symmetric) from
+    L and lbar = dloss/dL."""
+    h = handle or default_handle()
+    _check(h.lib.mfh_dpotrf_backward_vbatched(h.h, int(n.numel()), _p(n), _p(l_ptrs), _p(ldl), _p(lbar_ptrs),
+                                              _p(ldlbar), _p(abar_ptrs), _p(ldabar), _p(info), int(max_n)))
+
+
 def dgevd_vbatched(n, a_ptrs, lda, b_ptrs, ldb, w_ptrs, info, max_n, handle: Handle | None = None):
     """Factorization.Gevd (double) over a batch: a -> eigenvectors, b -> L,
     w -> eigenvalues (descending), in MatFlat's operation order."""

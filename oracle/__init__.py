@@ -118,6 +118,10 @@ def lib() -> ctypes.CDLL:
         _lib.mfo_dsvd_u.argtypes = [i, p, i, p, p, i, p, p, p]
         _lib.mfo_dgevd.restype = i
         _lib.mfo_dgevd.argtypes = [i, p, i, p, i, p, p, p]
+        _lib.mfo_dpotrf_backward.restype = None
+        _lib.mfo_dpotrf_backward.argtypes = [i, p, i, p, i, p, i]
+        _lib.mfo_dpotrf_backward_batch.restype = None
+        _lib.mfo_dpotrf_backward_batch.argtypes = [i, i, i, i64, p, p, p, i]
         _lib.mfo_dgevd_batch.restype = None
         _lib.mfo_dgevd_batch.argtypes = [i, p, p, p, p, p, p, p, p, p, p, i]
     return _lib
@@ -335,3 +339,18 @@ def dgevd_batch(n, lda, ldb, aoff, boff, woff, a, b, w, threads=1) -> np.ndarray
     lib().mfo_dgevd_batch(n.size, _ptr(n), _ptr(lda), _ptr(ldb), _ptr(aoff), _ptr(boff), _ptr(woff), _ptr(a),
                           _ptr(b), _ptr(w), _ptr(info), threads)
     return info
+
+
+def dpotrf_backward(n: int, l: np.ndarray, ldl: int, lbar: np.ndarray, ldlb: int, ldab: int | None = None) -> np.ndarray:
+    """Adjoint of the Cholesky factorization: Abar (full symmetric, ldab x n
+    buffer) from L and Lbar = dloss/dL."""
+    ldab = ldab or n
+    abar = np.zeros(ldab * n)
+    l = np.ascontiguousarray(l, dtype=np.float64)
+    lbar = np.ascontiguousarray(lbar, dtype=np.float64)
+    lib().mfo_dpotrf_backward(n, _ptr(l), ldl, _ptr(lbar), ldlb, _ptr(abar), ldab)
+    return abar
+
+
+def dpotrf_backward_batch(count, n, ld, stride, l, lbar, abar, threads=1) -> None:
+    lib().mfo_dpotrf_backward_batch(count, n, ld, stride, _ptr(l), _ptr(lbar), _ptr(abar), threads)

@@ -451,6 +451,57 @@ void mfo_dposv_backward(int n, const double* l, int lda, const double* x, const 
     }
 }
 
+/* Adjoint of the Cholesky factorization A = L L^T (no reference
+ * counterpart; MatFlat has no derivatives).  Given L (lower, as
+ * mfo_dpotrf leaves it) and Lbar = dloss/dL (lower triangle read), the
+ * gradient with respect to the symmetric matrix A is
+ *     Abar = L^-T M L^-1,  M = (Phi(L^T Lbar) + Phi(L^T Lbar)^T) / 2,
+ * Phi = lower triangle with the diagonal halved (I. Murray, "Differentiation
+ * of the Cholesky decomposition", arXiv:1602.07527, 2016, eq. 10, taken
+ * through the symmetric part).  Operation order, which the GPU kernel
+ * follows exactly:
+ *   M[i][j] = M[j][i] = 0.5 * s, s = sum over k = i..n-1 of L[k][i] Lbar[k][j]
+ *       (i >= j, s from 0.0 upward in k);
+ *   every column of M: Lower/Trans solve (MatFlat's axpy form,
+ *       Blas.SolveTriangular.cs:362-371) -> X = L^-T M;
+ *   every row of X: the same solve on the row (stride ldab) -> X L^-1;
+ *   Abar[i][j] = Abar[j][i] = 0.5 * (Abar[i][j] + Abar[j][i]) for i > j. */
+void mfo_dpotrf_backward(int n, const double* l, int ldl, const double* lbar, int ldlb, double* abar, int ldab)
+{
+    for (int j = 0; j < n; j++) {
+        for (int i = j; i < n; i++) {
+            double s = 0.0;
+            for (int k = i; k < n; k++) s = s + l[k + (size_t)i * ldl] * lbar[k + (size_t)j * ldlb];
+            const double m = 0.5 * s;
+            abar[i + (size_t)j * ldab] = m;
+            abar[j + (size_t)i * ldab] = m;
+        }
+    }
+    for (int c = 0; c < n; c++) mfo_dtrsv(MFO_LOWER, MFO_TRANS, n, l, ldl, abar + (size_t)c * ldab, 1);
+    for (int r = 0; r < n; r++) mfo_dtrsv(MFO_LOWER, MFO_TRANS, n, l, ldl, abar + r, ldab);
+    for (int j = 0; j < n; j++) {
+        for (int i = j + 1; i < n; i++) {
+            const double v = 0.5 * (abar[i + (size_t)j * ldab] + abar[j + (size_t)i * ldab]);
+            abar[i + (size_t)j * ldab] = v;
+            abar[j + (size_t)i * ldab] = v;
+        }
+    }
+}
+
+void mfo_dpotrf_backward_batch(int count, int n, int ld, int64_t stride, const double* l, const double* lbar,
+                               double* abar, int threads)
+{
+#ifdef _OPENMP
+    if (threads < 1) threads = 1;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(threads)
+#endif
+    for (int i = 0; i < count; i++) {
+        mfo_dpotrf_backward(n, l + (size_t)i * stride, ld, lbar + (size_t)i * stride, ld, abar + (size_t)i * stride,
+                            ld);
+    }
+    (void)threads;
+}
+
 /* ------------------------------------------------------------------ complex
  * Complex128 factorizations and triangular solve.  Buffers hold interleaved
  * (re, im) doubles; element (i, j) of a matrix is at 2*(i + j*lda).

@@ -165,6 +165,16 @@ void mfo_dposv_backward_batch(int count, int n, int lda, int64_t stride, const d
 void mfo_dposv_backward(int n, const double* l, int lda, const double* x, const double* w,
                         double* abar, int ldab, double* bbar);
 
+/* Adjoint of A = L L^T: Abar = L^-T M L^-1 with M the symmetrized
+ * Phi(L^T Lbar) / 2 (full symmetric n x n, leading dimension ldab); the
+ * operation order is fixed in matflat_oracle.c. */
+void mfo_dpotrf_backward(int n, const double* l, int ldl, const double* lbar, int ldlb, double* abar, int ldab);
+
+/* mfo_dpotrf_backward over a fixed-size batch (matrices stride elements apart,
+ * leading dimension ld). */
+void mfo_dpotrf_backward_batch(int count, int n, int ld, int64_t stride, const double* l, const double* lbar,
+                               double* abar, int threads);
+
 #ifdef __cplusplus
 }
 #endif
