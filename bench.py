@@ -673,13 +673,24 @@ def adjoint_leg(steps: int, warmup: int, count: int = 50_000, n: int = 32, seed:
                                                                 handle=handle), None, copies, steps, warmup, stream)
     bwd = float(np.mean(bt))
     ok_b = bool(torch.all(info == 0).item())
+    # the Cholesky adjoint: Abar from each copy's L and a lower-triangular Lbar
+    lbar = torch.tril(torch.randn((count, n, n), dtype=torch.float64, device=dev, generator=g))
+    plb = ptrs(lbar, n * n)
+    ct, _ = rotated_calls(lambda i: hip.dpotrf_backward_vbatched(ns, pa[i], ns, plb, ns, pab, ns, info, n,
+                                                                 handle=handle), None, copies, steps, warmup, stream)
+    cbwd = float(np.mean(ct))
+    ok_c = bool(torch.all(info == 0).item())
     handle.close()
     fb = count * 8.0 * (n * (n + 1) / 2 + n * n + 2 * n)
     bb = count * 8.0 * (2 * n * n + 3 * n)
+    cb = count * 8.0 * (n * (n + 1) + n * n)
     res = {"workload": f"{count} SPD solves n = {n} (exact mode) and their adjoint (dL/dA, dL/db for L = w.x)",
            "working_copies": copies, "forward_ms": round(fwd, 5), "backward_ms": round(bwd, 5),
-           "forward_plus_backward_per_s": round(count / ((fwd + bwd) / 1e3), 1), "all_info_zero": ok_f and ok_b,
-           "forward_roofline": hbm_entry(fb, fwd), "backward_roofline": hbm_entry(bb, bwd)}
+           "forward_plus_backward_per_s": round(count / ((fwd + bwd) / 1e3), 1), "all_info_zero": ok_f and ok_b and ok_c,
+           "forward_roofline": hbm_entry(fb, fwd), "backward_roofline": hbm_entry(bb, bwd),
+           "cholesky_backward": {"workload": f"{count} Cholesky adjoints n = {n}: dloss/dA from L and dloss/dL",
+                                 "ms_per_call": round(cbwd, 5), "per_s": round(count / (cbwd / 1e3), 1),
+                                 "roofline": hbm_entry(cb, cbwd)}}
     if cpu:
         import oracle
 
@@ -700,6 +711,19 @@ def adjoint_leg(steps: int, warmup: int, count: int = 50_000, n: int = 32, seed:
             assert np.all(info_c == 0)
             return dt
 
+        l_host = a[0].cpu().numpy().reshape(-1)  # copy 0's factors (the posv left L in place)
+        lb_host = lbar.cpu().numpy().reshape(-1)
+
+        def run_chol():
+            ab = np.empty_like(l_host)
+            t0 = time.perf_counter()
+            oracle.dpotrf_backward_batch(count, n, n, n * n, l_host, lb_host, ab, threads=threads)
+            return time.perf_counter() - t0
+
+        cmed, creps = timed_cpu(run_chol, 4.0, min_reps=3, max_reps=10)
+        res["cholesky_backward"]["cpu_baseline"] = cpu_entry(
+            count / cmed, "adjoints/s", f"the full {count}-matrix batch, the oracle's Cholesky adjoint "
+            f"(oracle/matflat_oracle.c), OpenMP dynamic over {threads} threads, median of {creps} runs", creps)
         med, reps = timed_cpu(run, 4.0, min_reps=3, max_reps=10)
         res["cpu_baseline"] = cpu_entry(count / med, "forward+backward/s",
                                         f"the full {count}-matrix batch: Cholesky + two solves in MatFlat's order, "
@@ -707,7 +731,8 @@ def adjoint_leg(steps: int, warmup: int, count: int = 50_000, n: int = 32, seed:
                                         f"(oracle/matflat_oracle.c), OpenMP dynamic over {threads} threads, "
                                         f"median of {reps} runs", reps)
     log(f"  adjoint n={n} x{count}: forward {fwd:.4f} ms ({res['forward_roofline']['achieved']} GB/s), backward "
-        f"{bwd:.4f} ms ({res['backward_roofline']['achieved']} GB/s)"
+        f"{bwd:.4f} ms ({res['backward_roofline']['achieved']} GB/s); cholesky backward {cbwd:.4f} ms "
+        f"({res['cholesky_backward']['roofline']['achieved']} GB/s)"
         + (f"; cpu {res['cpu_baseline']['value']:.0f}/s" if cpu else ""))
     return res
 

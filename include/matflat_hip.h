@@ -351,8 +351,8 @@ int mfh_dposv_backward_vbatched(mfh_handle* h, int count, const int* n, const do
  * Phi = the lower triangle with its diagonal halved -- the gradient with
  * respect to the symmetric matrix the factor came from, written as a full
  * symmetric n x n matrix with leading dimension ldabar (which must not
- * overlap l or lbar).  Both triangular solves are MatFlat's Lower/Trans solve
- * (Blas.SolveTriangular.cs:362-371).  info[b] = 0, or -(position) for invalid
+ * overlap l or lbar).  Both triangular solves run in dot form with
+ * Internals.Dot's pairing (Internals.cs:265-292).  info[b] = 0, or -(position) for invalid
  * per-matrix arguments.  No check of L's diagonal: a zero gives inf/NaN, as
  * the triangular solve does.  n <= MFH_SMALL_MAX_N. */
 int mfh_dpotrf_backward_vbatched(mfh_handle* h, int count, const int* n, const double* const* l,

@@ -134,8 +134,11 @@ int launch_t(mfh_handle* h, int count, const MfhAdjointBatch& B, int max_n)
 // itself.  Every value follows the oracle's order exactly:
 //   1. one thread per (i >= j): s = sum_{k >= i} L[k][i] Lbar[k][j] from 0.0
 //      upward, W[i][j] = W[j][i] = 0.5 s;
-//   2. one lane per column of W: MatFlat's Lower/Trans solve (axpy form,
-//      Blas.SolveTriangular.cs:362-371) -> L^-T M;
+//   2. one lane per column x of W: L^T x = m in dot form, j descending,
+//      x[j] = (x[j] - Dot(L[j+1.., j], x[j+1..])) / L[j][j] with
+//      Internals.Dot's pairing -> L^-T M.  The dot only loads (eight pairs in
+//      flight) and each step stores once; the axpy form would store inside
+//      its inner loop and wait a full LDS round trip per element;
 //   3. one lane per row: the same solve along the row -> L^-T M L^-1;
 //   4. abar[i][j] = abar[j][i] = 0.5 (W[i][j] + W[j][i]) for i > j.
 template <int T, bool WLDS>
@@ -170,14 +173,31 @@ __global__ __launch_bounds__(T) void k_potrf_backward(MfhCholAdjointBatch B, int
         double* W = WLDS ? Wl : Ab;
         const int ldw = WLDS ? ld : ldab;
         __syncthreads();  // the previous matrix's reads of LDS are done
-        for (int e = tid; e < n * n; e += T) {
-            const int j = e / n;
-            const int i = e - j * n;
-            if (i >= j) {
-                A1[i + j * ld] = L[i + (size_t)j * ldl];
-                const double v = Lb[i + (size_t)j * ldlb];
-                if (i > j) A1[j + i * ld] = v;
-                else dLb[i] = v;
+        {
+            typedef const __attribute__((address_space(1))) double* GSrc;
+            const int tot = n * n;
+            for (int e0 = 0; e0 < tot; e0 += 8 * T) {  // eight loads of each input in flight per thread
+                double lv[8], bv[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int e = e0 + u * T + tid;
+                    const int ec = e < tot ? e : tot - 1;
+                    const int j = ec / n;
+                    const int i = ec - j * n;
+                    lv[u] = i >= j ? ((GSrc)L)[i + (size_t)j * ldl] : 0.0;
+                    bv[u] = i >= j ? ((GSrc)Lb)[i + (size_t)j * ldlb] : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int e = e0 + u * T + tid;
+                    const int j = e / n;
+                    const int i = e - j * n;
+                    if (e < tot && i >= j) {
+                        A1[i + j * ld] = lv[u];
+                        if (i > j) A1[j + i * ld] = bv[u];
+                        else dLb[i] = bv[u];
+                    }
+                }
             }
         }
         __syncthreads();
@@ -185,8 +205,22 @@ __global__ __launch_bounds__(T) void k_potrf_backward(MfhCholAdjointBatch B, int
             const int j = e / n;
             const int i = e - j * n;
             if (i >= j) {
+                // s = sum_{k >= i} L[k][i] Lbar[k][j], added in k order; the
+                // loads of eight terms go out before their additions
                 double s = 0.0;
-                for (int k = i; k < n; ++k) s = s + A1[k + i * ld] * (k == j ? dLb[j] : A1[j + k * ld]);
+                int k = i;
+                if (k == j) {  // the diagonal term (i == j): Lbar[j][j] sits beside the matrix
+                    s = s + A1[k + i * ld] * dLb[j];
+                    ++k;
+                }
+                for (; k + 8 <= n; k += 8) {
+                    double p[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) p[u] = A1[k + u + i * ld] * A1[j + (k + u) * ld];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) s = s + p[u];
+                }
+                for (; k < n; ++k) s = s + A1[k + i * ld] * A1[j + k * ld];
                 const double m = 0.5 * s;
                 W[i + (size_t)j * ldw] = m;
                 W[j + (size_t)i * ldw] = m;
@@ -196,18 +230,16 @@ __global__ __launch_bounds__(T) void k_potrf_backward(MfhCholAdjointBatch B, int
         for (int c = tid; c < n; c += T) {
             double* x = W + (size_t)c * ldw;
             for (int jj = n - 1; jj >= 0; --jj) {
-                const double y = x[jj] / A1[jj + jj * ld];
-                x[jj] = y;
-                for (int r = 0; r < jj; ++r) x[r] = x[r] - A1[jj + r * ld] * y;
+                const double d = mfh_dot_exact(n - 1 - jj, A1 + (jj + 1) + jj * ld, 1, x + jj + 1, 1);
+                x[jj] = (x[jj] - d) / A1[jj + jj * ld];
             }
         }
         __syncthreads();
         for (int r0 = tid; r0 < n; r0 += T) {
             double* x = W + r0;
             for (int jj = n - 1; jj >= 0; --jj) {
-                const double y = x[(size_t)jj * ldw] / A1[jj + jj * ld];
-                x[(size_t)jj * ldw] = y;
-                for (int r = 0; r < jj; ++r) x[(size_t)r * ldw] = x[(size_t)r * ldw] - A1[jj + r * ld] * y;
+                const double d = mfh_dot_exact(n - 1 - jj, A1 + (jj + 1) + jj * ld, 1, x + (size_t)(jj + 1) * ldw, ldw);
+                x[(size_t)jj * ldw] = (x[(size_t)jj * ldw] - d) / A1[jj + jj * ld];
             }
         }
         __syncthreads();

@@ -462,10 +462,19 @@ void mfo_dposv_backward(int n, const double* l, int lda, const double* x, const 
  * follows exactly:
  *   M[i][j] = M[j][i] = 0.5 * s, s = sum over k = i..n-1 of L[k][i] Lbar[k][j]
  *       (i >= j, s from 0.0 upward in k);
- *   every column of M: Lower/Trans solve (MatFlat's axpy form,
- *       Blas.SolveTriangular.cs:362-371) -> X = L^-T M;
- *   every row of X: the same solve on the row (stride ldab) -> X L^-1;
+ *   every column x of M: L^T x = m solved in dot form, j = n-1 down to 0:
+ *       x[j] = (x[j] - Dot(n-1-j, L[j+1..][j], x[j+1..])) / L[j][j] with
+ *       Internals.Dot's pairing (Internals.cs:265-292) -> X = L^-T M;
+ *   every row of X: the same solve along the row (stride ldab) -> X L^-1;
  *   Abar[i][j] = Abar[j][i] = 0.5 * (Abar[i][j] + Abar[j][i]) for i > j. */
+static void lt_solve_dot(int n, const double* l, int ldl, double* x, int incx)
+{
+    for (int j = n - 1; j >= 0; j--) {
+        const double d = mfo_ddot(n - 1 - j, l + (j + 1) + (size_t)j * ldl, 1, x + (size_t)(j + 1) * incx, incx);
+        x[(size_t)j * incx] = (x[(size_t)j * incx] - d) / l[j + (size_t)j * ldl];
+    }
+}
+
 void mfo_dpotrf_backward(int n, const double* l, int ldl, const double* lbar, int ldlb, double* abar, int ldab)
 {
     for (int j = 0; j < n; j++) {
@@ -477,8 +486,8 @@ void mfo_dpotrf_backward(int n, const double* l, int ldl, const double* lbar, in
             abar[j + (size_t)i * ldab] = m;
         }
     }
-    for (int c = 0; c < n; c++) mfo_dtrsv(MFO_LOWER, MFO_TRANS, n, l, ldl, abar + (size_t)c * ldab, 1);
-    for (int r = 0; r < n; r++) mfo_dtrsv(MFO_LOWER, MFO_TRANS, n, l, ldl, abar + r, ldab);
+    for (int c = 0; c < n; c++) lt_solve_dot(n, l, ldl, abar + (size_t)c * ldab, 1);
+    for (int r = 0; r < n; r++) lt_solve_dot(n, l, ldl, abar + r, ldab);
     for (int j = 0; j < n; j++) {
         for (int i = j + 1; i < n; i++) {
             const double v = 0.5 * (abar[i + (size_t)j * ldab] + abar[j + (size_t)i * ldab]);
