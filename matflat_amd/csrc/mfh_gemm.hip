@@ -712,6 +712,146 @@ __global__ __launch_bounds__(64 * FW) void k_gemm_mfma(MfhGemmBatch B, int count
     }
 }
 
+// ---- FAST mode, m, n <= 128: one workgroup per product ----------------------
+// The tile kernel above has each wavefront read its own 32-row panel of op(A)
+// and 32-column panel of op(B), so a product with mt x nt tiles reads its
+// operands about 2 mt nt / (mt + nt) times from HBM (2.7x the algorithmic
+// bytes measured on m, n, k ~ U{8..128}).  Here one workgroup of four
+// wavefronts owns a whole product: chunks of PKC k-values of op(A) (m x PKC)
+// and op(B) (PKC x n) are loaded once, in coalesced runs along whichever
+// index is contiguous in memory, into LDS (k-major, as the MFMA fragments
+// want them), and the wavefronts take the product's 32x32 tiles of C
+// round-robin (at most PQ each), keeping their accumulators across
+// chunks.  The next chunk's loads are in flight during the current chunk's
+// MFMAs.  Values past k are stored as zero; LDS rows past m or n are never
+// combined into a stored element.
+constexpr int PW = 8;          // wavefronts per workgroup
+constexpr int PKC = 16;        // k values per chunk
+constexpr int PMAX = 128;      // largest m and n of this path
+constexpr int PLD = PMAX + 4;  // LDS row length (doubles)
+constexpr int PPER = PMAX * PKC / (64 * PW);  // chunk elements per thread and operand
+constexpr int PQ = (PMAX / 32) * (PMAX / 32) / PW;  // tiles per wavefront
+
+// Element e of a chunk, over a fixed frame of PMAX rows (shifts, no
+// division): op(A)(r, kk) is read along r for NoTrans (A[r + kk lda]) and
+// along kk for Trans (A[kk + r lda]); op(B)(kk, c) along kk for NoTrans
+// (B[kk + c ldb]) and along c for Trans (B[c + kk ldb]).  Lanes outside the
+// matrix or past k load nothing and store zero.
+__device__ __forceinline__ void chunk_pos(bool kfast, int e, int& r, int& kk)
+{
+    if (kfast) {
+        kk = e & (PKC - 1);
+        r = e / PKC;
+    } else {
+        r = e & (PMAX - 1);
+        kk = e / PMAX;
+    }
+}
+
+__global__ __launch_bounds__(64 * PW) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_gemm_prod(
+    MfhGemmBatch B, int count, const int* cnt)
+{
+    static_assert((PKC & (PKC - 1)) == 0 && (PMAX & (PMAX - 1)) == 0, "shift mapping");
+    __shared__ double As[PKC * PLD];
+    __shared__ double Bs[PKC * PLD];
+    typedef const __attribute__((address_space(1))) double* GSrc;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    for (int p = blockIdx.x; p < count; p += gridDim.x) {
+        if (cnt[p] == 0) continue;  // invalid: info written by the check
+        const int m = B.m[p], n = B.n[p], k = B.k[p];
+        const bool ta = B.transa[p] == MFH_TRANS, tb = B.transb[p] == MFH_TRANS;
+        const int lda = B.lda[p], ldb = B.ldb[p];
+        const double* A = B.a[p];
+        const double* Bm = B.b[p];
+        const int mt = (m + 31) >> 5, nt = (n + 31) >> 5;
+        const int T = mt * nt;
+        double ra[PPER], rb[PPER];
+        auto load = [&](int k0) {
+#pragma unroll
+            for (int u = 0; u < PPER; ++u) {
+                const int e = tid + u * 64 * PW;
+                int r, kk, c, kb;
+                chunk_pos(ta, e, r, kk);
+                chunk_pos(!tb, e, c, kb);
+                const bool ina = r < m && k0 + kk < k;
+                const bool inb = c < n && k0 + kb < k;
+                ra[u] = ina ? ((GSrc)A)[ta ? (long)(k0 + kk) + (long)r * lda : (long)r + (long)(k0 + kk) * lda] : 0.0;
+                rb[u] = inb ? ((GSrc)Bm)[tb ? (long)c + (long)(k0 + kb) * ldb : (long)(k0 + kb) + (long)c * ldb] : 0.0;
+            }
+        };
+        mfh_d4 acc[PQ][2][2];
+#pragma unroll
+        for (int q = 0; q < PQ; ++q)
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) acc[q][i][j] = mfh_d4{0.0, 0.0, 0.0, 0.0};
+        const int nchunks = (k + PKC - 1) / PKC;
+        load(0);
+        for (int ch = 0; ch < nchunks; ++ch) {
+            __syncthreads();  // every wavefront is done with the previous chunk
+#pragma unroll
+            for (int u = 0; u < PPER; ++u) {
+                const int e = tid + u * 64 * PW;
+                int r, kk, c, kb;
+                chunk_pos(ta, e, r, kk);
+                chunk_pos(!tb, e, c, kb);
+                As[kk * PLD + r] = ra[u];
+                Bs[kb * PLD + c] = rb[u];
+            }
+            __syncthreads();
+            if (ch + 1 < nchunks) load((ch + 1) * PKC);
+#pragma unroll
+            for (int q = 0; q < PQ; ++q) {
+                const int t = wave + PW * q;
+                if (t < T) {
+                    const int i0 = (t % mt) * 32, j0 = (t / mt) * 32;
+                    const bool i1 = i0 + 16 < m, j1 = j0 + 16 < n;
+#pragma unroll
+                    for (int ks = 0; ks < PKC / 4; ++ks) {
+                        const int row = (ks * 4 + (lane >> 4)) * PLD + (lane & 15);
+                        const double a0 = As[row + i0], a1 = As[row + i0 + 16];
+                        const double b0 = Bs[row + j0], b1 = Bs[row + j0 + 16];
+                        acc[q][0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a0, acc[q][0][0], 0, 0, 0);
+                        if (j1) acc[q][0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a0, acc[q][0][1], 0, 0, 0);
+                        if (i1) acc[q][1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a1, acc[q][1][0], 0, 0, 0);
+                        if (i1 && j1)
+                            acc[q][1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a1, acc[q][1][1], 0, 0, 0);
+                    }
+                }
+            }
+        }
+        const double alpha = B.alpha[p], beta = B.beta[p];
+        double* C = B.c[p];
+        const long ldc = B.ldc[p];
+#pragma unroll
+        for (int q = 0; q < PQ; ++q) {
+            const int t = wave + PW * q;
+            if (t < T) {
+                const int i0 = (t % mt) * 32, j0 = (t / mt) * 32;
+#pragma unroll
+                for (int i = 0; i < 2; ++i) {
+                    const int gi = i0 + 16 * i + (lane & 15);
+#pragma unroll
+                    for (int j = 0; j < 2; ++j)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            const int gj = j0 + 16 * j + (lane >> 4) + 4 * r;
+                            if (gi < m && gj < n) {
+                                double* cp = C + gi + gj * ldc;
+                                double v = alpha * acc[q][i][j][r];
+                                if (beta != 0.0) v = __builtin_fma(beta, *cp, v);
+                                *cp = v;
+                            }
+                        }
+                }
+            }
+        }
+    }
+}
+
 }  // namespace
 
 // Workspace layout of one GEMM call (ints): per-product tile counts, chunk
@@ -736,6 +876,17 @@ static GemmWsLayout gemm_layout(int count, int max_m, int max_n, bool fast)
     return L;
 }
 
+// MFH_GEMM_TILES=1 in the environment keeps FAST FP64 products with m, n <=
+// 128 on the tile kernel (for comparing the two).
+static bool prod_kernel_off()
+{
+    static const bool off = [] {
+        const char* e = getenv("MFH_GEMM_TILES");
+        return e != nullptr && e[0] == '1';
+    }();
+    return off;
+}
+
 size_t mfh_gemm_ws_bytes(int count, int max_m, int max_n, bool fast)
 {
     return gemm_layout(count, max_m, max_n, fast).bytes;
@@ -758,6 +909,18 @@ int mfh_launch_gemm(mfh_handle* h, int count, const MfhGemmBatch& B, int max_m, 
     hipLaunchKernelGGL(k_gemm_validate_scan, dim3(nchunks), dim3(SCAN_T), 0, h->stream, B, count, ws, max_m, max_n,
                        cplx ? 1 : 0, ts);
     MFH_HIP_CHECK(hipGetLastError());
+    if (fast && !cplx && max_m <= PMAX && max_n <= PMAX && !prod_kernel_off()) {
+        // every product fits one workgroup's LDS chunks: no tile list needed
+        int per_cu = 0;
+        MFH_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_gemm_prod, 64 * PW, 0));
+        if (per_cu < 1) per_cu = 1;
+        long grid = (long)per_cu * h->num_cus;
+        if (grid > count) grid = count;
+        hipLaunchKernelGGL(k_gemm_prod, dim3((unsigned)grid), dim3(64 * PW), 0, h->stream, B, count,
+                           (const int*)ws.cnt);
+        MFH_HIP_CHECK(hipGetLastError());
+        return MFH_OK;
+    }
     hipLaunchKernelGGL(k_gemm_map, dim3(nchunks), dim3(SCAN_T), 0, h->stream, ws, count, nchunks);
     MFH_HIP_CHECK(hipGetLastError());
     if (fast) {
