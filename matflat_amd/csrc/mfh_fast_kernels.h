@@ -91,8 +91,7 @@ struct FastLds {
     static constexpr int NR = 8 * RB;
     static constexpr int PB = PLD * NR;        // one panel buffer
     static constexpr int NPB = NW > 1 ? 2 : 1;  // double-buffered across wavefronts
-    static constexpr int dgb = NPB * PB;       // [RB][8][8] diagonal blocks of -L
-    static constexpr int rdg = dgb + 64 * RB;  // [NR] 1 / L[j][j]
+    static constexpr int rdg = NPB * PB;       // [NR] 1 / L[j][j]
     static constexpr int yv = rdg + NR;        // [NR] y = L^-1 b (SPD solve)
     static constexpr int xw = yv + NR;         // [NW][NR] each wave's copy of x
     static constexpr int part = xw + NW * NR;  // [2][NW][8] block-column sums
@@ -194,13 +193,14 @@ __device__ __forceinline__ void trailing(AccT<RB, NW, W>& acc, const double* pr,
 // Unscaled right-looking steps with the pivot and the row values broadcast
 // by v_readlane; each column is scaled by 1/sqrt(d) as it finishes.  The
 // buffer holds the panel negated (-P) and receives -L: the negated rows back
-// and the diagonal block also to dgb[p] (-L_pp), and, for the SPD solve,
+// (rows 8p .. 8p + 7, the diagonal block -L_pp, stay in the buffer to the
+// end: later panels write only rows below them), and, for the SPD solve,
 // 1/L[j][j] in rdg and the negated row of b (row n: -y, y = L^-1 b) in yv.
 // Columns past n (nc <= t < 8) are run with a unit pivot: they hold zeros
 // and finite garbage only, and their rdg and yv entries are written as 0.
 // Returns the first failing row + 1 (or the fail passed in).
 template <int RB, int R, bool POSV>
-__device__ __forceinline__ int panel_factor(double* pb, double* dgb, double* rdg, double* yv, int p, int n, int lane,
+__device__ __forceinline__ int panel_factor(double* pb, double* rdg, double* yv, int p, int n, int lane,
                                             int fail)
 {
     constexpr int NR = 8 * RB;
@@ -251,10 +251,8 @@ __device__ __forceinline__ int panel_factor(double* pb, double* dgb, double* rdg
             }
         }
     }
-    if (lane < 8) {
-        if constexpr (POSV) rdg[8 * p + lane] = ymine;
-#pragma unroll
-        for (int t = 0; t < 8; ++t) dgb[64 * p + 8 * lane + t] = v[0][t];
+    if constexpr (POSV) {
+        if (lane < 8) rdg[8 * p + lane] = ymine;
     }
     return fail;
 }
@@ -265,7 +263,7 @@ __device__ __forceinline__ int panel_factor(double* pb, double* dgb, double* rdg
 // padding rows are not part of A.  offl = i0 + j*lda and offu = j + i0*lda
 // for the lane's row i0 = lr of block row 0 and column j = 8P + lc.
 template <int RB, int NW, int W, int P>
-__device__ __forceinline__ void store_panel_p(GPtr A, int lda, const double* pr, const double* dg, int offl, int offu,
+__device__ __forceinline__ void store_panel_p(GPtr A, int lda, const double* pr, int offl, int offu,
                                               int nrow, bool jin, int lr, int lc)
 {
     using O = Own<RB, NW, W>;
@@ -278,8 +276,9 @@ __device__ __forceinline__ void store_panel_p(GPtr A, int lda, const double* pr,
                 gstore(A, offu + 8 * a * lda, 0.0);
             }
         } else if (a == P) {
-            // the diagonal block, from its copy dg = dgb[P] (element (lr, lc))
-            if (jin && 8 * a < nrow) gstore(A, offl + 8 * a, lr >= lc ? -dg[8 * lr + lc] : 0.0);
+            // the diagonal block (element (lr, lc)); above its diagonal the
+            // buffer holds leftovers of the unscaled steps
+            if (jin && 8 * a < nrow) gstore(A, offl + 8 * a, lr >= lc ? -pr[PLD * 8 * a] : 0.0);
         }
     }
 }
@@ -327,30 +326,51 @@ __device__ __forceinline__ double colsum_p(const AccT<RB, NW, W>& acc, const dou
     return s;
 }
 
-// Pointer read from a pointer array at a uniform index, made provably
-// uniform so that it sits in scalar registers.
-template <typename T>
-__device__ __forceinline__ T* uniform_ptr(T* p)
+// Reads through the constant address space: the size, pointer and queue
+// arrays are not written while a kernel runs, so uniform indices give
+// scalar loads and the values land in scalar registers.
+__device__ __forceinline__ int cload_i(const int* p, int i)
 {
-    const unsigned long long v = (unsigned long long)p;
-    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
-    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
-    return (T*)(((unsigned long long)hi << 32) | lo);
+    return ((const __attribute__((address_space(4))) int*)p)[i];
+}
+__device__ __forceinline__ GPtr cload_p(double* const* p, int i)
+{
+    return (GPtr)((double* const __attribute__((address_space(4)))*)p)[i];
+}
+
+// One matrix's order, leading dimension and pointers (x: SPD solve only).
+struct FastMeta {
+    int n, lda;
+    GPtr a, x;
+};
+
+template <bool POSV>
+__device__ __forceinline__ FastMeta fast_meta(const MfhCholBatch& B, int b)
+{
+    FastMeta m{0, 0, nullptr, nullptr};
+    if (b >= 0) {
+        m.n = cload_i(B.n, b);
+        m.lda = cload_i(B.lda, b);
+        m.a = cload_p(B.a, b);
+        if constexpr (POSV) m.x = cload_p(B.x, b);
+    }
+    return m;
 }
 
 // One matrix, wave W's share.
 template <int RB, int NW, int W, bool POSV>
-__device__ __forceinline__ void fast_matrix(const MfhCholBatch& B, double* lds, int b, int lane0)
+__device__ __forceinline__ void fast_matrix(const MfhCholBatch& B, const FastMeta& M, const FastMeta& MN, double* lds,
+                                            int b, int lane0)
 {
     using O = Own<RB, NW, W>;
     using LY = FastLds<RB, NW>;
     constexpr int NR = 8 * RB;
     const int lr0 = lane0 & 7, lc0 = lane0 >> 3;
-    const int n = __builtin_amdgcn_readfirstlane(B.n[b]);
-    const int lda = __builtin_amdgcn_readfirstlane(B.lda[b]);
+    const int n = M.n;
+    const int lda = M.lda;
     const int rows = POSV ? n + 1 : n;
     const int np = (n + 7) >> 3;  // panels
-    GPtr A = (GPtr)uniform_ptr(B.a[b]);
+    GPtr A = M.a;
     double* rdg = lds + LY::rdg;
     double* yv = lds + LY::yv;
 
@@ -383,7 +403,7 @@ __device__ __forceinline__ void fast_matrix(const MfhCholBatch& B, double* lds, 
     }
     if constexpr (POSV) {
         // Row n of the triangle holds b (lanes with lr == n % 8, block n / 8).
-        GPtr X = (GPtr)uniform_ptr(B.x[b]);
+        GPtr X = M.x;
         const bool rown = lr == (n & 7);
 #pragma unroll
         for (int m = 0; m < O::na; ++m) {
@@ -402,12 +422,33 @@ __device__ __forceinline__ void fast_matrix(const MfhCholBatch& B, double* lds, 
     const int nrow = n - lr, ncol = n - lc;
 
     int fail = 0;
+    // Prefetch of the next matrix's upper triangle into the caches: one
+    // dword from every 128-byte line (rows 0, 16, 32, ... and row j of each
+    // column j), issued once this matrix's first column block has arrived,
+    // so that its HBM reads overlap this factorization instead of joining
+    // the next matrix's burst of tile loads.  The values are only kept
+    // alive until this matrix ends.
+    constexpr bool PF = W == 0 && RB >= 8 && RB <= 11;
+    unsigned pf[PF ? RB : 1];
+    // panel 0's blocks go to the buffer before the loop, so that the
+    // prefetch follows the arrival of this matrix's first column block
+    publish_p<RB, NW, W, 0>(acc, lds + PLD * lr + lc);
+    if constexpr (PF) {
+#pragma unroll
+        for (int u = 0; u < RB; ++u) {
+            const int slot = lane0 + 64 * u, j = slot >> 3, sl = slot & 7;
+            const int row = sl == 7 ? j : 16 * sl;
+            const bool on = j < MN.n && (sl == 7 || row <= j);
+            pf[u] = 0;
+            if (on) pf[u] = *(const __attribute__((address_space(1))) unsigned*)((GBytes)MN.a + (unsigned)((j * MN.lda + row) << 3));
+        }
+    }
     // not unrolled: one copy of the panel step keeps the code small
 #pragma clang loop unroll(disable)
     for (int p = 0; p < np; ++p) {
         double* pb = lds + (LY::NPB > 1 ? (p & 1) * LY::PB : 0);
         const double* pr = pb + PLD * lr + lc;  // rows 8a + lr, column lc (publish, stores)
-        for_panel<RB>(p, [&](auto P) { publish_p<RB, NW, W, decltype(P)::value>(acc, pb + PLD * lr + lc); });
+        if (p > 0) for_panel<RB>(p, [&](auto P) { publish_p<RB, NW, W, decltype(P)::value>(acc, pb + PLD * lr + lc); });
         __syncthreads();
         if constexpr (W == 0) {
             // the panel is the critical path of the whole matrix: its
@@ -416,11 +457,11 @@ __device__ __forceinline__ void fast_matrix(const MfhCholBatch& B, double* lds, 
             constexpr int RMAX = (NR + 63) / 64;
             const int rp = rows - 8 * p;  // rows of the panel
             if (RMAX >= 3 && rp > 128)
-                fail = panel_factor<RB, (RMAX >= 3 ? 3 : RMAX), POSV>(pb, lds + LY::dgb, rdg, yv, p, n, lane0, fail);
+                fail = panel_factor<RB, (RMAX >= 3 ? 3 : RMAX), POSV>(pb, rdg, yv, p, n, lane0, fail);
             else if (RMAX >= 2 && rp > 64)
-                fail = panel_factor<RB, (RMAX >= 2 ? 2 : 1), POSV>(pb, lds + LY::dgb, rdg, yv, p, n, lane0, fail);
+                fail = panel_factor<RB, (RMAX >= 2 ? 2 : 1), POSV>(pb, rdg, yv, p, n, lane0, fail);
             else
-                fail = panel_factor<RB, 1, POSV>(pb, lds + LY::dgb, rdg, yv, p, n, lane0, fail);
+                fail = panel_factor<RB, 1, POSV>(pb, rdg, yv, p, n, lane0, fail);
             if (lane0 == 0) lds[LY::flag] = (double)fail;
             __builtin_amdgcn_s_setprio(0);
         }
@@ -430,7 +471,7 @@ __device__ __forceinline__ void fast_matrix(const MfhCholBatch& B, double* lds, 
         const int offl = offl0 + 8 * p * lda, offu = offu0 + 8 * p;
         const bool jin = 8 * p < ncol;
         for_panel<RB>(p, [&](auto P) {
-            store_panel_p<RB, NW, W, decltype(P)::value>(A, lda, pr, lds + LY::dgb + 64 * decltype(P)::value, offl,
+            store_panel_p<RB, NW, W, decltype(P)::value>(A, lda, pr, offl,
                                                          offu, nrow, jin, lr, lc);
         });
         trailing<RB, NW, W>(acc, pb + PLD * lr, pb + PLD * lc, p);
@@ -439,6 +480,12 @@ __device__ __forceinline__ void fast_matrix(const MfhCholBatch& B, double* lds, 
         // columns of L before the failing panel have been written
         if (W == 0 && lane0 == 0) B.info[b] = fail;
         return;
+    }
+    if constexpr (PF) {
+        unsigned sink = 0;
+#pragma unroll
+        for (int u = 0; u < RB; ++u) sink ^= pf[u];
+        asm volatile("" ::"v"(sink));
     }
     if (W == 0 && lane0 == 0) B.info[b] = 0;
     if constexpr (POSV) {
@@ -450,7 +497,6 @@ __device__ __forceinline__ void fast_matrix(const MfhCholBatch& B, double* lds, 
         // a chain of dependent steps per block column
         __builtin_amdgcn_s_setprio(2);
         for (int i = lane0; i < NR; i += 64) xw[i] = 0.0;
-        const double* dgb = lds + LY::dgb;
         double* part = lds + LY::part;
         const int t = lane0 & 7;
 #pragma clang loop unroll(disable)
@@ -466,13 +512,15 @@ __device__ __forceinline__ void fast_matrix(const MfhCholBatch& B, double* lds, 
 #pragma unroll
             for (int w = 0; w < NW; ++w) z += pk[8 * w + t];
             const double rd = rdg[8 * K + t];
+            // -L_KK: rows 8K .. 8K + 7 of panel K's buffer
+            const double* dg = lds + (LY::NPB > 1 ? (K & 1) * LY::PB : 0) + PLD * 8 * K;
             double lrow[8], lcol[8];
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
-                lrow[k] = dgb[64 * K + 8 * t + k];  // -L[t][k]
-                lcol[k] = dgb[64 * K + 8 * k + t];  // -L[k][t]
+                lrow[k] = dg[PLD * t + k];  // -L[t][k]
+                lcol[k] = dg[PLD * k + t];  // -L[k][t]
             }
-            // The tile, dgb and yv hold negated values (-P, -L_KK, -y), so z
+            // The tile, dg and yv hold negated values (-P, -L_KK, -y), so z
             // here is -z and the forward sweep gives -w; then
             // y - w = (-w) - (-y).  Columns past n have rd = 0 and yv = 0,
             // so their x is 0.
@@ -495,20 +543,21 @@ __device__ __forceinline__ void fast_matrix(const MfhCholBatch& B, double* lds, 
         }
         __builtin_amdgcn_s_setprio(0);
         if constexpr (W == 0) {
-            GPtr Xo = (GPtr)uniform_ptr(B.x[b]);
+            GPtr Xo = M.x;
             for (int r = lane0; r < n; r += 64) gstore(Xo, r, xw[r]);
         }
     }
 }
 
 template <int RB, int NW, int W, bool POSV>
-__device__ __forceinline__ void fast_wave(const MfhCholBatch& B, double* lds, int b, int lane, int w)
+__device__ __forceinline__ void fast_wave(const MfhCholBatch& B, const FastMeta& M, const FastMeta& MN, double* lds,
+                                          int b, int lane, int w)
 {
     if constexpr (W + 1 < NW) {
-        if (w == W) fast_matrix<RB, NW, W, POSV>(B, lds, b, lane);
-        else fast_wave<RB, NW, W + 1, POSV>(B, lds, b, lane, w);
+        if (w == W) fast_matrix<RB, NW, W, POSV>(B, M, MN, lds, b, lane);
+        else fast_wave<RB, NW, W + 1, POSV>(B, M, MN, lds, b, lane, w);
     } else {
-        fast_matrix<RB, NW, W, POSV>(B, lds, b, lane);
+        fast_matrix<RB, NW, W, POSV>(B, M, MN, lds, b, lane);
     }
 }
 
@@ -517,7 +566,7 @@ __device__ __forceinline__ void fast_wave(const MfhCholBatch& B, double* lds, in
 template <int RB, int NW>
 __host__ __device__ constexpr int fast_min_waves()
 {
-    return 1;
+    return (RB == 13 && NW == 1) ? 2 : 1;
 }
 
 template <int RB, int NW, bool POSV>
@@ -526,13 +575,29 @@ __global__ __launch_bounds__(64 * NW, (fast_min_waves<RB, NW>())) void k_chol_fa
     extern __shared__ double lds[];
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    for (int it = blockIdx.x;; it += gridDim.x) {
-        // One matrix per workgroup: its id and sizes are uniform, which keeps
-        // every panel and block test below a scalar branch.
-        const int b = __builtin_amdgcn_readfirstlane(mfh_item(q, it));
-        if (b < 0) break;
-        fast_wave<RB, NW, 0, POSV>(B, lds, b, lane, w);
+    // One matrix per workgroup: its id and sizes are uniform, which keeps
+    // every panel and block test below a scalar branch.  The ids and sizes
+    // are read a matrix ahead: the scalar loads of the next matrix's id and
+    // of the following one's sizes and pointers go out beside this matrix's
+    // tile loads, so a matrix starts with its tile loads instead of a chain
+    // of dependent reads (queue extent, id, sizes and pointers, tile).
+    const int qn = q.items ? cload_i(q.count, 0) : q.direct;
+    const int q0 = q.items ? cload_i(q.start, 0) : 0;
+    const int G = gridDim.x;
+    auto item = [&](int idx) -> int { return idx < qn ? (q.items ? cload_i(q.items, q0 + idx) : idx) : -1; };
+    int it = blockIdx.x;
+    int b = item(it);
+    FastMeta m = fast_meta<POSV>(B, b);
+    int bn = item(it + G);
+    while (b >= 0) {
+        const FastMeta mn = fast_meta<POSV>(B, bn);
+        const int bnn = item(it + 2 * G);
+        fast_wave<RB, NW, 0, POSV>(B, m, mn, lds, b, lane, w);
         __syncthreads();
+        b = bn;
+        m = mn;
+        bn = bnn;
+        it += G;
     }
 }
 
