@@ -40,6 +40,10 @@
 namespace {
 
 constexpr int BT = 512;  // threads per workgroup
+// Workspace header (ints): [0] factorization counter, [1, 103) active
+// counts of up to 101 steps, [128, 229) the steps' work counters, then the
+// order histogram and cursors.
+constexpr int kGesvStepOff = 128, kGesvHistOff = 256, kGesvCursorOff = 384, kGesvHeaderInts = 512;
 constexpr int NWAVE = BT / 64;
 
 typedef float float4_t __attribute__((ext_vector_type(4)));
@@ -77,8 +81,10 @@ struct GesvArgs {
     char* ws;          // per-matrix workspaces, wstride bytes apart
     long long wstride;
     int ldw;           // row stride of W (elements)
-    int* counter;      // work counter (zeroed before launch)
+    int* counter;      // work counter of the factorization (zeroed before launch)
     int* active;       // [max_iter + 2]: matrices still active entering step it (zeroed before launch)
+    int* stepc;        // [max_iter + 1]: work counter of each solve step (zeroed before launch)
+    int* order;        // [count]: matrix ids by decreasing order n (8-row buckets)
     // state kept between the factorization and the refinement launches
     int* perm;         // [count][ldw] position -> original row of A
     double* resid;     // [count][ldw] last residual b - A x
@@ -1201,7 +1207,10 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(MFH_GESV_WAV
     const Lds<T> L = carve<T>(lds_raw, G.ldw);
     const int t = threadIdx.x;
     for (;;) {
-        if (t == 0) *L.flag = atomicAdd(G.counter, 1);
+        if (t == 0) {
+            const int k = atomicAdd(G.counter, 1);
+            *L.flag = k < G.count ? G.order[k] : G.count;
+        }
         __syncthreads();
         const int bi = __builtin_amdgcn_readfirstlane(*L.flag);
         __syncthreads();
@@ -1256,8 +1265,10 @@ __device__ __forceinline__ Lds<T> carve_step(char* base, int ldw)
     return L;
 }
 
-// Solve step `it` for every matrix still active; workgroups walk the batch
-// with a fixed stride.  The solve and the residual are bound by memory
+// Solve step `it` for every matrix still active; workgroups take the
+// matrices largest first through the step's work counter, so the launch
+// ends with the small ones (a fixed stride over the batch order left some
+// workgroups with several times the work of others).  The solve and the residual are bound by memory
 // latency: this kernel is held to MFH_STEP_WAVES wavefronts per SIMD (80
 // VGPRs at 6, no spills), so that three workgroups share a CU and their
 // loads overlap.
@@ -1273,7 +1284,18 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(MFH_STEP_WAV
     // previous step counted the matrices it left active; every workgroup
     // reads the same count)
     if (it > 0 && __builtin_amdgcn_readfirstlane(*(volatile int*)&G.active[it]) == 0) return;
-    for (int bi = blockIdx.x; bi < G.count; bi += gridDim.x) {
+    int* slot = (int*)L.rkey;  // free between matrices
+    for (;;) {
+        if (threadIdx.x == 0) {
+            // the matrix at the next position of the order (one that has
+            // already finished is skipped below)
+            const int k = atomicAdd(&G.stepc[it], 1);
+            *slot = k < G.count ? G.order[k] : G.count;
+        }
+        __syncthreads();
+        const int bi = __builtin_amdgcn_readfirstlane(*slot);
+        __syncthreads();
+        if (bi >= G.count) break;
         const int active = __builtin_amdgcn_readfirstlane(G.state[bi]) == 0;
         if (active) {
             const int n = __builtin_amdgcn_readfirstlane(G.n[bi]);
@@ -1284,8 +1306,60 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(MFH_STEP_WAV
     }
 }
 
-// Zeroes the work counter and the per-step active counts (a kernel, not a
-// memset: the launch sequence may be captured into a graph).
+// Largest-first order of the batch for the factorization and solve
+// launches: a counting sort of the matrix ids by n / 8 (descending), in
+// three small kernels (histogram, offsets, scatter) with LDS-aggregated
+// atomics.  Invalid orders sort last; the kernels report them.
+constexpr int kOrderKeys = MFH_BLOCKED_MAX_N / 8 + 1;
+__device__ __forceinline__ int order_key(const int* n, int i)
+{
+    const int v = n[i];
+    return (v >= 1 && v <= MFH_BLOCKED_MAX_N) ? v >> 3 : 0;
+}
+
+__global__ __launch_bounds__(256) void k_order_hist(const int* n, int count, int* hist)
+{
+    __shared__ int hl[kOrderKeys];
+    for (int k = threadIdx.x; k < kOrderKeys; k += blockDim.x) hl[k] = 0;
+    __syncthreads();
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) atomicAdd(&hl[order_key(n, i)], 1);
+    __syncthreads();
+    for (int k = threadIdx.x; k < kOrderKeys; k += blockDim.x)
+        if (hl[k]) atomicAdd(&hist[k], hl[k]);
+}
+
+__global__ void k_order_offsets(const int* hist, int* cursor)
+{
+    if (threadIdx.x == 0) {
+        int run = 0;
+        for (int k = kOrderKeys - 1; k >= 0; --k) {
+            cursor[k] = run;
+            run += hist[k];
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_order_scatter(const int* n, int count, int* cursor, int* order)
+{
+    __shared__ int hl[kOrderKeys];
+    __shared__ int base[kOrderKeys];
+    for (int k = threadIdx.x; k < kOrderKeys; k += blockDim.x) hl[k] = 0;
+    __syncthreads();
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int key = i < count ? order_key(n, i) : -1;
+    int pos = -1;
+    if (key >= 0) pos = atomicAdd(&hl[key], 1);
+    __syncthreads();
+    for (int k = threadIdx.x; k < kOrderKeys; k += blockDim.x)
+        if (hl[k]) base[k] = atomicAdd(&cursor[k], hl[k]);
+    __syncthreads();
+    if (key >= 0) order[base[key] + pos] = i;
+}
+
+// Zeroes the work counters, the per-step active counts and the order
+// histogram (a kernel, not a memset: the launch sequence may be captured
+// into a graph).
 __global__ void k_zero_counter(int* c, int n)
 {
     for (int i = threadIdx.x; i < n; i += blockDim.x) c[i] = 0;
@@ -1341,7 +1415,16 @@ int launch_gesv(mfh_handle* h, GesvArgs G)
     if (per_cu < 1) per_cu = 1;
     long grid = (long)per_cu * h->num_cus;
     if (grid > G.count) grid = G.count;
-    hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(128), 0, h->stream, G.counter, 2 + G.max_iter + 1);
+    hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(256), 0, h->stream, G.counter, kGesvHeaderInts);
+    {
+        int* hist = G.counter + kGesvHistOff;
+        int* cursor = G.counter + kGesvCursorOff;
+        const unsigned blocks = (unsigned)((G.count + 255) / 256);
+        hipLaunchKernelGGL(k_order_hist, dim3(blocks), dim3(256), 0, h->stream, G.n, G.count, hist);
+        hipLaunchKernelGGL(k_order_offsets, dim3(1), dim3(64), 0, h->stream, hist, cursor);
+        hipLaunchKernelGGL(k_order_scatter, dim3(blocks), dim3(256), 0, h->stream, G.n, G.count, cursor, G.order);
+        MFH_HIP_CHECK(hipGetLastError());
+    }
     int st = mfh_timing_begin(h, MIXED ? MFH_KERNEL_GESV_MIXED : MFH_KERNEL_GESV_F64, G.max_n);
     if (st) return st;
     hipLaunchKernelGGL((k_gesv<T, MIXED>), dim3((unsigned)grid), dim3(BT), lds, h->stream, G);
@@ -1384,7 +1467,7 @@ int launch_gesv(mfh_handle* h, GesvArgs G)
 
 namespace {
 struct GesvLayout {
-    size_t per, w, perm, resid, norms, zero, state, total;
+    size_t per, w, perm, resid, norms, zero, state, order, total;
 };
 
 GesvLayout gesv_layout(int count, int max_n, bool mixed)
@@ -1393,13 +1476,14 @@ GesvLayout gesv_layout(int count, int max_n, bool mixed)
     const size_t ldw = (size_t)((max_n + 7) & ~7);
     GesvLayout g;
     g.per = up(ldw * max_n * (mixed ? 4 : 8));
-    g.w = 512;  // work counter, then the active counts of up to 101 steps
+    g.w = sizeof(int) * kGesvHeaderInts;  // counters and the order histogram
     g.perm = g.w + g.per * (size_t)count;
     g.resid = g.perm + up(sizeof(int) * ldw * count);
     g.norms = g.resid + up(sizeof(double) * ldw * count);
     g.zero = g.norms + up(sizeof(double) * 2 * count);
     g.state = g.zero + up(sizeof(int) * count);
-    g.total = g.state + up(sizeof(int) * count);
+    g.order = g.state + up(sizeof(int) * count);
+    g.total = g.order + up(sizeof(int) * count);
     return g;
 }
 }  // namespace
@@ -1431,6 +1515,8 @@ static int launch_gesv_common(mfh_handle* h, int count, const int* n, const doub
     char* base = (char*)h->ws;
     G.counter = (int*)base;
     G.active = (int*)base + 1;
+    G.stepc = (int*)base + kGesvStepOff;
+    G.order = (int*)(base + lay.order);
     G.ws = base + lay.w;
     G.wstride = (long long)lay.per;
     G.ldw = ldw;
