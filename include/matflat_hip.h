@@ -108,7 +108,7 @@ int mfh_set_chol_shape(mfh_handle* handle, int bucket, int lanes, int rows_per_l
 
 /* Concurrency of the FAST-mode Cholesky / SPD-solve size buckets (no MatFlat
  * counterpart): the per-bucket launches are forked over `streams` (1..4,
- * default 3) streams of default priority that the handle owns, largest
+ * default 2) streams of default priority that the handle owns, largest
  * bucket first, and joined back into the handle's stream, so a bucket's
  * last partial round overlaps the next bucket.  1 keeps them in order on
  * the handle's stream.  With per-launch timing on (mfh_set_timing) the

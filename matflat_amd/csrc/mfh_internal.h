@@ -47,7 +47,7 @@ struct mfh_handle {
     int fast_waves[18] = {};              // FAST Cholesky waves per matrix by row blocks (0 = default)
     int chol_shape[MFH_NBUCKETS][2] = {}; // exact Cholesky: lanes per matrix, rows per lane, by bucket (0 = default)
     bool timing = false;                  // bracket launches with events
-    int bucket_streams = 3;               // FAST bucket launches forked over this many streams (1 = in order)
+    int bucket_streams = 2;               // FAST bucket launches forked over this many streams (1 = in order)
     std::vector<hipStream_t> side;        // side streams of the fork (default priority), created on first use
     std::vector<hipEvent_t> side_ev;      // fork / join events, one per side stream plus one
     std::vector<MfhTimedLaunch> timed;    // recorded launches not yet read
