@@ -428,7 +428,9 @@ __device__ __forceinline__ void fast_matrix(const MfhCholBatch& B, const FastMet
     // so that its HBM reads overlap this factorization instead of joining
     // the next matrix's burst of tile loads.  The values are only kept
     // alive until this matrix ends.
-    constexpr bool PF = W == 0 && RB >= 8 && RB <= 11;
+    // (for 81..88 rows only: elsewhere its registers would cost a wavefront
+    // per SIMD, which measured slower than the prefetch gained)
+    constexpr bool PF = W == 0 && RB == 11;
     unsigned pf[PF ? RB : 1];
     // panel 0's blocks go to the buffer before the loop, so that the
     // prefetch follows the arrival of this matrix's first column block
@@ -561,12 +563,21 @@ __device__ __forceinline__ void fast_wave(const MfhCholBatch& B, const FastMeta&
     }
 }
 
-// Waves per SIMD the register allocation must leave room for: the smallest
-// allocation whose register-file step still fits the tile.
+// Waves per SIMD the register allocation must leave room for, for the
+// one-wavefront kernels whose natural allocation sits just above a step of
+// the 512-register file: a bucket runs as many matrices at once as its
+// wavefronts per SIMD, and its time is per-matrix latency.  Measured on
+// MI355X (each bucket alone, 100,000-solve batch): 41..48 rows 5 instead of
+// 4 (0.111 -> 0.102 ms), 57..64 rows 4 instead of 3, 65..72 rows 3 instead of
+// 2 (0.274 -> 0.240 ms, no spill at 168), 73..80 rows 3 (0.331 -> 0.296 ms,
+// 68 bytes spilled), 97..104 rows 2 instead of 1 (0.484 -> 0.335 ms); 17..24
+// rows 8.  Capping 25..32, 49..56 and 81..88 rows measured slower (their
+// spills cost more than the extra matrices gain).
 template <int RB, int NW>
 __host__ __device__ constexpr int fast_min_waves()
 {
-    return (RB == 13 && NW == 1) ? 2 : 1;
+    if constexpr (NW != 1) return 1;
+    return RB == 3 ? 8 : RB == 6 ? 5 : RB == 8 ? 4 : RB == 9 ? 3 : RB == 10 ? 3 : RB == 13 ? 2 : 1;
 }
 
 template <int RB, int NW, bool POSV>
