@@ -359,8 +359,7 @@ __device__ __forceinline__ FastMeta fast_meta(const MfhCholBatch& B, int b)
 
 // One matrix, wave W's share.
 template <int RB, int NW, int W, bool POSV>
-__device__ __forceinline__ void fast_matrix(const MfhCholBatch& B, const FastMeta& M, const FastMeta& MN, double* lds,
-                                            int b, int lane0)
+__device__ __forceinline__ void fast_matrix(const MfhCholBatch& B, const FastMeta& M, double* lds, int b, int lane0)
 {
     using O = Own<RB, NW, W>;
     using LY = FastLds<RB, NW>;
@@ -422,29 +421,10 @@ __device__ __forceinline__ void fast_matrix(const MfhCholBatch& B, const FastMet
     const int nrow = n - lr, ncol = n - lc;
 
     int fail = 0;
-    // Prefetch of the next matrix's upper triangle into the caches: one
-    // dword from every 128-byte line (rows 0, 16, 32, ... and row j of each
-    // column j), issued once this matrix's first column block has arrived,
-    // so that its HBM reads overlap this factorization instead of joining
-    // the next matrix's burst of tile loads.  The values are only kept
-    // alive until this matrix ends.
-    // (for 81..88 rows only: elsewhere its registers would cost a wavefront
-    // per SIMD, which measured slower than the prefetch gained)
-    constexpr bool PF = W == 0 && RB == 11;
-    unsigned pf[PF ? RB : 1];
-    // panel 0's blocks go to the buffer before the loop, so that the
-    // prefetch follows the arrival of this matrix's first column block
+    // panel 0's blocks go to the buffer before the loop (the loop publishes
+    // panels 1..: one fewer copy of the switch over the tile per iteration,
+    // and a few registers fewer)
     publish_p<RB, NW, W, 0>(acc, lds + PLD * lr + lc);
-    if constexpr (PF) {
-#pragma unroll
-        for (int u = 0; u < RB; ++u) {
-            const int slot = lane0 + 64 * u, j = slot >> 3, sl = slot & 7;
-            const int row = sl == 7 ? j : 16 * sl;
-            const bool on = j < MN.n && (sl == 7 || row <= j);
-            pf[u] = 0;
-            if (on) pf[u] = *(const __attribute__((address_space(1))) unsigned*)((GBytes)MN.a + (unsigned)((j * MN.lda + row) << 3));
-        }
-    }
     // not unrolled: one copy of the panel step keeps the code small
 #pragma clang loop unroll(disable)
     for (int p = 0; p < np; ++p) {
@@ -482,12 +462,6 @@ __device__ __forceinline__ void fast_matrix(const MfhCholBatch& B, const FastMet
         // columns of L before the failing panel have been written
         if (W == 0 && lane0 == 0) B.info[b] = fail;
         return;
-    }
-    if constexpr (PF) {
-        unsigned sink = 0;
-#pragma unroll
-        for (int u = 0; u < RB; ++u) sink ^= pf[u];
-        asm volatile("" ::"v"(sink));
     }
     if (W == 0 && lane0 == 0) B.info[b] = 0;
     if constexpr (POSV) {
@@ -552,14 +526,13 @@ __device__ __forceinline__ void fast_matrix(const MfhCholBatch& B, const FastMet
 }
 
 template <int RB, int NW, int W, bool POSV>
-__device__ __forceinline__ void fast_wave(const MfhCholBatch& B, const FastMeta& M, const FastMeta& MN, double* lds,
-                                          int b, int lane, int w)
+__device__ __forceinline__ void fast_wave(const MfhCholBatch& B, const FastMeta& M, double* lds, int b, int lane, int w)
 {
     if constexpr (W + 1 < NW) {
-        if (w == W) fast_matrix<RB, NW, W, POSV>(B, M, MN, lds, b, lane);
-        else fast_wave<RB, NW, W + 1, POSV>(B, M, MN, lds, b, lane, w);
+        if (w == W) fast_matrix<RB, NW, W, POSV>(B, M, lds, b, lane);
+        else fast_wave<RB, NW, W + 1, POSV>(B, M, lds, b, lane, w);
     } else {
-        fast_matrix<RB, NW, W, POSV>(B, M, MN, lds, b, lane);
+        fast_matrix<RB, NW, W, POSV>(B, M, lds, b, lane);
     }
 }
 
@@ -603,7 +576,7 @@ __global__ __launch_bounds__(64 * NW, (fast_min_waves<RB, NW>())) void k_chol_fa
     while (b >= 0) {
         const FastMeta mn = fast_meta<POSV>(B, bn);
         const int bnn = item(it + 2 * G);
-        fast_wave<RB, NW, 0, POSV>(B, m, mn, lds, b, lane, w);
+        fast_wave<RB, NW, 0, POSV>(B, m, lds, b, lane, w);
         __syncthreads();
         b = bn;
         m = mn;
