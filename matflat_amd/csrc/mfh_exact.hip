@@ -382,9 +382,15 @@ __global__ __launch_bounds__(128) void k_getrf_exact_w2(MfhGetrfBatch B, MfhQueu
         for (int k = 0; k < kmax; ++k) {
             const int i = k + lane;
             const bool act = i < m;
-            double sp = 0.0, p2 = 0.0;
+            double sp = 0.0, p2 = 0.0, aik = 0.0, xk1 = 0.0;
             if (w0) {
-                // all of the candidate's dot but its last pair
+                // all of the candidate's dot but its last pair, and the two
+                // entries of the row the rest needs (wavefront 1 writes only
+                // row k-1 before the barrier)
+                if (act) {
+                    aik = L[i + k * ml];
+                    if (k >= 1) xk1 = L[i + (k - 1) * ml];
+                }
                 if (act && k >= 2) {
                     sp = mfh_dot_exact(k - 2, L + i, ml, L + k * ml, 1);
                     p2 = L[i + (k - 2) * ml] * L[(k - 2) + k * ml];
@@ -400,13 +406,13 @@ __global__ __launch_bounds__(128) void k_getrf_exact_w2(MfhGetrfBatch B, MfhQueu
                 if (act) {
                     double s;
                     if (k >= 2) {
-                        s = sp + (p2 + L[i + (k - 1) * ml] * L[(k - 1) + k * ml]);
+                        s = sp + (p2 + xk1 * L[(k - 1) + k * ml]);
                     } else if (k == 1) {
-                        s = L[i] * L[ml];  // Internals.Dot of length 1: x0 * y0
+                        s = xk1 * L[ml];  // Internals.Dot of length 1: x0 * y0
                     } else {
                         s = 0.0;
                     }
-                    w = L[i + k * ml] - s;
+                    w = aik - s;
                 }
                 // the largest key, then the lowest lane holding it (the
                 // lowest row: the strict '>' scan's choice)
