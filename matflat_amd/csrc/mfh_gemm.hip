@@ -345,7 +345,11 @@ __device__ inline void zgemm_tile(Cplx (*As)[TM + 1], Cplx (*Bs)[TN + 1], Cplx (
     }
 }
 
-__global__ __launch_bounds__(GT) void k_zgemm_exact(MfhGemmBatch B, int count, GemmWs ws)
+// Held to two wavefronts per SIMD (256 registers, 72 bytes spilled): left
+// alone the compiler took 256 VGPRs and AGPRs beyond them, one wavefront per
+// SIMD (200,000 products: 64.6 -> 43.0 ms).  Three (168 registers) spills
+// 420 bytes per lane.
+__global__ __launch_bounds__(GT, 2) void k_zgemm_exact(MfhGemmBatch B, int count, GemmWs ws)
 {
     __shared__ Cplx As[KC][TM + 1];
     __shared__ Cplx Bs[KC][TN + 1];
@@ -679,8 +683,12 @@ __device__ __forceinline__ void ztile(const MfhGemmBatch& B, int p, int i0, int 
 // workload with m, n, k ~ U{8..128}, at twice the registers.
 constexpr int FAST_TS = 32;
 
+// The complex instantiation is held to at least three wavefronts per SIMD:
+// left alone it took 108 VGPRs and 64 AGPRs (two per SIMD); bounded it fits
+// 128 registers with the same 24-byte spill and runs four (25.5 -> 24.4 ms
+// for 200,000 products).
 template <bool CPLX>
-__global__ __launch_bounds__(64 * FW) void k_gemm_mfma(MfhGemmBatch B, int count, GemmWs ws)
+__global__ __launch_bounds__(64 * FW, (CPLX ? 3 : 1)) void k_gemm_mfma(MfhGemmBatch B, int count, GemmWs ws)
 {
     constexpr int TS = FAST_TS;
     const int lane = threadIdx.x & 63;
