@@ -274,13 +274,22 @@ __device__ __forceinline__ double copy_in(const double* A, int lda, T* W, int ld
     for (int r0 = 0; r0 < n; r0 += 128) {
         const int r = r0 + rr;
         double rowsum = 0.0;
-        for (int c0 = 0; c0 < n; c0 += 32) {
-            double v[8];
+        // the next 32 columns' loads go out before this tile is written
+        // through LDS, so 16 loads per thread are in flight
+        auto load8 = [&](double (&v)[8], int c0) {
 #pragma unroll
             for (int h = 0; h < 8; ++h) {
                 const int c = c0 + cq + 4 * h;
                 v[h] = (r < n && c < n) ? gmem(A)[r + (size_t)c * lda] : 0.0;
             }
+        };
+        double vn[8];
+        load8(vn, 0);
+        for (int c0 = 0; c0 < n; c0 += 32) {
+            double v[8];
+#pragma unroll
+            for (int h = 0; h < 8; ++h) v[h] = vn[h];
+            if (c0 + 32 < n) load8(vn, c0 + 32);
 #pragma unroll
             for (int h = 0; h < 8; ++h) {
                 rowsum += __builtin_fabs(v[h]);
