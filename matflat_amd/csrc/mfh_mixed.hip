@@ -98,7 +98,7 @@ struct GesvArgs {
 
 // LDS layout (bytes), for row stride L = ldw:
 //   ut    [L][NB] T       U12 transposed (k contiguous); also the copy tile
-//   l11   [NB][NB] T
+//   l11   [NB][NB] T      L11 transposed (column j contiguous)
 //   cand  [2][NWAVE][NB] T, ckey/cidx [2][NWAVE]: pivot search
 //   dst   [L] int         final row of each panel row
 //   xv, rv [L] double     solution, residual / right-hand side
@@ -512,7 +512,8 @@ __device__ __forceinline__ void compose_rows(int k0, int nb, const Lds<T>& L)
 
 // Factor columns k0 .. k0+nb-1 of rows k0 .. n-1 (one row per thread, in
 // registers), compose the panel's row order (mapk/mapv, perm), write the
-// panel rows back to their final rows, and leave the unit-lower L11 in l11.
+// panel rows back to their final rows, and leave the unit-lower L11 in l11
+// (by columns: L11[i][j] at l11[j NB + i]).
 // Returns the first zero pivot (1-based) or 0.
 template <typename T>
 __device__ __forceinline__ int panel(T* W, int ldw, int n, int k0, int nb, const Lds<T>& L)
@@ -577,7 +578,7 @@ __device__ __forceinline__ int panel(T* W, int ldw, int n, int k0, int nb, const
         }
         if (d < k0 + nb) {
 #pragma unroll
-            for (int c = 0; c < NB; ++c) L.l11[(d - k0) * NB + c] = P[c];
+            for (int c = 0; c < NB; ++c) L.l11[c * NB + (d - k0)] = P[c];
         }
     }
     return zero;
@@ -638,12 +639,25 @@ __device__ __forceinline__ void trsm_u12(T* W, int ldw, int n, int k0, const Lds
         T u[NB];
 #pragma unroll
         for (int i = 0; i < NB; ++i) u[i] = W[(size_t)(k0 + i) * ldw + c];
-        // column-oriented substitution; the compiler fence (which takes the
-        // L11 address) keeps the loads of one column in flight at a time
+        // column-oriented substitution: column j of L11 is contiguous, read
+        // as 16-byte vectors from the one holding row j + 1 on (broadcast:
+        // every thread reads the same words); the compiler fence (which
+        // takes the L11 address) keeps the loads of one column in flight at
+        // a time
+        typedef T vec_t __attribute__((ext_vector_type(16 / sizeof(T))));
+        constexpr int VW = 16 / sizeof(T);
 #pragma unroll
         for (int j = 0; j < NB - 1; ++j) {
+            T lc[NB];
+            const vec_t* col = (const vec_t*)(L.l11 + j * NB);
 #pragma unroll
-            for (int i = j + 1; i < NB; ++i) u[i] -= L.l11[i * NB + j] * u[j];
+            for (int q = (j + 1) / VW; q < NB / VW; ++q) {
+                const vec_t v = col[q];
+#pragma unroll
+                for (int e = 0; e < VW; ++e) lc[q * VW + e] = v[e];
+            }
+#pragma unroll
+            for (int i = j + 1; i < NB; ++i) u[i] -= lc[i] * u[j];
             asm volatile("" ::"v"(L.l11) : "memory");
         }
 #pragma unroll
