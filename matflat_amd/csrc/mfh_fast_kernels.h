@@ -200,8 +200,7 @@ __device__ __forceinline__ void trailing(AccT<RB, NW, W>& acc, const double* pr,
 // and finite garbage only, and their rdg and yv entries are written as 0.
 // Returns the first failing row + 1 (or the fail passed in).
 template <int RB, int R, bool POSV>
-__device__ __forceinline__ int panel_factor(double* pb, double* rdg, double* yv, int p, int n, int lane,
-                                            int fail)
+__device__ __forceinline__ int panel_factor(double* pb, double* rdg, double* yv, int p, int n, int lane, int fail)
 {
     constexpr int NR = 8 * RB;
     const int nc = n - 8 * p < 8 ? n - 8 * p : 8;
@@ -453,8 +452,7 @@ __device__ __forceinline__ void fast_matrix(const MfhCholBatch& B, const FastMet
         const int offl = offl0 + 8 * p * lda, offu = offu0 + 8 * p;
         const bool jin = 8 * p < ncol;
         for_panel<RB>(p, [&](auto P) {
-            store_panel_p<RB, NW, W, decltype(P)::value>(A, lda, pr, offl,
-                                                         offu, nrow, jin, lr, lc);
+            store_panel_p<RB, NW, W, decltype(P)::value>(A, lda, pr, offl, offu, nrow, jin, lr, lc);
         });
         trailing<RB, NW, W>(acc, pb + PLD * lr, pb + PLD * lc, p);
     }
