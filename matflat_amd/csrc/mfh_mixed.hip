@@ -14,9 +14,9 @@
 //    FP32, 16 for FP64): the panel lives in registers (one row per thread),
 //    each column takes one block-wide argmax and one row exchange through
 //    LDS; the pivots are applied to the other columns by row swaps (W is
-//    row-major, so a swap moves two contiguous rows); U12 = L11^-1 A12 by
-//    column-parallel forward substitution, kept transposed in LDS; the
-//    trailing update A22 -= L21 U12 runs on MFMA:
+//    row-major, so a swap moves two contiguous rows), and U12 = L11^-1 A12
+//    by column-parallel forward substitution on the swapped values in
+//    registers, kept transposed in LDS; the trailing update A22 -= L21 U12 runs on MFMA:
 //      - FP64: v_mfma_f64_16x16x4f64, 16x16 output tiles, K = 16 in 4 steps;
 //      - FP32, n >= 256: v_mfma_f32_16x16x32_bf16 on a three-term BF16
 //        split of both operands (x = x1 + x2 + x3), six products summed
@@ -585,86 +585,66 @@ __device__ __forceinline__ int panel(T* W, int ldw, int n, int k0, int nb, const
 }
 
 // Apply the panel's row order (the map composed by compose_rows) to the
-// columns outside the panel: every thread moves its column through
-// registers, all loads first, then the stores, so the memory accesses are
-// independent.
+// columns outside the panel, and, when a trailing part exists, solve
+// U12 = L11^-1 A12 for the columns right of the panel on the way: every
+// thread moves its column through registers, all loads first (entries past
+// the map's end repeat its last entry, so the 2 NB loads are branch-free and
+// go out together); rows k0 .. k0+NB-1 of a column (the first NB entries of
+// the map) are A12's column after the swap, so the substitution runs on the
+// loaded values before they are stored, and U12 is also written, transposed,
+// to ut.  Only full panels have a trailing part (nb == NB there).
 template <typename T>
-__device__ __forceinline__ void swap_rows(T* W, int ldw, int n, int k0, int nb, const Lds<T>& L)
+__device__ __forceinline__ void swap_trsm(T* W, int ldw, int n, int k0, int nb, const Lds<T>& L)
 {
+    constexpr int NB = Prec<T>::NB;
+    constexpr int M = 2 * NB;  // compose_rows' map holds at most 2 nb rows
     const int t = threadIdx.x;
     __syncthreads();
     const int cnt = __builtin_amdgcn_readfirstlane(*L.mapn);
     const int c = t;  // n <= BT: one column per thread
-    if (c < n && (c < k0 || c >= k0 + nb)) {
-        T v[64];
-        if constexpr (sizeof(T) == 4) {
-            // Branch-free: entries past cnt repeat the last one (the same
-            // value loaded and stored again), so the 64 loads go out
-            // together.  (For FP64 the 64 values would not fit in
-            // registers beside the rest of the kernel.)
-#pragma unroll
-            for (int i = 0; i < 64; ++i) {
-                const int e = i < cnt ? i : cnt - 1;
-                v[i] = gmem(W)[(size_t)L.mapv[e] * ldw + c];
-            }
-#pragma unroll
-            for (int i = 0; i < 64; ++i) {
-                const int e = i < cnt ? i : cnt - 1;
-                gmem_w(W)[(size_t)L.mapk[e] * ldw + c] = v[i];
-            }
-        } else {
-#pragma unroll
-            for (int i = 0; i < 64; ++i)
-                if (i < cnt) v[i] = W[(size_t)L.mapv[i] * ldw + c];
-#pragma unroll
-            for (int i = 0; i < 64; ++i)
-                if (i < cnt && L.mapk[i] != L.mapv[i]) W[(size_t)L.mapk[i] * ldw + c] = v[i];
-        }
-    }
-    __syncthreads();
-}
-
-// U12 = L11^-1 A12 (columns k0+NB .. n-1), written to W and, transposed, to
-// ut.  Only full panels have a trailing part, so nb == NB here and every load
-// and store is unconditional (per-row conditions would split the code into
-// many blocks and defeat register allocation).
-template <typename T>
-__device__ __forceinline__ void trsm_u12(T* W, int ldw, int n, int k0, const Lds<T>& L)
-{
-    constexpr int NB = Prec<T>::NB;
     const int c0 = k0 + NB;
-    // n <= BT: one column per thread
-    const int c = c0 + (int)threadIdx.x;
-    if (c < n) {
-        T u[NB];
+    if (c < n && (c < k0 || c >= k0 + nb)) {
+        T v[M];
 #pragma unroll
-        for (int i = 0; i < NB; ++i) u[i] = W[(size_t)(k0 + i) * ldw + c];
-        // column-oriented substitution: column j of L11 is contiguous, read
-        // as 16-byte vectors from the one holding row j + 1 on (broadcast:
-        // every thread reads the same words); the compiler fence (which
-        // takes the L11 address) keeps the loads of one column in flight at
-        // a time
-        typedef T vec_t __attribute__((ext_vector_type(16 / sizeof(T))));
-        constexpr int VW = 16 / sizeof(T);
+        for (int i = 0; i < M; ++i) {
+            const int e = i < cnt ? i : cnt - 1;
+            v[i] = gmem(W)[(unsigned)(L.mapv[e] * ldw + c)];
+        }
+        // a wavefront holding a column right of the panel (the test is
+        // uniform); its lanes left of the panel run the same substitution
+        // with every multiplier zero, so their values stay as loaded
+        if (k0 + nb < n && wave_id() * 64 + 63 >= c0) {
+            const bool right = c >= c0;
+            // column-oriented substitution: column j of L11 is contiguous,
+            // read as 16-byte vectors from the one holding row j + 1 on
+            // (broadcast: every thread reads the same words); the compiler
+            // fence (which takes the L11 address) keeps the loads of one
+            // column in flight at a time
+            typedef T vec_t __attribute__((ext_vector_type(16 / sizeof(T))));
+            constexpr int VW = 16 / sizeof(T);
 #pragma unroll
-        for (int j = 0; j < NB - 1; ++j) {
-            T lc[NB];
-            const vec_t* col = (const vec_t*)(L.l11 + j * NB);
+            for (int j = 0; j < NB - 1; ++j) {
+                T lc[NB];
+                const vec_t* col = (const vec_t*)(L.l11 + j * NB);
 #pragma unroll
-            for (int q = (j + 1) / VW; q < NB / VW; ++q) {
-                const vec_t v = col[q];
+                for (int q = (j + 1) / VW; q < NB / VW; ++q) {
+                    const vec_t lv = col[q];
 #pragma unroll
-                for (int e = 0; e < VW; ++e) lc[q * VW + e] = v[e];
+                    for (int e = 0; e < VW; ++e) lc[q * VW + e] = lv[e];
+                }
+                const T m = right ? v[j] : (T)0;
+#pragma unroll
+                for (int i = j + 1; i < NB; ++i) v[i] -= lc[i] * m;
+                asm volatile("" ::"v"(L.l11) : "memory");
             }
+            if (right) {
 #pragma unroll
-            for (int i = j + 1; i < NB; ++i) u[i] -= lc[i] * u[j];
-            asm volatile("" ::"v"(L.l11) : "memory");
+                for (int i = 0; i < NB; ++i) L.ut[(size_t)(c - c0) * NB + i] = v[i];
+            }
         }
 #pragma unroll
-        for (int i = 0; i < NB; ++i) {
-            W[(size_t)(k0 + i) * ldw + c] = u[i];
-            L.ut[(size_t)(c - c0) * NB + i] = u[i];
-        }
+        for (int i = 0; i < M; ++i)
+            if (i < cnt) gmem_w(W)[(unsigned)(L.mapk[i] * ldw + c)] = v[i];
     }
     __syncthreads();
 }
@@ -1140,9 +1120,8 @@ __device__ __forceinline__ void gesv_factor(const GesvArgs& G, const Lds<T>& L, 
         const int nb = n - k0 < NB ? n - k0 : NB;
         const int z = panel<T>(W, ldw, n, k0, nb, L);
         if (zero == 0) zero = z;
-        swap_rows<T>(W, ldw, n, k0, nb, L);
+        swap_trsm<T>(W, ldw, n, k0, nb, L);
         if (k0 + nb < n) {
-            trsm_u12<T>(W, ldw, n, k0, L);
             if constexpr (MIXED) {
                 if (n >= 256) trailing_bf16x6(W, ldw, n, k0, L);
                 else trailing_f32_valu(W, ldw, n, k0, L);
