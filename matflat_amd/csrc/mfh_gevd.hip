@@ -45,6 +45,8 @@ struct GevdBatch {
 };
 
 constexpr int GW = 64;  // one wavefront per matrix
+// the QR sweep carries at most two rows of U per lane
+static_assert(MFH_SMALL_MAX_N <= 2 * GW, "rows of U per lane");
 
 // Math.Max(double, double) of .NET: NaN when either argument is NaN.
 __device__ __forceinline__ double net_max(double a, double b)
@@ -362,22 +364,56 @@ __global__ __launch_bounds__(GW) void k_gevd_exact(GevdBatch G, int count, int m
                 }
                 f = (skk + sp) * (skk - sp) + shift;
                 double g = skk * ekk;
+                // The sweep's chain runs in registers: s(j) and e(j) are
+                // carried from the step before (which last wrote them), and
+                // s(j+1), e(j+1), not yet touched by this sweep, are read at
+                // the top of the step, off the chain.  Every value and every
+                // store is the reference's (:1134-1177), in its order.
+                // The rotations of U's columns (j, j+1) carry column j+1 of
+                // the lane's rows (it is column j of the next rotation), so a
+                // step reads only column j+1, also at its top (:1167-1176).
+                double sj = S[k], ej = E[k];
+                double ux[2], uy[2];
+#pragma unroll
+                for (int r = 0; r < 2; ++r) {
+                    const int i = lane + GW * r;
+                    ux[r] = i < n ? U[i + (size_t)k * ldu] : 0.0;
+                }
                 for (int j = k; j < p - 1; ++j) {
+                    const double sj1 = S[j + 1], ej1 = E[j + 1];
+#pragma unroll
+                    for (int r = 0; r < 2; ++r) {
+                        const int i = lane + GW * r;
+                        uy[r] = i < n ? U[i + (size_t)(j + 1) * ldu] : 0.0;
+                    }
                     drotg(f, g, cs, sn);
                     if (j != k) E[j - 1] = f;
-                    const double sj = S[j], ej = E[j];
                     f = cs * sj + sn * ej;
-                    E[j] = cs * ej - sn * sj;
-                    g = sn * S[j + 1];
-                    S[j + 1] = S[j + 1] * cs;
+                    const double ejr = cs * ej - sn * sj;
+                    g = sn * sj1;
+                    const double sj1r = sj1 * cs;
                     drotg(f, g, cs, sn);
                     S[j] = f;
-                    const double ej2 = E[j], sj1 = S[j + 1];
-                    f = cs * ej2 + sn * sj1;
-                    S[j + 1] = -(sn * ej2) + cs * sj1;
-                    g = sn * E[j + 1];
-                    E[j + 1] = E[j + 1] * cs;
-                    rot_rows(U, ldu, n, lane, j, j + 1, cs, sn);
+                    f = cs * ejr + sn * sj1r;
+                    const double sj1s = -(sn * ejr) + cs * sj1r;
+                    g = sn * ej1;
+                    const double ej1r = ej1 * cs;
+                    E[j] = ejr;
+                    S[j + 1] = sj1s;
+                    E[j + 1] = ej1r;
+#pragma unroll
+                    for (int r = 0; r < 2; ++r) {
+                        const int i = lane + GW * r;
+                        if (i < n) U[i + (size_t)j * ldu] = cs * ux[r] + sn * uy[r];
+                        ux[r] = cs * uy[r] - sn * ux[r];
+                    }
+                    sj = sj1s;
+                    ej = ej1r;
+                }
+#pragma unroll
+                for (int r = 0; r < 2; ++r) {
+                    const int i = lane + GW * r;
+                    if (i < n) U[i + (size_t)(p - 1) * ldu] = ux[r];
                 }
                 E[p - 2] = f;
                 iter++;
