@@ -27,6 +27,7 @@
 // the reference's, and the results are bit-identical to MatFlat's.
 #include "mfh_device.h"
 #include "mfh_internal.h"
+#include "mfh_order.h"
 
 #pragma clang fp contract(off)
 
@@ -42,6 +43,8 @@ struct GevdBatch {
     int* info;
     int* n_chol;     // n where the matrix's arguments are valid, else 0 (the Cholesky skips it)
     int* info_chol;  // the Cholesky's info
+    const int* order;  // problem ids, largest order first
+    int* counter;      // work counter over order (zero before the launch)
 };
 
 constexpr int GW = 64;  // one wavefront per matrix
@@ -149,18 +152,32 @@ __global__ __launch_bounds__(GW) void k_gevd_exact(GevdBatch G, int count, int m
     double* E = S + max_n;
     const int lane = threadIdx.x;
 
-    for (int mi = blockIdx.x; mi < count; mi += gridDim.x) {
-        if (G.info[mi] != 0) continue;  // invalid arguments: nothing is written
-        const int fail = G.info_chol[mi];
+    // Problems are taken largest first through the work counter: with orders
+    // mixed from 8 to 128 and the time of one growing as n^3, a fixed
+    // stride over the batch left some workgroups with far more work than
+    // others at the end.  Every test that ends or skips an iteration is on
+    // a wave-uniform value (lane 0's slot, values read through
+    // readfirstlane), and no path that skips an iteration holds a per-lane
+    // branch: either makes the loop divergent to the compiler, which may
+    // then re-run the top of the loop for part of the wavefront, where the
+    // slot of the lanes other than lane 0 reads 0.
+    for (;;) {
+        int k = 0;
+        if (lane == 0) k = atomicAdd(G.counter, 1);
+        k = __builtin_amdgcn_readlane(k, 0);
+        if (k >= count) break;
+        const int mi = __builtin_amdgcn_readfirstlane(G.order[k]);
+        if (__builtin_amdgcn_readfirstlane(G.info[mi]) != 0) continue;  // invalid arguments: nothing is written
+        const int fail = __builtin_amdgcn_readfirstlane(G.info_chol[mi]);
         if (fail != 0) {  // B is not positive definite: b holds the Cholesky's partial writes
-            if (lane == 0) G.info[mi] = fail;
+            G.info[mi] = fail;  // every lane stores the same value
             continue;
         }
-        const int n = G.n[mi];
+        const int n = __builtin_amdgcn_readfirstlane(G.n[mi]);
         double* A = G.a[mi];
-        const int lda = G.lda[mi];
+        const int lda = __builtin_amdgcn_readfirstlane(G.lda[mi]);
         const double* L = G.b[mi];
-        const int ldb = G.ldb[mi];
+        const int ldb = __builtin_amdgcn_readfirstlane(G.ldb[mi]);
         double* U = ULDS ? sU : A;
         const int ldu = ULDS ? ldc : lda;
         __syncthreads();  // the previous matrix's last reads of LDS are done
@@ -459,7 +476,7 @@ __global__ __launch_bounds__(GW) void k_gevd_exact(GevdBatch G, int count, int m
                 A[i + (size_t)j * lda] = U[i + (size_t)j * ldu];
             }
         }
-        if (lane == 0) G.info[mi] = ok ? 0 : n + 1;
+        G.info[mi] = ok ? 0 : n + 1;  // every lane stores the same value
     }
 }
 
@@ -488,17 +505,29 @@ int launch_gevd(mfh_handle* h, int count, const GevdBatch& G, int max_n, int ldc
 
 }  // namespace
 
-size_t mfh_gevd_aux_bytes(int count) { return 2 * (((size_t)count * sizeof(int) + 255) & ~(size_t)255); }
+// The handle's second buffer holds n_chol, info_chol and the order (count
+// ints each), then the work counter, the order histogram and its cursors.
+constexpr int kGevdHistOff = 64, kGevdCursorOff = 128, kGevdHeaderInts = 256;
+static_assert(MfhOrder<MFH_SMALL_MAX_N>::keys <= kGevdCursorOff - kGevdHistOff &&
+                  kGevdCursorOff + MfhOrder<MFH_SMALL_MAX_N>::keys <= kGevdHeaderInts,
+              "order histogram");
+static size_t gevd_ints(int count) { return ((size_t)count * sizeof(int) + 255) & ~(size_t)255; }
+size_t mfh_gevd_aux_bytes(int count) { return 3 * gevd_ints(count) + kGevdHeaderInts * sizeof(int); }
 
 int mfh_launch_gevd(mfh_handle* h, int count, const int* n, double* const* a, const int* lda, double* const* b,
                     const int* ldb, double* const* w, int* info, int max_n)
 {
     int st = mfh_aux_reserve(h, mfh_gevd_aux_bytes(count));
     if (st) return st;
-    const size_t half = mfh_gevd_aux_bytes(count) / 2;
-    GevdBatch G{n, a, lda, b, ldb, w, info, (int*)h->aux, (int*)((char*)h->aux + half)};
+    char* aux = (char*)h->aux;
+    const size_t part = gevd_ints(count);
+    int* header = (int*)(aux + 3 * part);
+    GevdBatch G{n, a, lda, b, ldb, w, info, (int*)aux, (int*)(aux + part), (const int*)(aux + 2 * part), header};
     hipLaunchKernelGGL(k_gevd_check, dim3((unsigned)((count + GW - 1) / GW)), dim3(GW), 0, h->stream, G, count,
                        max_n);
+    hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(256), 0, h->stream, header, kGevdHeaderInts);
+    mfh_launch_order<MFH_SMALL_MAX_N>(h->stream, n, count, header + kGevdHistOff, header + kGevdCursorOff,
+                                      (int*)(aux + 2 * part));
     MFH_HIP_CHECK(hipGetLastError());
     // B = L L^T with the MatFlat-order Cholesky whatever the handle's math
     // mode; matrices with invalid arguments have order 0 there and are skipped.

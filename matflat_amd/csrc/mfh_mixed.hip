@@ -33,6 +33,7 @@
 //    the original FP64 A.
 #include "mfh_device.h"
 #include "mfh_internal.h"
+#include "mfh_order.h"
 
 // Neither solver is bit-identical to MatFlat: multiply-adds may fuse.
 #pragma clang fp contract(fast)
@@ -1308,65 +1309,6 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(MFH_STEP_WAV
     }
 }
 
-// Largest-first order of the batch for the factorization and solve
-// launches: a counting sort of the matrix ids by n / 8 (descending), in
-// three small kernels (histogram, offsets, scatter) with LDS-aggregated
-// atomics.  Invalid orders sort last; the kernels report them.
-constexpr int kOrderKeys = MFH_BLOCKED_MAX_N / 8 + 1;
-__device__ __forceinline__ int order_key(const int* n, int i)
-{
-    const int v = n[i];
-    return (v >= 1 && v <= MFH_BLOCKED_MAX_N) ? v >> 3 : 0;
-}
-
-__global__ __launch_bounds__(256) void k_order_hist(const int* n, int count, int* hist)
-{
-    __shared__ int hl[kOrderKeys];
-    for (int k = threadIdx.x; k < kOrderKeys; k += blockDim.x) hl[k] = 0;
-    __syncthreads();
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < count) atomicAdd(&hl[order_key(n, i)], 1);
-    __syncthreads();
-    for (int k = threadIdx.x; k < kOrderKeys; k += blockDim.x)
-        if (hl[k]) atomicAdd(&hist[k], hl[k]);
-}
-
-__global__ void k_order_offsets(const int* hist, int* cursor)
-{
-    if (threadIdx.x == 0) {
-        int run = 0;
-        for (int k = kOrderKeys - 1; k >= 0; --k) {
-            cursor[k] = run;
-            run += hist[k];
-        }
-    }
-}
-
-__global__ __launch_bounds__(256) void k_order_scatter(const int* n, int count, int* cursor, int* order)
-{
-    __shared__ int hl[kOrderKeys];
-    __shared__ int base[kOrderKeys];
-    for (int k = threadIdx.x; k < kOrderKeys; k += blockDim.x) hl[k] = 0;
-    __syncthreads();
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int key = i < count ? order_key(n, i) : -1;
-    int pos = -1;
-    if (key >= 0) pos = atomicAdd(&hl[key], 1);
-    __syncthreads();
-    for (int k = threadIdx.x; k < kOrderKeys; k += blockDim.x)
-        if (hl[k]) base[k] = atomicAdd(&cursor[k], hl[k]);
-    __syncthreads();
-    if (key >= 0) order[base[key] + pos] = i;
-}
-
-// Zeroes the work counters, the per-step active counts and the order
-// histogram (a kernel, not a memset: the launch sequence may be captured
-// into a graph).
-__global__ void k_zero_counter(int* c, int n)
-{
-    for (int i = threadIdx.x; i < n; i += blockDim.x) c[i] = 0;
-}
-
 // Factor-only mode: writes the FP64 factors from the row-major workspace W
 // back to the caller's column-major matrix (through 32 x 33 LDS tiles, both
 // sides coalesced), the row order to piv and the first zero pivot to info.
@@ -1417,16 +1359,12 @@ int launch_gesv(mfh_handle* h, GesvArgs G)
     if (per_cu < 1) per_cu = 1;
     long grid = (long)per_cu * h->num_cus;
     if (grid > G.count) grid = G.count;
-    hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(256), 0, h->stream, G.counter, kGesvHeaderInts);
-    {
-        int* hist = G.counter + kGesvHistOff;
-        int* cursor = G.counter + kGesvCursorOff;
-        const unsigned blocks = (unsigned)((G.count + 255) / 256);
-        hipLaunchKernelGGL(k_order_hist, dim3(blocks), dim3(256), 0, h->stream, G.n, G.count, hist);
-        hipLaunchKernelGGL(k_order_offsets, dim3(1), dim3(64), 0, h->stream, hist, cursor);
-        hipLaunchKernelGGL(k_order_scatter, dim3(blocks), dim3(256), 0, h->stream, G.n, G.count, cursor, G.order);
-        MFH_HIP_CHECK(hipGetLastError());
-    }
+    hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(256), 0, h->stream, G.counter, kGesvHeaderInts);
+    // largest first: a counting sort of the ids by n / 8
+    static_assert(MfhOrder<MFH_BLOCKED_MAX_N>::keys <= kGesvCursorOff - kGesvHistOff, "order histogram");
+    mfh_launch_order<MFH_BLOCKED_MAX_N>(h->stream, G.n, G.count, G.counter + kGesvHistOff,
+                                        G.counter + kGesvCursorOff, G.order);
+    MFH_HIP_CHECK(hipGetLastError());
     int st = mfh_timing_begin(h, MIXED ? MFH_KERNEL_GESV_MIXED : MFH_KERNEL_GESV_F64, G.max_n);
     if (st) return st;
     hipLaunchKernelGGL((k_gesv<T, MIXED>), dim3((unsigned)grid), dim3(BT), lds, h->stream, G);
