@@ -14,7 +14,7 @@
 //
 // The Cholesky is the exact Cholesky kernel of this library, launched first.
 // Everything after it runs in one wavefront per matrix (k_gevd_exact): C
-// lives in LDS (and U too when both fit), and
+// lives in LDS, U is built in a (as the reference builds it), and
 //   - every triangular solve, Householder dot and column update is one lane's
 //     own sequential loop over a row or column, in the reference's order;
 //   - the norms, the Householder scalars and the whole singular-value
@@ -538,6 +538,14 @@ int mfh_launch_gevd(mfh_handle* h, int count, const int* n, double* const* a, co
     h->math_mode = mode;
     if (st) return st;
     const int ldc = max_n | 1;  // odd: lanes walking rows of C hit distinct LDS banks
-    if (lds_bytes(max_n, ldc, true) <= 160 * 1024) return launch_gevd<true>(h, count, G, max_n, ldc);
+    // U is built in a, as the reference builds it, and only C is in LDS:
+    // 33.8 KB per problem at max_n = 64, four problems per CU instead of two
+    // with U in LDS too (10,000 pairs, n ~ U{8..64}: 19.6 against 29.7 ms;
+    // the sweep's rotations read one column of U per step, off the chain).
+    // MFH_GEVD_U_IN_LDS=1 builds the LDS variant for comparison.
+#ifndef MFH_GEVD_U_IN_LDS
+#define MFH_GEVD_U_IN_LDS 0
+#endif
+    if (MFH_GEVD_U_IN_LDS && lds_bytes(max_n, ldc, true) <= 160 * 1024) return launch_gevd<true>(h, count, G, max_n, ldc);
     return launch_gevd<false>(h, count, G, max_n, ldc);
 }

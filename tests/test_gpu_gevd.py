@@ -62,8 +62,8 @@ def check(cases, a_bufs, b_bufs, max_n, math="exact"):
 @pytest.mark.parametrize("small", [True, False])
 def test_golden_gevd(small):
     """The committed fixture (GevdTests.cs:94-105 shapes and larger ones):
-    small runs the orders up to 100 with U in LDS; otherwise all orders with
-    max_n 128 (U in a)."""
+    small runs the orders up to 100 (max_n 100); otherwise all orders with
+    max_n 128."""
     hip = hipmod()
     g = np.load(os.path.join(GOLDEN, "gevd.npz"), allow_pickle=False)
     meta = g["gevd_meta"]
